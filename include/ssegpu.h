@@ -1,0 +1,213 @@
+/*
+ * ssegpu.h -- C ABI of the MI355X SSE sweep library (libssegpu.so).
+ *
+ * The library runs the stochastic series expansion (SSE) sweep of
+ * StochasticSeriesExpansion.jl for a batch of independent walkers (Markov
+ * chains) on one gfx950 GPU: diagonal update, vertex-list construction and
+ * abstract-loop (worm) update, plus the operator-string measurement.
+ *
+ * Everything crosses the boundary as plain pointers and sizes.  The caller
+ * owns every input array only for the duration of a call; the library copies
+ * what it keeps.  Errors are int status codes (SSE_OK == 0) plus a
+ * thread-local message from sse_last_error().  One batch handle is used by one
+ * host thread at a time; each batch owns its own HIP stream.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * StochasticSeriesExpansion.jl v0.1.2 source tree):
+ *   sse_batch_create      <- MC(params)                  src/sse.jl:26-45
+ *                            + generate_sse_data / VertexData (the model and
+ *                            its tables arrive already flattened, see below)
+ *   sse_init              <- Carlo.init!(mc, ctx, params)  src/sse.jl:47-60
+ *   sse_sweep             <- Carlo.sweep!(mc, ctx)         src/sse.jl:62-68
+ *   sse_measure           <- Carlo.measure!(mc, ctx)       src/sse.jl:70-87,321-376
+ *   sse_get_walker        <- Carlo.write_checkpoint        src/sse.jl:89-97
+ *   sse_set_walker        <- Carlo.read_checkpoint         src/sse.jl:99-107
+ */
+#ifndef SSEGPU_H
+#define SSEGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSEGPU_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum sse_status {
+    SSE_OK = 0,
+    SSE_ERR_INVALID = 1,   /* invalid argument or unsupported model shape      */
+    SSE_ERR_HIP = 2,       /* HIP runtime error                               */
+    SSE_ERR_OOM = 3,       /* device allocation failed                        */
+    SSE_ERR_SCATTER = 4,   /* scatter fell through (r >= last cumprob) or hit
+                              an invalid transition; reference behaviour there
+                              is undefined (src/vertex_data.jl:124)           */
+    SSE_ERR_NO_DEVICE = 5, /* no usable gfx950 device                         */
+    SSE_ERR_STATE = 6      /* call out of order (e.g. sweep before init)      */
+};
+
+/* ---- operator encoding ---------------------------------------------------
+ * Device operator code (uint32):
+ *   0                                   identity
+ *   1 | vcode << 1 | bond << 11         bond operator
+ * vcode = diag | vidx << 1 (vidx 0-based < 512), bond 0-based < 2^21.
+ * The reference encodes OperCode as UInt64 1 | vcode' << 1 | bond' << 26 with
+ * vcode' = diag | vidx' << 1 and 1-based vidx', bond' (src/opercode.jl:18-47);
+ * sse_opcode_to_reference / sse_opcode_from_reference map losslessly.      */
+#define SSE_OP_VCODE_BITS 10
+#define SSE_OP_BOND_SHIFT 11
+#define SSE_VCODE_INVALID (-1)
+
+/* ---- VertexData of one bond type (src/vertex_data.jl:13-28) -------------
+ * Only two-site bonds (leg_count 4) are supported by the device kernels.
+ * Legs are 0-based: 0,1 = lower legs of bond sites 0,1 (state before the
+ * operator), 2,3 = upper legs (state after).                               */
+typedef struct sse_vertex_data_desc {
+    int32_t dims[2];                 /* local dimension of the two bond sites  */
+    double energy_offset;            /* src/vertex_data.jl:129-135             */
+    int32_t n_vertices;
+    const double *weights;           /* [n_vertices] |w|                       */
+    const int8_t *signs;             /* [n_vertices] sign of w                 */
+    const uint8_t *leg_states;       /* [n_vertices][4], 1-based local states  */
+    const int32_t *diagonal_vertices;/* [dims0*dims1] vcode or -1; index
+                                        (s0-1) + dims0*(s1-1) (join_idx)       */
+    int32_t max_worm_count;
+    const int32_t *transition_offset;/* [n_vertices][max_worm_count][4]: first
+                                        exit index or -1 (invalid Transition)  */
+    const int32_t *transition_count; /* same shape: number of exits            */
+    int32_t n_exits;
+    const double *exit_cumprob;      /* [n_exits] cumulative probabilities     */
+    const int32_t *exit_target;      /* [n_exits] target vcode                 */
+    const uint8_t *exit_leg;         /* [n_exits] 0-based exit leg             */
+    const uint8_t *exit_worm;        /* [n_exits] 1-based exit worm function   */
+} sse_vertex_data_desc;
+
+/* ---- SSEData + estimator tables (src/sse_data.jl:15-22) ------------------ */
+typedef struct sse_model_desc {
+    int32_t n_sites;
+    const uint8_t *site_dims;        /* [n_sites]                              */
+    int32_t n_bonds;
+    const int32_t *bond_type;        /* [n_bonds] 0-based VertexData index     */
+    const int32_t *bond_sites;       /* [n_bonds][2] 0-based site indices      */
+    int32_t n_types;
+    const sse_vertex_data_desc *types;
+    double energy_offset;            /* sum over bonds of type offsets         */
+    double normalization_site_count; /* normalization_site_count(model)        */
+    /* MagnetizationEstimator family (magnetization_estimator.jl:96-230):
+     * estimator e contributes coef[e][site] * mval[e][site][state-1].        */
+    int32_t n_estimators;
+    int32_t est_max_dim;
+    const double *est_coef;          /* [n_estimators][n_sites]                */
+    const double *est_mval;          /* [n_estimators][n_sites][est_max_dim]   */
+} sse_model_desc;
+
+/* ---- per-walker (per Carlo task) parameters (src/sse.jl:26-60) ----------- */
+typedef struct sse_walker_desc {
+    int32_t model;                   /* index into the models array            */
+    int32_t diagonal_warmup_sweeps;  /* default 5                              */
+    double T;                        /* temperature                            */
+    uint64_t seed;                   /* Philox4x32-10 key of this walker       */
+    int64_t init_opstring_cutoff;    /* < 0: round(n_sites*T), ties to even    */
+    double init_num_worms;           /* default 5                              */
+    double num_worms_attenuation_factor; /* default 0.01                       */
+    double target_worm_length_fraction;  /* default 2.0                        */
+    int64_t thermalization;          /* Carlo thermalization sweeps            */
+} sse_walker_desc;
+
+/* ---- walker scalars for checkpoints and validation ----------------------- */
+typedef struct sse_walker_scalars {
+    int64_t cutoff;                  /* L = length(operators)                  */
+    int64_t num_operators;           /* n                                      */
+    double avg_worm_length;
+    double num_worms;
+    uint64_t rng_counter;            /* next 64-bit word of the Philox stream  */
+    int64_t sweeps;                  /* sweeps done (Carlo ctx.sweeps)         */
+    int64_t last_scatter_calls;      /* W of the last sweep                    */
+    double last_total_worm_length;   /* total_worm_length of the last sweep    */
+    double last_worm_length_fraction;/* WormLengthFraction if measured, else NaN */
+} sse_walker_scalars;
+
+/* Number of doubles sse_measure writes per walker: 5 + 5*n_estimators.
+ * Order: Sign, OperatorCount, SignOperatorCount, SignOperatorCount2,
+ * SignEnergy, then per estimator SignMag, SignAbsMag, SignMag2, SignMag4,
+ * SignMagChi (src/sse.jl:73-82, magnetization_estimator.jl:218-227).      */
+
+typedef struct sse_batch sse_batch;
+
+/* kernel strategies */
+enum sse_mode {
+    SSE_MODE_WAVE = 0,    /* one wavefront per walker: parallel diagonal update
+                             over 64-slot chunks, fused vertex-list linking,
+                             LDS-resident worm records (default)            */
+    SSE_MODE_LANE = 1     /* one lane per walker, reference order slot by slot
+                             (the plain form)                               */
+};
+
+/* Create a batch of n_walkers walkers on the current HIP device.  models and
+ * walkers are copied.  Replaces MC(params) (src/sse.jl:26-45).            */
+int sse_batch_create(const sse_model_desc *models, int32_t n_models,
+                     const sse_walker_desc *walkers, int32_t n_walkers,
+                     int32_t mode, sse_batch **out);
+int sse_batch_destroy(sse_batch *b);
+
+/* Carlo.init! for every walker: random spins (one int draw per site),
+ * identity opstring of the initial cutoff, diagonal warm-up sweeps.       */
+int sse_init(sse_batch *b);
+
+/* n_sweeps x Carlo.sweep! for every walker (diagonal update, vertex list,
+ * worm update incl. the worm-number controller while not thermalized).
+ * Carlo's counter rule: a walker is thermalized during a sweep when
+ * sweeps_done > thermalization; sweeps_done is incremented after it.     */
+int sse_sweep(sse_batch *b, int32_t n_sweeps);
+
+/* Carlo.measure! for every walker.  out: [n_walkers][5+5*n_estimators] of
+ * the walker's model (host pointer), or NULL to keep results on device.   */
+int sse_measure(sse_batch *b, double *out);
+int32_t sse_measure_width(const sse_batch *b, int32_t walker);
+
+/* Fused sweep+measure loop: n_sweeps sweeps, each followed by a measure once
+ * the walker is thermalized; the per-walker observables are accumulated on
+ * device into sums and sums of squares ([n_walkers][width] each).  The
+ * sequence is captured into a HIP graph and replayed.                      */
+int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure);
+int sse_read_accumulators(sse_batch *b, double *sums, double *sumsq,
+                          int64_t *counts);
+int sse_reset_accumulators(sse_batch *b);
+
+/* Walker state (checkpoint / validation).  ops: [cutoff] device operator
+ * codes or NULL; spins: [n_sites] 1-based states or NULL.                   */
+int sse_get_walker(sse_batch *b, int32_t walker, sse_walker_scalars *s,
+                   uint32_t *ops, int64_t ops_capacity, uint8_t *spins);
+int sse_set_walker(sse_batch *b, int32_t walker, const sse_walker_scalars *s,
+                   const uint32_t *ops, const uint8_t *spins);
+
+/* Validation: run only some phases of a sweep.  phases is a bit mask of
+ * 1 = diagonal update, 2 = vertex list, 4 = worm update.  Used to compare
+ * intermediate results with a CPU restatement.                            */
+int sse_sweep_phases(sse_batch *b, int32_t phases);
+/* Vertex list of walker w after the last vertex-list phase:
+ * links [4*cutoff] linked leg ids (4p+leg) or -1, first/last [n_sites]. */
+int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links,
+                        int64_t links_capacity, int32_t *first, int32_t *last);
+
+/* Per-phase device time of the last sse_sweep/sse_run call in ms
+ * (diagonal+vertex list, worm, measure), measured with HIP events on the
+ * batch stream.                                                            */
+int sse_phase_times(const sse_batch *b, double *ms3);
+/* Sum over walkers of cutoff L, operators n and scatter calls W since the
+ * last reset (counters taken from the chains themselves).                  */
+int sse_work_counters(sse_batch *b, int64_t *L_sum, int64_t *n_sum,
+                      int64_t *W_sum, int64_t *walker_sweeps);
+
+/* Encoding helpers (src/opercode.jl:18-71). */
+uint64_t sse_opcode_to_reference(uint32_t op);
+uint32_t sse_opcode_from_reference(uint64_t op);
+
+const char *sse_last_error(void);
+int32_t sse_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SSEGPU_H */

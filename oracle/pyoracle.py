@@ -1,0 +1,182 @@
+"""ctypes binding of oracle/libsseoracle.so (test infrastructure only).
+
+The oracle is the checker: tests compare the GPU path against it, smoke()
+runs one tiny case against it, bench.py times it as the CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libsseoracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib(capi):
+    """Load the oracle library; `capi` supplies the shared data-format structs."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    L = C.CDLL(LIB)
+    vp = C.c_void_p
+    L.so_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                   C.POINTER(C.c_uint32)]
+    L.so_philox_word.argtypes = [C.c_uint64, C.c_uint64]
+    L.so_philox_word.restype = C.c_uint64
+    L.so_word_to_float.argtypes = [C.c_uint64]
+    L.so_word_to_float.restype = C.c_double
+    L.so_range.argtypes = [C.c_uint64, C.POINTER(C.c_uint64), C.c_uint64]
+    L.so_range.restype = C.c_uint64
+    L.so_expm1.argtypes = [C.c_double]
+    L.so_expm1.restype = C.c_double
+    L.so_tanh.argtypes = [C.c_double]
+    L.so_tanh.restype = C.c_double
+    L.so_walker_create.argtypes = [C.POINTER(capi.ModelDesc), C.POINTER(capi.WalkerDesc)]
+    L.so_walker_create.restype = vp
+    L.so_walker_destroy.argtypes = [vp]
+    L.so_init.argtypes = [vp]
+    L.so_sweep.argtypes = [vp]
+    L.so_sweep_phases.argtypes = [vp, C.c_int]
+    L.so_measure.argtypes = [vp, C.POINTER(C.c_double)]
+    L.so_measure_width.argtypes = [vp]
+    L.so_get_scalars.argtypes = [vp, C.POINTER(capi.WalkerScalars)]
+    L.so_get_ops.argtypes = [vp, C.POINTER(C.c_uint32), C.c_int64]
+    L.so_get_ops.restype = C.c_int64
+    L.so_get_spins.argtypes = [vp, C.POINTER(C.c_uint8)]
+    L.so_set_state.argtypes = [vp, C.POINTER(capi.WalkerScalars), C.POINTER(C.c_uint32),
+                               C.POINTER(C.c_uint8)]
+    L.so_get_vertex_list.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int64)]
+    L.so_make_vertex_list.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
+                                      C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int64)]
+    L.so_run_timed.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    L.so_run_timed.restype = C.c_double
+    L.so_run_series.argtypes = [vp, C.c_int64, C.POINTER(C.c_double)]
+    L.so_run_series.restype = C.c_int64
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class OracleWalker:
+    """One walker of the CPU restatement (reference order of operations)."""
+
+    def __init__(self, capi, flat, walker):
+        self.capi = capi
+        self.L = lib(capi)
+        self.descs = capi.ModelDescs([flat])
+        self.wd = walker
+        self.flat = flat
+        self.h = self.L.so_walker_create(self.descs.array, C.byref(self.wd))
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.so_walker_destroy(self.h)
+        except Exception:
+            pass
+
+    def init(self):
+        self.L.so_init(self.h)
+
+    def sweep(self, n=1):
+        for _ in range(n):
+            err = self.L.so_sweep(self.h)
+            if err:
+                raise RuntimeError(f"oracle sweep error {err}")
+
+    def sweep_phases(self, phases):
+        return self.L.so_sweep_phases(self.h, phases)
+
+    def measure(self):
+        out = np.zeros(self.flat.measure_width)
+        self.L.so_measure(self.h, _p(out, C.c_double))
+        return out
+
+    def scalars(self):
+        s = self.capi.WalkerScalars()
+        self.L.so_get_scalars(self.h, C.byref(s))
+        return s.as_dict()
+
+    def ops(self):
+        s = self.scalars()
+        out = np.zeros(max(1, s["cutoff"]), dtype=np.uint32)
+        self.L.so_get_ops(self.h, _p(out, C.c_uint32), len(out))
+        return out[:s["cutoff"]]
+
+    def spins(self):
+        out = np.zeros(self.flat.n_sites, dtype=np.uint8)
+        self.L.so_get_spins(self.h, _p(out, C.c_uint8))
+        return out
+
+    def state(self):
+        return self.scalars(), self.ops(), self.spins()
+
+    def set_state(self, scalars, ops, spins):
+        s = self.capi.WalkerScalars()
+        for k, _ in self.capi.WalkerScalars._fields_:
+            if k in scalars:
+                setattr(s, k, scalars[k])
+        ops = np.ascontiguousarray(ops, dtype=np.uint32)
+        spins = np.ascontiguousarray(spins, dtype=np.uint8)
+        s.cutoff = len(ops)
+        return self.L.so_set_state(self.h, C.byref(s), _p(ops, C.c_uint32), _p(spins, C.c_uint8))
+
+    def vertex_list(self):
+        L = self.scalars()["cutoff"]
+        links = np.zeros(max(1, 4 * L), dtype=np.int64)
+        first = np.zeros(self.flat.n_sites, dtype=np.int64)
+        last = np.zeros(self.flat.n_sites, dtype=np.int64)
+        self.L.so_get_vertex_list(self.h, _p(links, C.c_int64), _p(first, C.c_int64),
+                                  _p(last, C.c_int64))
+        return links[:4 * L], first, last
+
+    def run_series(self, n_sweeps):
+        out = np.zeros((n_sweeps, self.flat.measure_width))
+        rows = self.L.so_run_series(self.h, n_sweeps, _p(out, C.c_double))
+        if rows < 0:
+            raise RuntimeError("oracle sweep error")
+        return out[:rows]
+
+    def run_timed(self, n_sweeps, measure=True):
+        sums = np.zeros(self.flat.measure_width)
+        t = self.L.so_run_timed(self.h, n_sweeps, int(measure), _p(sums, C.c_double))
+        return t, sums
+
+
+def make_vertex_list(capi, n_sites, bond_sites, ops):
+    L_ = lib(capi)
+    ops = np.ascontiguousarray(ops, dtype=np.uint32)
+    bs = np.ascontiguousarray(bond_sites, dtype=np.int32)
+    L = len(ops)
+    links = np.zeros(max(1, 4 * L), dtype=np.int64)
+    first = np.zeros(n_sites, dtype=np.int64)
+    last = np.zeros(n_sites, dtype=np.int64)
+    L_.so_make_vertex_list(n_sites, _p(bs, C.c_int32), _p(ops, C.c_uint32), L,
+                           _p(links, C.c_int64), _p(first, C.c_int64), _p(last, C.c_int64))
+    return links[:4 * L], first, last
+
+
+def philox_block(capi, ctr, key):
+    L_ = lib(capi)
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    L_.so_philox4x32_10(c, k, o)
+    return tuple(o)

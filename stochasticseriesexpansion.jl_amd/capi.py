@@ -1,0 +1,336 @@
+"""ctypes binding of the C ABI in include/ssegpu.h (libssegpu.so).
+
+This is the Python stand-in for the Julia ``ccall`` layer: it marshals a
+FlatModel and walker parameters into the plain structs of the ABI and loads
+the in-tree HIP library.  There is no CPU fallback: if the library is not
+built, or no gfx950 device is present, calls fail with an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libssegpu.so")
+
+SSE_MODE_WAVE = 0
+SSE_MODE_LANE = 1
+
+STATUS = {0: "SSE_OK", 1: "SSE_ERR_INVALID", 2: "SSE_ERR_HIP", 3: "SSE_ERR_OOM",
+          4: "SSE_ERR_SCATTER", 5: "SSE_ERR_NO_DEVICE", 6: "SSE_ERR_STATE"}
+
+
+class VertexDataDesc(C.Structure):
+    _fields_ = [("dims", C.c_int32 * 2), ("energy_offset", C.c_double),
+                ("n_vertices", C.c_int32), ("weights", C.POINTER(C.c_double)),
+                ("signs", C.POINTER(C.c_int8)), ("leg_states", C.POINTER(C.c_uint8)),
+                ("diagonal_vertices", C.POINTER(C.c_int32)), ("max_worm_count", C.c_int32),
+                ("transition_offset", C.POINTER(C.c_int32)),
+                ("transition_count", C.POINTER(C.c_int32)), ("n_exits", C.c_int32),
+                ("exit_cumprob", C.POINTER(C.c_double)), ("exit_target", C.POINTER(C.c_int32)),
+                ("exit_leg", C.POINTER(C.c_uint8)), ("exit_worm", C.POINTER(C.c_uint8))]
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("n_sites", C.c_int32), ("site_dims", C.POINTER(C.c_uint8)),
+                ("n_bonds", C.c_int32), ("bond_type", C.POINTER(C.c_int32)),
+                ("bond_sites", C.POINTER(C.c_int32)), ("n_types", C.c_int32),
+                ("types", C.POINTER(VertexDataDesc)), ("energy_offset", C.c_double),
+                ("normalization_site_count", C.c_double), ("n_estimators", C.c_int32),
+                ("est_max_dim", C.c_int32), ("est_coef", C.POINTER(C.c_double)),
+                ("est_mval", C.POINTER(C.c_double))]
+
+
+class WalkerDesc(C.Structure):
+    _fields_ = [("model", C.c_int32), ("diagonal_warmup_sweeps", C.c_int32),
+                ("T", C.c_double), ("seed", C.c_uint64), ("init_opstring_cutoff", C.c_int64),
+                ("init_num_worms", C.c_double), ("num_worms_attenuation_factor", C.c_double),
+                ("target_worm_length_fraction", C.c_double), ("thermalization", C.c_int64)]
+
+
+class WalkerScalars(C.Structure):
+    _fields_ = [("cutoff", C.c_int64), ("num_operators", C.c_int64),
+                ("avg_worm_length", C.c_double), ("num_worms", C.c_double),
+                ("rng_counter", C.c_uint64), ("sweeps", C.c_int64),
+                ("last_scatter_calls", C.c_int64), ("last_total_worm_length", C.c_double),
+                ("last_worm_length_fraction", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class ModelDescs:
+    """Array of ModelDesc built from FlatModels; holds every numpy buffer the
+    structs point into."""
+
+    def __init__(self, flats):
+        self.flats = list(flats)
+        self.keep = []
+        self.array = (ModelDesc * len(self.flats))()
+        for k, fm in enumerate(self.flats):
+            self.array[k] = self._model(fm)
+
+    def _arr(self, a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        if a.size == 0:
+            a = np.zeros(1, dtype=dtype)
+        self.keep.append(a)
+        return a
+
+    def _model(self, fm):
+        types = (VertexDataDesc * len(fm.types))()
+        self.keep.append(types)
+        for t, d in enumerate(fm.types):
+            vd = types[t]
+            vd.dims[0], vd.dims[1] = d["dims"]
+            vd.energy_offset = d["energy_offset"]
+            vd.n_vertices = d["n_vertices"]
+            vd.weights = _ptr(self._arr(d["weights"], np.float64), C.c_double)
+            vd.signs = _ptr(self._arr(d["signs"], np.int8), C.c_int8)
+            vd.leg_states = _ptr(self._arr(d["leg_states"], np.uint8), C.c_uint8)
+            vd.diagonal_vertices = _ptr(self._arr(d["diagonal_vertices"], np.int32), C.c_int32)
+            vd.max_worm_count = d["max_worm_count"]
+            vd.transition_offset = _ptr(self._arr(d["transition_offset"], np.int32), C.c_int32)
+            vd.transition_count = _ptr(self._arr(d["transition_count"], np.int32), C.c_int32)
+            vd.n_exits = d["n_exits"]
+            vd.exit_cumprob = _ptr(self._arr(d["exit_cumprob"], np.float64), C.c_double)
+            vd.exit_target = _ptr(self._arr(d["exit_target"], np.int32), C.c_int32)
+            vd.exit_leg = _ptr(self._arr(d["exit_leg"], np.uint8), C.c_uint8)
+            vd.exit_worm = _ptr(self._arr(d["exit_worm"], np.uint8), C.c_uint8)
+        m = ModelDesc()
+        m.n_sites = fm.n_sites
+        m.site_dims = _ptr(self._arr(fm.site_dims, np.uint8), C.c_uint8)
+        m.n_bonds = fm.n_bonds
+        m.bond_type = _ptr(self._arr(fm.bond_type, np.int32), C.c_int32)
+        m.bond_sites = _ptr(self._arr(fm.bond_sites, np.int32), C.c_int32)
+        m.n_types = len(fm.types)
+        m.types = C.cast(types, C.POINTER(VertexDataDesc))
+        m.energy_offset = fm.energy_offset
+        m.normalization_site_count = fm.normalization_site_count
+        m.n_estimators = fm.n_estimators
+        m.est_max_dim = fm.est_max_dim
+        m.est_coef = _ptr(self._arr(fm.est_coef, np.float64), C.c_double)
+        m.est_mval = _ptr(self._arr(fm.est_mval, np.float64), C.c_double)
+        return m
+
+
+def walker_desc(model=0, T=1.0, seed=0, init_opstring_cutoff=-1, diagonal_warmup_sweeps=5,
+                init_num_worms=5.0, num_worms_attenuation_factor=0.01,
+                target_worm_length_fraction=2.0, thermalization=0):
+    w = WalkerDesc()
+    w.model = model
+    w.T = T
+    w.seed = seed
+    w.init_opstring_cutoff = init_opstring_cutoff
+    w.diagonal_warmup_sweeps = diagonal_warmup_sweeps
+    w.init_num_worms = init_num_worms
+    w.num_worms_attenuation_factor = num_worms_attenuation_factor
+    w.target_worm_length_fraction = target_worm_length_fraction
+    w.thermalization = thermalization
+    return w
+
+
+def walker_array(walkers):
+    arr = (WalkerDesc * len(walkers))()
+    for k, w in enumerate(walkers):
+        arr[k] = w
+    return arr
+
+
+def default_cutoff(n_sites, T):
+    """round(Int, N*T) with ties to even (sse.jl:50-51)."""
+    return int(round(n_sites * T))
+
+
+_LIB = None
+
+
+def load_library(path=None):
+    """Load libssegpu.so from the package directory and declare signatures."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"libssegpu.so not built ({p}); run __graft_entry__.build()")
+    lib = C.CDLL(p)
+    vp = C.c_void_p
+    lib.sse_batch_create.argtypes = [C.POINTER(ModelDesc), C.c_int32, C.POINTER(WalkerDesc),
+                                     C.c_int32, C.c_int32, C.POINTER(vp)]
+    lib.sse_batch_destroy.argtypes = [vp]
+    lib.sse_init.argtypes = [vp]
+    lib.sse_sweep.argtypes = [vp, C.c_int32]
+    lib.sse_measure.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.sse_measure_width.argtypes = [vp, C.c_int32]
+    lib.sse_measure_width.restype = C.c_int32
+    lib.sse_run.argtypes = [vp, C.c_int32, C.c_int32]
+    lib.sse_read_accumulators.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int64)]
+    lib.sse_reset_accumulators.argtypes = [vp]
+    lib.sse_get_walker.argtypes = [vp, C.c_int32, C.POINTER(WalkerScalars),
+                                   C.POINTER(C.c_uint32), C.c_int64, C.POINTER(C.c_uint8)]
+    lib.sse_set_walker.argtypes = [vp, C.c_int32, C.POINTER(WalkerScalars),
+                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)]
+    lib.sse_sweep_phases.argtypes = [vp, C.c_int32]
+    lib.sse_get_vertex_list.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.c_int64,
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    lib.sse_phase_times.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.sse_work_counters.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    lib.sse_opcode_to_reference.argtypes = [C.c_uint32]
+    lib.sse_opcode_to_reference.restype = C.c_uint64
+    lib.sse_opcode_from_reference.argtypes = [C.c_uint64]
+    lib.sse_opcode_from_reference.restype = C.c_uint32
+    lib.sse_last_error.restype = C.c_char_p
+    lib.sse_abi_version.restype = C.c_int32
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+EXPORTED_SYMBOLS = (
+    "sse_batch_create", "sse_batch_destroy", "sse_init", "sse_sweep", "sse_measure",
+    "sse_measure_width", "sse_run", "sse_read_accumulators", "sse_reset_accumulators",
+    "sse_get_walker", "sse_set_walker", "sse_sweep_phases", "sse_get_vertex_list",
+    "sse_phase_times", "sse_work_counters", "sse_opcode_to_reference",
+    "sse_opcode_from_reference", "sse_last_error", "sse_abi_version")
+
+
+class SSEError(RuntimeError):
+    pass
+
+
+def check(lib, status):
+    if status != 0:
+        msg = lib.sse_last_error()
+        raise SSEError(f"{STATUS.get(status, status)}: {msg.decode() if msg else ''}")
+    return status
+
+
+class Batch:
+    """A batch of walkers resident on the GPU (owns one sse_batch handle)."""
+
+    def __init__(self, flats, walkers, mode=SSE_MODE_WAVE, lib=None):
+        self.lib = lib or load_library()
+        self.models = ModelDescs(flats)
+        self.walkers = list(walkers)
+        self.flats = self.models.flats
+        warr = walker_array(self.walkers)
+        h = C.c_void_p()
+        check(self.lib, self.lib.sse_batch_create(self.models.array, len(self.flats), warr,
+                                                  len(self.walkers), mode, C.byref(h)))
+        self.handle = h
+        self.mode = mode
+
+    def close(self):
+        if self.handle:
+            self.lib.sse_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_walkers(self):
+        return len(self.walkers)
+
+    def init(self):
+        check(self.lib, self.lib.sse_init(self.handle))
+
+    def sweep(self, n=1):
+        check(self.lib, self.lib.sse_sweep(self.handle, n))
+
+    def sweep_phases(self, phases):
+        check(self.lib, self.lib.sse_sweep_phases(self.handle, phases))
+
+    def width(self, w=0):
+        return self.lib.sse_measure_width(self.handle, w)
+
+    def measure(self):
+        widths = [self.flats[w.model].measure_width for w in self.walkers]
+        total = sum(widths)
+        out = np.zeros(total, dtype=np.float64)
+        check(self.lib, self.lib.sse_measure(self.handle, _ptr(out, C.c_double)))
+        res, k = [], 0
+        for wd in widths:
+            res.append(out[k:k + wd])
+            k += wd
+        return res
+
+    def run(self, n_sweeps, measure=True):
+        check(self.lib, self.lib.sse_run(self.handle, n_sweeps, int(bool(measure))))
+
+    def accumulators(self):
+        widths = [self.flats[w.model].measure_width for w in self.walkers]
+        total = sum(widths)
+        s = np.zeros(total)
+        s2 = np.zeros(total)
+        cnt = np.zeros(len(self.walkers), dtype=np.int64)
+        check(self.lib, self.lib.sse_read_accumulators(self.handle, _ptr(s, C.c_double),
+                                                       _ptr(s2, C.c_double),
+                                                       _ptr(cnt, C.c_int64)))
+        out, k = [], 0
+        for w, wd in enumerate(widths):
+            out.append((s[k:k + wd], s2[k:k + wd], int(cnt[w])))
+            k += wd
+        return out
+
+    def reset_accumulators(self):
+        check(self.lib, self.lib.sse_reset_accumulators(self.handle))
+
+    def get_walker(self, w, with_ops=True, with_spins=True):
+        s = WalkerScalars()
+        check(self.lib, self.lib.sse_get_walker(self.handle, w, C.byref(s), None, 0, None))
+        ops = np.zeros(max(1, s.cutoff), dtype=np.uint32) if with_ops else None
+        n_sites = self.flats[self.walkers[w].model].n_sites
+        spins = np.zeros(n_sites, dtype=np.uint8) if with_spins else None
+        check(self.lib, self.lib.sse_get_walker(
+            self.handle, w, C.byref(s),
+            _ptr(ops, C.c_uint32) if ops is not None else None, len(ops) if ops is not None else 0,
+            _ptr(spins, C.c_uint8) if spins is not None else None))
+        if ops is not None:
+            ops = ops[:s.cutoff]
+        return s.as_dict(), ops, spins
+
+    def set_walker(self, w, scalars, ops, spins):
+        s = WalkerScalars()
+        for k, _ in WalkerScalars._fields_:
+            if k in scalars:
+                setattr(s, k, scalars[k])
+        ops = np.ascontiguousarray(ops, dtype=np.uint32)
+        spins = np.ascontiguousarray(spins, dtype=np.uint8)
+        s.cutoff = len(ops)
+        check(self.lib, self.lib.sse_set_walker(self.handle, w, C.byref(s), _ptr(ops, C.c_uint32),
+                                                _ptr(spins, C.c_uint8)))
+
+    def vertex_list(self, w):
+        sc, _, _ = self.get_walker(w, with_ops=False, with_spins=False)
+        L = sc["cutoff"]
+        n_sites = self.flats[self.walkers[w].model].n_sites
+        links = np.zeros(max(1, 4 * L), dtype=np.int32)
+        first = np.zeros(n_sites, dtype=np.int32)
+        last = np.zeros(n_sites, dtype=np.int32)
+        check(self.lib, self.lib.sse_get_vertex_list(self.handle, w, _ptr(links, C.c_int32),
+                                                     len(links), _ptr(first, C.c_int32),
+                                                     _ptr(last, C.c_int32)))
+        return links[:4 * L], first, last
+
+    def phase_times(self):
+        t = np.zeros(3)
+        check(self.lib, self.lib.sse_phase_times(self.handle, _ptr(t, C.c_double)))
+        return t
+
+    def work_counters(self):
+        a = [C.c_int64() for _ in range(4)]
+        check(self.lib, self.lib.sse_work_counters(self.handle, *[C.byref(x) for x in a]))
+        return tuple(x.value for x in a)

@@ -1,0 +1,802 @@
+// capi.hip -- C ABI of libssegpu.so (declared in include/ssegpu.h).
+//
+// A batch owns: the flattened models (uploaded once), the walker array, a
+// slab pool from which every walker's opstring/vertex-list slab is carved
+// (device-side bump allocation lets the cutoff grow inside the kernels), the
+// spin/first/last arrays, measurement buffers and one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ssegpu.h"
+#include "sse_common.h"
+#include "sse_lane.h"
+#include "sse_wave.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHECK(expr)                                                                     \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return fail(SSE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));   \
+    } while (0)
+
+}  // namespace
+
+struct sse_batch {
+    int device = 0;
+    int mode = SSE_MODE_WAVE;
+    hipStream_t stream = nullptr;
+    int n_models = 0, n_walkers = 0;
+    std::vector<sse_walker_desc> wdesc;
+    std::vector<int32_t> widths, offsets, nsites;
+    std::vector<DevModel> h_models;
+    std::vector<int> model_max_dim;
+    DevModel *d_models = nullptr;
+    std::vector<void *> allocs;
+    DevWalker *d_walkers = nullptr;
+    std::vector<DevWalker> h_walkers;
+    DevPool *d_pool = nullptr;
+    char *pool_base = nullptr;
+    size_t pool_size = 0;
+    size_t pool_host_next = 0;
+    uint8_t *d_spins = nullptr;
+    int32_t *d_first = nullptr, *d_last = nullptr;
+    double *d_out = nullptr, *d_sum = nullptr, *d_sumsq = nullptr;
+    int64_t *d_count = nullptr;
+    uint8_t *d_scratch = nullptr;
+    int max_sites = 0, total_width = 0;
+    int64_t max_records = 0;
+    bool inited = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double phase_ms[3] = {0, 0, 0};
+    WaveConfig wave;
+};
+
+template <class T>
+static int dev_alloc(sse_batch *b, T **p, size_t count)
+{
+    void *q = nullptr;
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0)
+        bytes = 16;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+        return fail(SSE_ERR_OOM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    b->allocs.push_back(q);
+    *p = (T *)q;
+    return SSE_OK;
+}
+
+static int validate_model(const sse_model_desc &m, int k)
+{
+    char buf[256];
+    if (m.n_sites <= 0 || m.n_bonds <= 0 || m.n_types <= 0 || !m.site_dims || !m.bond_type ||
+        !m.bond_sites || !m.types) {
+        snprintf(buf, sizeof buf, "model %d: empty graph or missing arrays", k);
+        return fail(SSE_ERR_INVALID, buf);
+    }
+    if (m.n_sites >= (1 << 20) || m.n_bonds >= (1 << 21) || m.n_types >= (1 << 11)) {
+        snprintf(buf, sizeof buf, "model %d: too many sites/bonds/types", k);
+        return fail(SSE_ERR_INVALID, buf);
+    }
+    for (int i = 0; i < m.n_sites; ++i)
+        if (m.site_dims[i] < 1) {
+            snprintf(buf, sizeof buf, "model %d: site %d has dimension 0 (not on any bond)", k, i);
+            return fail(SSE_ERR_INVALID, buf);
+        }
+    for (int b = 0; b < m.n_bonds; ++b) {
+        int t = m.bond_type[b], s0 = m.bond_sites[2 * b], s1 = m.bond_sites[2 * b + 1];
+        if (t < 0 || t >= m.n_types || s0 < 0 || s0 >= m.n_sites || s1 < 0 || s1 >= m.n_sites ||
+            s0 == s1) {
+            snprintf(buf, sizeof buf, "model %d: bond %d invalid", k, b);
+            return fail(SSE_ERR_INVALID, buf);
+        }
+        const sse_vertex_data_desc &vd = m.types[t];
+        if (vd.dims[0] != m.site_dims[s0] || vd.dims[1] != m.site_dims[s1]) {
+            snprintf(buf, sizeof buf, "model %d: bond %d site dimensions inconsistent", k, b);
+            return fail(SSE_ERR_INVALID, buf);
+        }
+    }
+    for (int t = 0; t < m.n_types; ++t) {
+        const sse_vertex_data_desc &vd = m.types[t];
+        if (vd.n_vertices <= 0 || vd.n_vertices > 512 || vd.dims[0] < 1 || vd.dims[1] < 1 ||
+            vd.dims[0] > 255 || vd.dims[1] > 255 || vd.max_worm_count < 1) {
+            snprintf(buf, sizeof buf, "model %d type %d: unsupported table shape", k, t);
+            return fail(SSE_ERR_INVALID, buf);
+        }
+        for (int i = 0; i < vd.dims[0] * vd.dims[1]; ++i) {
+            int dv = vd.diagonal_vertices[i];
+            if (dv >= 0 && ((dv >> 1) >= vd.n_vertices || !(dv & 1))) {
+                snprintf(buf, sizeof buf, "model %d type %d: bad diagonal vertex", k, t);
+                return fail(SSE_ERR_INVALID, buf);
+            }
+        }
+        int nt = vd.n_vertices * vd.max_worm_count * 4;
+        for (int i = 0; i < nt; ++i) {
+            int off = vd.transition_offset[i], cnt = vd.transition_count[i];
+            if (off >= 0 && (cnt < 0 || off + cnt > vd.n_exits)) {
+                snprintf(buf, sizeof buf, "model %d type %d: transition out of range", k, t);
+                return fail(SSE_ERR_INVALID, buf);
+            }
+        }
+        for (int e = 0; e < vd.n_exits; ++e) {
+            if (vd.exit_leg[e] > 3 || (vd.exit_target[e] >> 1) >= vd.n_vertices ||
+                vd.exit_target[e] < 0 || vd.exit_worm[e] < 1 || vd.exit_worm[e] > vd.max_worm_count) {
+                snprintf(buf, sizeof buf, "model %d type %d: exit %d out of range", k, t, e);
+                return fail(SSE_ERR_INVALID, buf);
+            }
+        }
+    }
+    if (m.n_estimators < 0 || (m.n_estimators > 0 && (!m.est_coef || !m.est_mval)))
+        return fail(SSE_ERR_INVALID, "estimator tables missing");
+    return SSE_OK;
+}
+
+// Upload one model: every array into one device buffer, pointers fixed up.
+static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
+{
+    std::vector<char> blob;
+    auto put = [&](const void *src, size_t bytes) {
+        size_t off = (blob.size() + 15) & ~(size_t)15;
+        blob.resize(off + bytes + 16);
+        if (bytes)
+            memcpy(blob.data() + off, src, bytes);
+        blob.resize(off + bytes);
+        return off;
+    };
+    struct VOff {
+        size_t w, s, ls, dv, to, tc, cum, tgt, el, ew;
+    };
+    std::vector<VOff> voff(m.n_types);
+    int max_dim = 1;
+    for (int i = 0; i < m.n_sites; ++i)
+        max_dim = m.site_dims[i] > max_dim ? m.site_dims[i] : max_dim;
+    for (int t = 0; t < m.n_types; ++t) {
+        const sse_vertex_data_desc &vd = m.types[t];
+        size_t nt = (size_t)vd.n_vertices * vd.max_worm_count * 4;
+        voff[t].w = put(vd.weights, sizeof(double) * vd.n_vertices);
+        voff[t].s = put(vd.signs, vd.n_vertices);
+        voff[t].ls = put(vd.leg_states, 4 * (size_t)vd.n_vertices);
+        voff[t].dv = put(vd.diagonal_vertices, sizeof(int32_t) * vd.dims[0] * vd.dims[1]);
+        voff[t].to = put(vd.transition_offset, sizeof(int32_t) * nt);
+        voff[t].tc = put(vd.transition_count, sizeof(int32_t) * nt);
+        voff[t].cum = put(vd.exit_cumprob, sizeof(double) * vd.n_exits);
+        voff[t].tgt = put(vd.exit_target, sizeof(int32_t) * vd.n_exits);
+        voff[t].el = put(vd.exit_leg, vd.n_exits);
+        voff[t].ew = put(vd.exit_worm, vd.n_exits);
+    }
+    std::vector<int2> brec(m.n_bonds);
+    for (int k = 0; k < m.n_bonds; ++k) {
+        brec[k].x = m.bond_sites[2 * k] | (m.bond_type[k] << 20);
+        brec[k].y = m.bond_sites[2 * k + 1];
+    }
+    size_t o_sd = put(m.site_dims, m.n_sites);
+    size_t o_bt = put(m.bond_type, sizeof(int32_t) * m.n_bonds);
+    size_t o_bs = put(m.bond_sites, sizeof(int32_t) * 2 * m.n_bonds);
+    size_t o_br = put(brec.data(), sizeof(int2) * m.n_bonds);
+    size_t ne = (size_t)m.n_estimators;
+    size_t o_ec = put(m.est_coef, sizeof(double) * ne * m.n_sites);
+    size_t o_em = put(m.est_mval, sizeof(double) * ne * m.n_sites * (m.n_estimators ? m.est_max_dim : 0));
+    size_t o_types = put(nullptr, sizeof(DevVD) * m.n_types);
+
+    char *d = nullptr;
+    int rc = dev_alloc(b, &d, blob.size() + 16);
+    if (rc)
+        return rc;
+    std::vector<DevVD> types(m.n_types);
+    for (int t = 0; t < m.n_types; ++t) {
+        const sse_vertex_data_desc &vd = m.types[t];
+        DevVD &x = types[t];
+        x.dims0 = vd.dims[0];
+        x.dims1 = vd.dims[1];
+        x.nv = vd.n_vertices;
+        x.max_worm = vd.max_worm_count;
+        x.n_exits = vd.n_exits;
+        x.pad = 0;
+        x.energy_offset = vd.energy_offset;
+        x.weights = (const double *)(d + voff[t].w);
+        x.signs = (const int8_t *)(d + voff[t].s);
+        x.leg_states = (const uint8_t *)(d + voff[t].ls);
+        x.diag_vertex = (const int32_t *)(d + voff[t].dv);
+        x.toff = (const int32_t *)(d + voff[t].to);
+        x.tcnt = (const int32_t *)(d + voff[t].tc);
+        x.cum = (const double *)(d + voff[t].cum);
+        x.tgt = (const int32_t *)(d + voff[t].tgt);
+        x.eleg = (const uint8_t *)(d + voff[t].el);
+        x.ewrm = (const uint8_t *)(d + voff[t].ew);
+    }
+    memcpy(blob.data() + o_types, types.data(), sizeof(DevVD) * m.n_types);
+    HIPCHECK(hipMemcpy(d, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    dm.n_sites = m.n_sites;
+    dm.n_bonds = m.n_bonds;
+    dm.n_types = m.n_types;
+    dm.n_est = m.n_estimators;
+    dm.est_max_dim = m.n_estimators ? m.est_max_dim : 1;
+    dm.max_dim = max_dim;
+    dm.energy_offset = m.energy_offset;
+    dm.norm = m.normalization_site_count;
+    dm.site_dims = (const uint8_t *)(d + o_sd);
+    dm.bond_type = (const int32_t *)(d + o_bt);
+    dm.bond_sites = (const int32_t *)(d + o_bs);
+    dm.bond_rec = (const int2 *)(d + o_br);
+    dm.est_coef = (const double *)(d + o_ec);
+    dm.est_mval = (const double *)(d + o_em);
+    dm.types = (const DevVD *)(d + o_types);
+    return SSE_OK;
+}
+
+static size_t env_size(const char *name, size_t dflt)
+{
+    const char *s = getenv(name);
+    if (!s || !*s)
+        return dflt;
+    return (size_t)strtoull(s, nullptr, 10);
+}
+
+// host-side bump allocation of a slab (only while no kernel runs)
+static char *host_slab(sse_batch *b, int64_t cap)
+{
+    size_t bytes = slab_bytes(cap);
+    if (b->pool_host_next + bytes > b->pool_size)
+        return nullptr;
+    char *p = b->pool_base + b->pool_host_next;
+    b->pool_host_next += bytes;
+    return p;
+}
+
+static int sync_pool_to_device(sse_batch *b)
+{
+    DevPool p;
+    p.next = b->pool_host_next;
+    p.size = b->pool_size;
+    p.base = b->pool_base;
+    p.overflow = 0;
+    p.status = 0;
+    HIPCHECK(hipMemcpy(b->d_pool, &p, sizeof p, hipMemcpyHostToDevice));
+    return SSE_OK;
+}
+
+static int sync_pool_from_device(sse_batch *b)
+{
+    DevPool p;
+    HIPCHECK(hipMemcpy(&p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));
+    b->pool_host_next = p.next > p.size ? p.size : p.next;
+    if (p.overflow)
+        return fail(SSE_ERR_OOM, "slab pool exhausted; raise SSE_POOL_MB");
+    return SSE_OK;
+}
+
+static int check_walker_errors(sse_batch *b)
+{
+    DevPool p;
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    HIPCHECK(hipMemcpy(&p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));
+    if (p.overflow)
+        return fail(SSE_ERR_OOM, "slab pool exhausted; raise SSE_POOL_MB");
+    if (p.status) {
+        if (p.status & (1 << SSE_ERR_SCATTER))
+            return fail(SSE_ERR_SCATTER, "scatter fell through (r >= last cumprob) or hit an "
+                                         "invalid transition");
+        return fail(SSE_ERR_STATE, "walker error flag set");
+    }
+    return SSE_OK;
+}
+
+extern "C" {
+
+int32_t sse_abi_version(void) { return SSEGPU_ABI_VERSION; }
+
+const char *sse_last_error(void) { return g_err.c_str(); }
+
+uint64_t sse_opcode_to_reference(uint32_t op)
+{
+    if (op == 0)
+        return 0;
+    uint64_t vc = (op >> 1) & 1023u;
+    uint64_t bond = op >> SSE_OP_BOND_SHIFT;
+    uint64_t vref = (vc & 1) | (((vc >> 1) + 1) << 1);
+    return 1ull | (vref << 1) | ((bond + 1) << 26);
+}
+
+uint32_t sse_opcode_from_reference(uint64_t op)
+{
+    if (op == 0)
+        return 0;
+    uint64_t vref = (op & ((1ull << 25) - 1)) >> 1;
+    uint64_t bond = (op >> 26) - 1;
+    uint64_t vc = (vref & 1) | (((vref >> 1) - 1) << 1);
+    return (uint32_t)(1u | (vc << 1) | (bond << SSE_OP_BOND_SHIFT));
+}
+
+int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_walker_desc *walkers,
+                     int32_t n_walkers, int32_t mode, sse_batch **out)
+{
+    if (!out || !models || !walkers || n_models <= 0 || n_walkers <= 0)
+        return fail(SSE_ERR_INVALID, "sse_batch_create: invalid arguments");
+    if (mode != SSE_MODE_WAVE && mode != SSE_MODE_LANE)
+        return fail(SSE_ERR_INVALID, "sse_batch_create: unknown mode");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(SSE_ERR_NO_DEVICE, "no HIP device");
+    for (int k = 0; k < n_models; ++k) {
+        int rc = validate_model(models[k], k);
+        if (rc)
+            return rc;
+    }
+    for (int w = 0; w < n_walkers; ++w) {
+        const sse_walker_desc &wd = walkers[w];
+        if (wd.model < 0 || wd.model >= n_models || !(wd.T > 0) || wd.init_num_worms < 1 ||
+            wd.diagonal_warmup_sweeps < 0)
+            return fail(SSE_ERR_INVALID, "sse_batch_create: invalid walker parameters");
+    }
+    sse_batch *b = new sse_batch();
+    b->mode = mode;
+    (void)hipGetDevice(&b->device);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(SSE_ERR_HIP, "hipStreamCreate failed");
+    }
+    for (int i = 0; i < 4; ++i)
+        (void)hipEventCreate(&b->ev[i]);
+    b->n_models = n_models;
+    b->n_walkers = n_walkers;
+    b->wdesc.assign(walkers, walkers + n_walkers);
+    b->h_models.resize(n_models);
+    int rc = SSE_OK;
+    for (int k = 0; k < n_models && !rc; ++k) {
+        rc = upload_model(b, models[k], b->h_models[k]);
+        b->max_sites = std::max(b->max_sites, (int)models[k].n_sites);
+    }
+    if (!rc)
+        rc = dev_alloc(b, &b->d_models, n_models);
+    if (!rc && hipMemcpy(b->d_models, b->h_models.data(), sizeof(DevModel) * n_models,
+                         hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(SSE_ERR_HIP, "model upload failed");
+    // initial cutoffs and slab capacities
+    size_t need = 0;
+    std::vector<int64_t> L0(n_walkers), cap0(n_walkers);
+    b->widths.resize(n_walkers);
+    b->offsets.resize(n_walkers);
+    b->nsites.resize(n_walkers);
+    int off = 0;
+    for (int w = 0; w < n_walkers && !rc; ++w) {
+        const sse_walker_desc &wd = walkers[w];
+        const sse_model_desc &m = models[wd.model];
+        L0[w] = wd.init_opstring_cutoff >= 0 ? wd.init_opstring_cutoff
+                                             : (int64_t)std::nearbyint((double)m.n_sites * wd.T);
+        cap0[w] = slab_capacity(L0[w]);
+        need += slab_bytes(cap0[w]);
+        b->widths[w] = 5 + 5 * m.n_estimators;
+        b->offsets[w] = off;
+        b->nsites[w] = m.n_sites;
+        off += b->widths[w];
+    }
+    b->total_width = off;
+    if (!rc) {
+        size_t free_b = 0, total_b = 0;
+        (void)hipMemGetInfo(&free_b, &total_b);
+        size_t dflt = std::max<size_t>((size_t)2 << 30, need * 64);
+        dflt = std::min(dflt, (size_t)((double)free_b * 0.4));
+        dflt = std::max(dflt, need * 4 + ((size_t)64 << 20));
+        b->pool_size = env_size("SSE_POOL_MB", 0) ? env_size("SSE_POOL_MB", 0) << 20 : dflt;
+        if (b->pool_size < need)
+            rc = fail(SSE_ERR_OOM, "SSE_POOL_MB smaller than the initial slabs");
+    }
+    if (!rc)
+        rc = dev_alloc(b, &b->pool_base, b->pool_size);
+    if (!rc)
+        rc = dev_alloc(b, &b->d_pool, 1);
+    size_t nsp = (size_t)n_walkers * b->max_sites;
+    if (!rc)
+        rc = dev_alloc(b, &b->d_spins, nsp);
+    if (!rc)
+        rc = dev_alloc(b, &b->d_first, nsp);
+    if (!rc)
+        rc = dev_alloc(b, &b->d_last, nsp);
+    if (!rc)
+        rc = dev_alloc(b, &b->d_scratch, nsp);
+    if (!rc)
+        rc = dev_alloc(b, &b->d_out, (size_t)b->total_width);
+    if (!rc)
+        rc = dev_alloc(b, &b->d_sum, (size_t)b->total_width);
+    if (!rc)
+        rc = dev_alloc(b, &b->d_sumsq, (size_t)b->total_width);
+    if (!rc)
+        rc = dev_alloc(b, &b->d_count, (size_t)n_walkers);
+    if (!rc)
+        rc = dev_alloc(b, &b->d_walkers, (size_t)n_walkers);
+    if (!rc) {
+        b->h_walkers.resize(n_walkers);
+        for (int w = 0; w < n_walkers && !rc; ++w) {
+            const sse_walker_desc &wd = walkers[w];
+            DevWalker &x = b->h_walkers[w];
+            memset(&x, 0, sizeof x);
+            x.model = wd.model;
+            x.warmup = wd.diagonal_warmup_sweeps;
+            x.T = wd.T;
+            x.target_wlf = wd.target_worm_length_fraction;
+            x.atten = wd.num_worms_attenuation_factor;
+            x.key = wd.seed;
+            x.thermalization = wd.thermalization;
+            x.init_cutoff = L0[w];
+            x.avg_worm_length = 1.0;
+            x.num_worms = wd.init_num_worms;
+            x.L = L0[w];
+            x.n = 0;
+            x.cap = cap0[w];
+            x.ctr = 0;
+            x.n_sites = b->nsites[w];
+            x.last_wlf = NAN;
+            char *s = host_slab(b, cap0[w]);
+            if (!s) {
+                rc = fail(SSE_ERR_OOM, "slab pool too small");
+                break;
+            }
+            x.ops = (uint32_t *)s;
+            x.links = (int32_t *)(s + (size_t)x.cap * 4);
+            x.spins = b->d_spins + (size_t)w * b->max_sites;
+            x.first = b->d_first + (size_t)w * b->max_sites;
+            x.last = b->d_last + (size_t)w * b->max_sites;
+            x.meas_offset = b->offsets[w];
+            x.meas_width = b->widths[w];
+        }
+    }
+    if (!rc && hipMemcpy(b->d_walkers, b->h_walkers.data(), sizeof(DevWalker) * n_walkers,
+                         hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(SSE_ERR_HIP, "walker upload failed");
+    if (!rc)
+        rc = sync_pool_to_device(b);
+    if (!rc) {
+        (void)hipMemset(b->d_sum, 0, sizeof(double) * b->total_width);
+        (void)hipMemset(b->d_sumsq, 0, sizeof(double) * b->total_width);
+        (void)hipMemset(b->d_count, 0, sizeof(int64_t) * n_walkers);
+        rc = wave_configure(b->wave, b->h_models, b->max_sites);
+    }
+    if (rc) {
+        std::string keep = g_err;
+        sse_batch_destroy(b);
+        g_err = keep;
+        return rc;
+    }
+    *out = b;
+    return SSE_OK;
+}
+
+int sse_batch_destroy(sse_batch *b)
+{
+    if (!b)
+        return SSE_OK;
+    if (b->stream)
+        (void)hipStreamSynchronize(b->stream);
+    for (void *p : b->allocs)
+        (void)hipFree(p);
+    for (int i = 0; i < 4; ++i)
+        if (b->ev[i])
+            (void)hipEventDestroy(b->ev[i]);
+    if (b->stream)
+        (void)hipStreamDestroy(b->stream);
+    delete b;
+    return SSE_OK;
+}
+
+static int launch_sweep(sse_batch *b, int phases)
+{
+    int n = b->n_walkers;
+    if (b->mode == SSE_MODE_LANE) {
+        int threads = 64;
+        int blocks = (n + threads - 1) / threads;
+        hipLaunchKernelGGL(lane_sweep_kernel, dim3(blocks), dim3(threads), 0, b->stream,
+                           b->d_walkers, b->d_models, b->d_pool, 0, n, phases);
+    } else {
+        int rc = wave_launch_sweep(b->wave, b->stream, b->d_walkers, b->d_models, b->d_pool, n,
+                                   phases, b->ev);
+        if (rc)
+            return rc;
+    }
+    HIPCHECK(hipGetLastError());
+    return SSE_OK;
+}
+
+int sse_init(sse_batch *b)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    int n = b->n_walkers;
+    if (b->mode == SSE_MODE_LANE) {
+        hipLaunchKernelGGL(lane_init_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream,
+                           b->d_walkers, b->d_models, b->d_pool, 0, n);
+        HIPCHECK(hipGetLastError());
+    } else {
+        int rc = wave_launch_init(b->wave, b->stream, b->d_walkers, b->d_models, b->d_pool, n);
+        if (rc)
+            return rc;
+    }
+    int rc = check_walker_errors(b);
+    if (rc)
+        return rc;
+    b->inited = true;
+    return SSE_OK;
+}
+
+int sse_sweep(sse_batch *b, int32_t n_sweeps)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    if (!b->inited)
+        return fail(SSE_ERR_STATE, "sse_sweep before sse_init");
+    if (n_sweeps < 0)
+        return fail(SSE_ERR_INVALID, "negative sweep count");
+    HIPCHECK(hipEventRecord(b->ev[0], b->stream));
+    for (int k = 0; k < n_sweeps; ++k) {
+        int rc = launch_sweep(b, 7);
+        if (rc)
+            return rc;
+    }
+    HIPCHECK(hipEventRecord(b->ev[3], b->stream));
+    int rc = check_walker_errors(b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[3]);
+    b->phase_ms[0] = ms;
+    b->phase_ms[1] = b->phase_ms[2] = 0;
+    return rc;
+}
+
+int sse_sweep_phases(sse_batch *b, int32_t phases)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    if (!b->inited)
+        return fail(SSE_ERR_STATE, "sweep before init");
+    if (phases <= 0 || phases > 7)
+        return fail(SSE_ERR_INVALID, "phases must be a mask of 1|2|4");
+    int rc = launch_sweep(b, phases);
+    if (rc)
+        return rc;
+    return check_walker_errors(b);
+}
+
+static int launch_measure(sse_batch *b, int mode)
+{
+    int n = b->n_walkers;
+    if (b->mode == SSE_MODE_LANE) {
+        hipLaunchKernelGGL(lane_measure_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream,
+                           b->d_walkers, b->d_models, b->d_out, b->d_sum, b->d_sumsq, b->d_count,
+                           b->d_scratch, b->max_sites, n, mode);
+    } else {
+        int rc = wave_launch_measure(b->wave, b->stream, b->d_walkers, b->d_models, b->d_out,
+                                     b->d_sum, b->d_sumsq, b->d_count, n, mode);
+        if (rc)
+            return rc;
+    }
+    HIPCHECK(hipGetLastError());
+    return SSE_OK;
+}
+
+int32_t sse_measure_width(const sse_batch *b, int32_t walker)
+{
+    if (!b || walker < 0 || walker >= b->n_walkers)
+        return -1;
+    return b->widths[walker];
+}
+
+int sse_measure(sse_batch *b, double *out)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    if (!b->inited)
+        return fail(SSE_ERR_STATE, "measure before init");
+    int rc = launch_measure(b, 0);
+    if (rc)
+        return rc;
+    if (out) {
+        HIPCHECK(hipMemcpyAsync(out, b->d_out, sizeof(double) * b->total_width,
+                                hipMemcpyDeviceToHost, b->stream));
+    }
+    return check_walker_errors(b);
+}
+
+int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    if (!b->inited)
+        return fail(SSE_ERR_STATE, "run before init");
+    HIPCHECK(hipEventRecord(b->ev[0], b->stream));
+    for (int k = 0; k < n_sweeps; ++k) {
+        int rc = launch_sweep(b, 7);
+        if (!rc && measure)
+            rc = launch_measure(b, 1);
+        if (rc)
+            return rc;
+    }
+    HIPCHECK(hipEventRecord(b->ev[3], b->stream));
+    int rc = check_walker_errors(b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[3]);
+    b->phase_ms[0] = ms;
+    return rc;
+}
+
+int sse_read_accumulators(sse_batch *b, double *sums, double *sumsq, int64_t *counts)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    if (sums)
+        HIPCHECK(hipMemcpy(sums, b->d_sum, sizeof(double) * b->total_width, hipMemcpyDeviceToHost));
+    if (sumsq)
+        HIPCHECK(hipMemcpy(sumsq, b->d_sumsq, sizeof(double) * b->total_width,
+                           hipMemcpyDeviceToHost));
+    if (counts)
+        HIPCHECK(hipMemcpy(counts, b->d_count, sizeof(int64_t) * b->n_walkers,
+                           hipMemcpyDeviceToHost));
+    return SSE_OK;
+}
+
+int sse_reset_accumulators(sse_batch *b)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    HIPCHECK(hipMemsetAsync(b->d_sum, 0, sizeof(double) * b->total_width, b->stream));
+    HIPCHECK(hipMemsetAsync(b->d_sumsq, 0, sizeof(double) * b->total_width, b->stream));
+    HIPCHECK(hipMemsetAsync(b->d_count, 0, sizeof(int64_t) * b->n_walkers, b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    return SSE_OK;
+}
+
+int sse_get_walker(sse_batch *b, int32_t walker, sse_walker_scalars *s, uint32_t *ops,
+                   int64_t ops_capacity, uint8_t *spins)
+{
+    if (!b || walker < 0 || walker >= b->n_walkers)
+        return fail(SSE_ERR_INVALID, "sse_get_walker: bad walker");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    DevWalker w;
+    HIPCHECK(hipMemcpy(&w, b->d_walkers + walker, sizeof w, hipMemcpyDeviceToHost));
+    if (s) {
+        s->cutoff = w.L;
+        s->num_operators = w.n;
+        s->avg_worm_length = w.avg_worm_length;
+        s->num_worms = w.num_worms;
+        s->rng_counter = w.ctr;
+        s->sweeps = w.sweeps;
+        s->last_scatter_calls = w.last_scatter;
+        s->last_total_worm_length = w.last_total;
+        s->last_worm_length_fraction = w.last_wlf;
+    }
+    if (ops && ops_capacity > 0 && w.L > 0) {
+        int64_t k = w.L < ops_capacity ? w.L : ops_capacity;
+        HIPCHECK(hipMemcpy(ops, w.ops, sizeof(uint32_t) * k, hipMemcpyDeviceToHost));
+    }
+    if (spins)
+        HIPCHECK(hipMemcpy(spins, w.spins, w.n_sites, hipMemcpyDeviceToHost));
+    return SSE_OK;
+}
+
+int sse_set_walker(sse_batch *b, int32_t walker, const sse_walker_scalars *s, const uint32_t *ops,
+                   const uint8_t *spins)
+{
+    if (!b || walker < 0 || walker >= b->n_walkers || !s || !ops || !spins || s->cutoff < 0)
+        return fail(SSE_ERR_INVALID, "sse_set_walker: bad arguments");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    DevWalker w;
+    HIPCHECK(hipMemcpy(&w, b->d_walkers + walker, sizeof w, hipMemcpyDeviceToHost));
+    const DevModel &m = b->h_models[w.model];
+    int64_t n = 0;
+    for (int64_t p = 0; p < s->cutoff; ++p) {
+        uint32_t op = ops[p];
+        if (op) {
+            ++n;
+            if ((op >> SSE_OP_BOND_SHIFT) >= (uint32_t)m.n_bonds)
+                return fail(SSE_ERR_INVALID, "sse_set_walker: operator bond out of range");
+        }
+    }
+    if (n != s->num_operators)
+        return fail(SSE_ERR_INVALID, "sse_set_walker: num_operators does not match ops");
+    int rc = sync_pool_from_device(b);
+    if (rc)
+        return rc;
+    if (s->cutoff > w.cap) {
+        int64_t cap = slab_capacity(s->cutoff);
+        char *slab = host_slab(b, cap);
+        if (!slab)
+            return fail(SSE_ERR_OOM, "slab pool exhausted");
+        w.ops = (uint32_t *)slab;
+        w.links = (int32_t *)(slab + (size_t)cap * 4);
+        w.cap = cap;
+        rc = sync_pool_to_device(b);
+        if (rc)
+            return rc;
+    }
+    w.L = s->cutoff;
+    w.n = n;
+    w.avg_worm_length = s->avg_worm_length;
+    w.num_worms = s->num_worms;
+    w.ctr = s->rng_counter;
+    w.sweeps = s->sweeps;
+    w.error = 0;
+    if (w.L > 0)
+        HIPCHECK(hipMemcpy(w.ops, ops, sizeof(uint32_t) * w.L, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(w.spins, spins, w.n_sites, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(b->d_walkers + walker, &w, sizeof w, hipMemcpyHostToDevice));
+    b->inited = true;
+    return SSE_OK;
+}
+
+int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links, int64_t links_capacity,
+                        int32_t *first, int32_t *last)
+{
+    if (!b || walker < 0 || walker >= b->n_walkers)
+        return fail(SSE_ERR_INVALID, "sse_get_vertex_list: bad walker");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    DevWalker w;
+    HIPCHECK(hipMemcpy(&w, b->d_walkers + walker, sizeof w, hipMemcpyDeviceToHost));
+    if (b->mode == SSE_MODE_LANE) {
+        if (links && links_capacity >= 4 * w.L && w.L > 0)
+            HIPCHECK(hipMemcpy(links, w.links, sizeof(int32_t) * 4 * w.L, hipMemcpyDeviceToHost));
+    } else {
+        int rc = wave_dense_vertex_list(b->wave, w, links, links_capacity);
+        if (rc)
+            return rc;
+    }
+    if (first)
+        HIPCHECK(hipMemcpy(first, w.first, sizeof(int32_t) * w.n_sites, hipMemcpyDeviceToHost));
+    if (last)
+        HIPCHECK(hipMemcpy(last, w.last, sizeof(int32_t) * w.n_sites, hipMemcpyDeviceToHost));
+    if (b->mode != SSE_MODE_LANE)
+        wave_fix_first_last(w, links, first, last);
+    return SSE_OK;
+}
+
+int sse_phase_times(const sse_batch *b, double *ms3)
+{
+    if (!b || !ms3)
+        return fail(SSE_ERR_INVALID, "bad arguments");
+    for (int i = 0; i < 3; ++i)
+        ms3[i] = b->phase_ms[i];
+    return SSE_OK;
+}
+
+int sse_work_counters(sse_batch *b, int64_t *L_sum, int64_t *n_sum, int64_t *W_sum,
+                      int64_t *walker_sweeps)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    std::vector<DevWalker> h(b->n_walkers);
+    HIPCHECK(hipMemcpy(h.data(), b->d_walkers, sizeof(DevWalker) * b->n_walkers,
+                       hipMemcpyDeviceToHost));
+    int64_t a = 0, c = 0, d = 0, e = 0;
+    for (auto &w : h) {
+        a += w.acc_L;
+        c += w.acc_n;
+        d += w.acc_W;
+        e += w.acc_sweeps;
+    }
+    if (L_sum)
+        *L_sum = a;
+    if (n_sum)
+        *n_sum = c;
+    if (W_sum)
+        *W_sum = d;
+    if (walker_sweeps)
+        *walker_sweeps = e;
+    return SSE_OK;
+}
+
+}  // extern "C"

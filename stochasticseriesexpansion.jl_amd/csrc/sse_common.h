@@ -1,0 +1,250 @@
+// sse_common.h -- device-side data layout and helpers of the SSE sweep.
+//
+// Layout in HBM (one allocation per batch unless noted):
+//   DevModel[n_models]      model graph + per-bond-type VertexData tables
+//   DevWalker[n_walkers]    per-walker scalars and slab pointers
+//   slab pool               per walker one slab of  ops[cap] (u32) followed by
+//                           links[4*cap] (i32), carved from a device bump
+//                           allocator so the cutoff L grows on device
+//   spins/first/last        [n_walkers][n_sites] u8 / i32 / i32
+//
+// Operator code (u32): 0 = identity, else 1 | vcode<<1 | bond<<11,
+// vcode = diag | vidx<<1 (vidx 0-based).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SSE_WAVE 64
+
+struct DevVD {
+    int32_t dims0, dims1, nv, max_worm, n_exits, pad;
+    double energy_offset;
+    const double *weights;     // [nv]
+    const int8_t *signs;       // [nv]
+    const uint8_t *leg_states; // [nv][4]
+    const int32_t *diag_vertex;// [dims0*dims1]
+    const int32_t *toff;       // [nv][max_worm][4]
+    const int32_t *tcnt;       // [nv][max_worm][4]
+    const double *cum;         // [n_exits]
+    const int32_t *tgt;        // [n_exits]
+    const uint8_t *eleg;       // [n_exits]
+    const uint8_t *ewrm;       // [n_exits]
+};
+
+struct DevModel {
+    int32_t n_sites, n_bonds, n_types, n_est, est_max_dim, max_dim;
+    double energy_offset, norm;
+    const uint8_t *site_dims;  // [n_sites]
+    const int32_t *bond_type;  // [n_bonds]
+    const int32_t *bond_sites; // [n_bonds][2]
+    const DevVD *types;        // [n_types]
+    const double *est_coef;    // [n_est][n_sites]
+    const double *est_mval;    // [n_est][n_sites][est_max_dim]
+    // packed per-bond record for the kernels: sites and type in one int2
+    const int2 *bond_rec;      // [n_bonds] (site0 | type<<20, site1)
+};
+
+struct DevWalker {
+    // parameters
+    int32_t model, warmup;
+    double T, target_wlf, atten;
+    uint64_t key;
+    int64_t thermalization, init_cutoff;
+    // chain state
+    double avg_worm_length, num_worms;
+    int64_t L, n, cap;
+    uint64_t ctr;
+    int64_t sweeps, measured;
+    // diagnostics of the last sweep
+    int64_t last_scatter;
+    double last_total, last_wlf;
+    int32_t error, n_sites;
+    // buffers
+    uint32_t *ops;    // [cap]
+    int32_t *links;   // [4*cap]
+    uint8_t *spins;   // [n_sites]
+    int32_t *first;   // [n_sites]
+    int32_t *last;    // [n_sites]
+    // work counters (sum over sweeps since reset)
+    int64_t acc_L, acc_n, acc_W, acc_sweeps;
+    int32_t meas_offset, meas_width;
+};
+
+struct DevPool {
+    unsigned long long next;  // bytes used
+    unsigned long long size;
+    char *base;
+    int32_t overflow;
+    int32_t status;           // bit (1 << code) per walker error code seen
+};
+
+__device__ __forceinline__ void flag_error(DevPool *pool, int code)
+{
+    atomicOr(&pool->status, 1 << code);
+}
+
+#define SSE_OP_BOND_SHIFT_D 11
+__device__ __forceinline__ uint32_t op_bond(uint32_t op) { return op >> SSE_OP_BOND_SHIFT_D; }
+__device__ __forceinline__ uint32_t op_vcode(uint32_t op) { return (op >> 1) & 1023u; }
+__device__ __forceinline__ uint32_t op_make(uint32_t bond, uint32_t vcode)
+{
+    return 1u | (vcode << 1) | (bond << SSE_OP_BOND_SHIFT_D);
+}
+
+// ---------------------------------------------------------------- Philox4x32-10
+// Counter-based generator of Salmon et al. (SC'11).  64-bit word i of a
+// walker's stream with key k: block b = i/2 with counter (lo32 b, hi32 b, 0, 0)
+// and key (lo32 k, hi32 k); word = out0|out1<<32 for even i, out2|out3<<32 odd.
+__device__ __forceinline__ void philox_block(uint64_t b, uint64_t key, uint32_t &o0, uint32_t &o1,
+                                             uint32_t &o2, uint32_t &o3)
+{
+    uint32_t c0 = (uint32_t)b, c1 = (uint32_t)(b >> 32), c2 = 0u, c3 = 0u;
+    uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    o0 = c0; o1 = c1; o2 = c2; o3 = c3;
+}
+
+__device__ __forceinline__ double word_to_float(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
+
+// Per-lane sequential stream with a one-block cache.
+struct LaneRng {
+    uint64_t key, ctr, blk;
+    uint32_t o0, o1, o2, o3;
+    __device__ void init(uint64_t k, uint64_t c)
+    {
+        key = k;
+        ctr = c;
+        blk = ~0ull;
+    }
+    __device__ __forceinline__ uint64_t next()
+    {
+        uint64_t b = ctr >> 1;
+        if (b != blk) {
+            philox_block(b, key, o0, o1, o2, o3);
+            blk = b;
+        }
+        uint64_t w = (ctr & 1) ? ((uint64_t)o2 | ((uint64_t)o3 << 32))
+                               : ((uint64_t)o0 | ((uint64_t)o1 << 32));
+        ++ctr;
+        return w;
+    }
+    __device__ __forceinline__ double next_float() { return word_to_float(next()); }
+    // rand(1:s) - 1, Lemire's nearly divisionless method on 64-bit words
+    __device__ __forceinline__ uint64_t range(uint64_t s)
+    {
+        uint64_t x = next();
+        uint64_t l = x * s;
+        uint64_t h = __umul64hi(x, s);
+        if (l < s) {
+            uint64_t t = (0ull - s) % s;
+            while (l < t) {
+                x = next();
+                l = x * s;
+                h = __umul64hi(x, s);
+            }
+        }
+        return h;
+    }
+};
+
+// ---------------------------------------------------------------- tanh
+// Worm-number controller tanh.  Evaluated by exactly this sequence of IEEE
+// operations (no contraction): Cody-Waite reduction x = k ln2 + r and the
+// degree-16 Taylor polynomial of expm1(r); tanh(x) = e/(e+2), e = expm1(2|x|).
+__device__ __forceinline__ double sse_expm1(double x)
+{
+    const double C[16] = {
+        0x1.0000000000000p+0, 0x1.0000000000000p-1, 0x1.5555555555555p-3, 0x1.5555555555555p-5,
+        0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13, 0x1.a01a01a01a01ap-16,
+        0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26, 0x1.1eed8eff8d898p-29,
+        0x1.6124613a86d09p-33, 0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-41, 0x1.ae7f3e733b81fp-45};
+    if (x == 0.0)
+        return x;
+    double kf = rint(x * 0x1.71547652b82fep+0);
+    double r = (x - kf * 0x1.62e42fee00000p-1) - kf * 0x1.a39ef35793c76p-33;
+    double p = C[15];
+#pragma unroll
+    for (int j = 14; j >= 0; --j)
+        p = p * r + C[j];
+    double em1 = p * r;
+    int k = (int)kf;
+    if (k == 0)
+        return em1;
+    double two_k = ldexp(1.0, k);
+    return two_k * em1 + (two_k - 1.0);
+}
+
+__device__ __forceinline__ double sse_tanh(double x)
+{
+    double ax = fabs(x);
+    if (!(ax < 22.0))
+        return copysign(1.0, x);
+    if (ax < 0x1p-28)
+        return x;
+    double e = sse_expm1(2.0 * ax);
+    double t = e / (e + 2.0);
+    return copysign(t, x);
+}
+
+// Worm-number controller and WormLengthFraction (sse.jl:200-217).
+__device__ __forceinline__ void worm_controller(DevWalker &w, double total, bool thermalized)
+{
+    w.last_total = total;
+    w.last_wlf = __longlong_as_double(0x7ff8000000000000ll);
+    if (thermalized && w.n != 0)
+        w.last_wlf = total / (double)w.n;
+    double avg = total / ceil(w.num_worms);
+    if (!thermalized) {
+        w.avg_worm_length += w.atten * (avg - w.avg_worm_length);
+        double target = w.target_wlf * (double)w.n / w.avg_worm_length;
+        w.num_worms += w.atten * (target - w.num_worms + 100.0 * sse_tanh(target - w.num_worms));
+        if (w.atten != 0.0) {
+            double hi = 1.0 + (double)w.n / 2.0;
+            double x = w.num_worms;
+            w.num_worms = x > hi ? hi : (x < 1.0 ? 1.0 : x);
+        }
+    }
+}
+
+// Growth rule of the cutoff (sse.jl:138-145): n >= L/2 -> L = floor(1.5 L + 100).
+__device__ __forceinline__ int64_t grown_cutoff(int64_t L) { return (3 * L) / 2 + 100; }
+
+// Capacity of a fresh slab for cutoff L: two further growth steps of headroom.
+__device__ __host__ __forceinline__ int64_t slab_capacity(int64_t L)
+{
+    int64_t c = L < 64 ? 64 : L;
+    c = (3 * c) / 2 + 100;
+    c = (3 * c) / 2 + 100;
+    return (c + 63) & ~(int64_t)63;
+}
+
+__device__ __host__ __forceinline__ uint64_t slab_bytes(int64_t cap)
+{
+    uint64_t b = (uint64_t)cap * 4u + (uint64_t)cap * 16u;
+    return (b + 255u) & ~(uint64_t)255u;
+}
+
+// Bump allocation from the batch's slab pool; returns nullptr on exhaustion.
+__device__ __forceinline__ char *pool_alloc(DevPool *pool, uint64_t bytes)
+{
+    unsigned long long off = atomicAdd(&pool->next, (unsigned long long)bytes);
+    if (off + bytes > pool->size) {
+        atomicExch(&pool->overflow, 1);
+        return nullptr;
+    }
+    return pool->base + off;
+}

@@ -1,0 +1,367 @@
+// sse_lane.h -- one lane per walker: the sweep in the reference's order, slot
+// by slot (diagonal update sse.jl:137-191, make_vertex_list! vertex_list.jl:15-54,
+// worm_update sse.jl:193-303, measure sse.jl:70-87,305-376).  This is the
+// plain GPU form: launched with one walker it is the one-walker-per-launch
+// baseline; launched over a batch every lane runs its own chain.
+#pragma once
+#include "sse_common.h"
+#include "../../include/ssegpu.h"
+
+// Grow the cutoff if n >= L/2, reallocating the slab from the pool when the
+// capacity is exceeded.  Returns false (walker untouched) on pool exhaustion.
+__device__ bool lane_grow(DevWalker &w, DevPool *pool)
+{
+    if ((double)w.n >= (double)w.L * 0.5) {
+        int64_t nl = grown_cutoff(w.L);
+        if (nl > w.cap) {
+            int64_t cap = slab_capacity(nl);
+            char *s = pool_alloc(pool, slab_bytes(cap));
+            if (!s) {
+                w.error = SSE_ERR_OOM;
+                return false;
+            }
+            uint32_t *nops = (uint32_t *)s;
+            for (int64_t p = 0; p < w.L; ++p)
+                nops[p] = w.ops[p];
+            w.ops = nops;
+            w.links = (int32_t *)(s + (size_t)cap * 4);
+            w.cap = cap;
+        }
+        for (int64_t p = w.L; p < nl; ++p)
+            w.ops[p] = 0u;
+        w.L = nl;
+    }
+    return true;
+}
+
+__device__ void lane_diagonal(DevWalker &w, const DevModel &m, LaneRng &rng)
+{
+    const double pm_raw = (double)m.n_bonds / w.T;
+    const double pr_raw = w.T / (double)m.n_bonds;
+    uint32_t *ops = w.ops;
+    uint8_t *spins = w.spins;
+    int64_t n = w.n;
+    const int64_t L = w.L;
+    for (int64_t p = 0; p < L; ++p) {
+        uint32_t op = ops[p];
+        if (op == 0u) {
+            uint32_t b = (uint32_t)rng.range((uint64_t)m.n_bonds);
+            int2 br = m.bond_rec[b];
+            const DevVD &vd = m.types[br.x >> 20];
+            int s0 = br.x & 0xFFFFF, s1 = br.y;
+            int idx = (spins[s0] - 1) + vd.dims0 * (spins[s1] - 1);
+            int32_t dv = vd.diag_vertex[idx];
+            double weight = dv < 0 ? 0.0 : vd.weights[dv >> 1];
+            double p_make = pm_raw / (double)(L - n);
+            if (rng.next_float() < p_make * weight) {
+                ops[p] = op_make(b, (uint32_t)dv);
+                n += 1;
+            }
+        } else {
+            uint32_t b = op_bond(op);
+            int2 br = m.bond_rec[b];
+            const DevVD &vd = m.types[br.x >> 20];
+            uint32_t vc = op_vcode(op);
+            if (vc & 1u) {
+                double weight = vd.weights[vc >> 1];
+                double p_remove = (double)(L - n + 1) * pr_raw;
+                if (rng.next_float() * weight < p_remove) {
+                    ops[p] = 0u;
+                    n -= 1;
+                }
+            } else {
+                const uint8_t *ls = vd.leg_states + 4 * (vc >> 1);
+                spins[br.x & 0xFFFFF] = ls[2];
+                spins[br.y] = ls[3];
+            }
+        }
+    }
+    w.n = n;
+}
+
+__device__ void lane_vertex_list(DevWalker &w, const DevModel &m)
+{
+    int32_t *links = w.links;
+    const uint32_t *ops = w.ops;
+    for (int64_t k = 0; k < 4 * w.L; ++k)
+        links[k] = -1;
+    for (int i = 0; i < m.n_sites; ++i)
+        w.first[i] = w.last[i] = -1;
+    for (int64_t p = 0; p < w.L; ++p) {
+        uint32_t op = ops[p];
+        if (op == 0u)
+            continue;
+        int2 br = m.bond_rec[op_bond(op)];
+        int sites[2] = {br.x & 0xFFFFF, br.y};
+        for (int s = 0; s < 2; ++s) {
+            int site = sites[s];
+            int32_t lst = w.last[site];
+            int32_t me = (int32_t)(4 * p + s);
+            if (lst != -1) {
+                links[lst] = me;
+                links[me] = lst;
+            } else {
+                w.first[site] = me;
+            }
+            w.last[site] = (int32_t)(4 * p + 2 + s);
+        }
+    }
+    for (int i = 0; i < m.n_sites; ++i) {
+        int32_t f = w.first[i];
+        if (f != -1) {
+            links[f] = w.last[i];
+            links[w.last[i]] = f;
+        }
+    }
+}
+
+// scatter (vertex_data.jl:106-125); returns false on fall-through
+__device__ __forceinline__ bool lane_scatter(const DevVD &vd, uint32_t vcode, int leg_in, int worm_in,
+                                             double r, int &leg_out, int &worm_out, uint32_t &nv)
+{
+    int k = ((int)(vcode >> 1) * vd.max_worm + (worm_in - 1)) * 4 + leg_in;
+    int32_t off = vd.toff[k];
+    int32_t cnt = vd.tcnt[k];
+    if (off < 0)
+        return false;
+    for (int e = off; e < off + cnt; ++e) {
+        if (r < vd.cum[e]) {
+            leg_out = vd.eleg[e];
+            worm_out = vd.ewrm[e];
+            nv = (uint32_t)vd.tgt[e];
+            return true;
+        }
+    }
+    return false;
+}
+
+__device__ int64_t lane_worm_traverse(DevWalker &w, const DevModel &m, LaneRng &rng, int64_t &scat)
+{
+    if (w.n == 0)
+        return 0;
+    int64_t p0;
+    int l0;
+    for (;;) {
+        p0 = (int64_t)rng.range((uint64_t)w.L);
+        l0 = (int)rng.range(4);
+        if (w.links[4 * p0 + l0] >= 0)
+            break;
+    }
+    int2 b0 = m.bond_rec[op_bond(w.ops[p0])];
+    int site0 = (l0 & 1) ? b0.y : (b0.x & 0xFFFFF);
+    int wf0 = 1 + (int)rng.range((uint64_t)(m.site_dims[site0] - 1));
+    int leg_in = l0, wf = wf0;
+    int64_t p = p0, len = 1;
+    for (;;) {
+        uint32_t op = w.ops[p];
+        uint32_t b = op_bond(op);
+        int2 br = m.bond_rec[b];
+        const DevVD &vd = m.types[br.x >> 20];
+        int leg_out, wf_out;
+        uint32_t nv;
+        double r = rng.next_float();
+        scat += 1;
+        if (!lane_scatter(vd, op_vcode(op), leg_in, wf, r, leg_out, wf_out, nv)) {
+            w.error = SSE_ERR_SCATTER;
+            return len;
+        }
+        w.ops[p] = op_make(b, nv);
+        int site_out = (leg_out & 1) ? br.y : (br.x & 0xFFFFF);
+        int dim_out = m.site_dims[site_out];
+        if (p == p0 && leg_out == l0 && wf_out == dim_out - wf0)
+            break;
+        len += 1;
+        wf = wf_out;
+        int32_t nx = w.links[4 * p + leg_out];
+        leg_in = nx & 3;
+        p = nx >> 2;
+        if (p == p0 && leg_in == l0 && wf == wf0)
+            break;
+    }
+    return len;
+}
+
+__device__ void lane_worm_update(DevWalker &w, const DevModel &m, LaneRng &rng, bool therm)
+{
+    double total = 1.0;
+    int64_t nworms = (int64_t)ceil(w.num_worms);
+    int64_t scat = 0;
+    for (int64_t k = 0; k < nworms && !w.error; ++k)
+        total += (double)lane_worm_traverse(w, m, rng, scat);
+    w.last_scatter = scat;
+    worm_controller(w, total, therm);
+    for (int i = 0; i < m.n_sites; ++i) {
+        int32_t f = w.first[i];
+        if (f < 0) {
+            w.spins[i] = (uint8_t)(1 + rng.range(m.site_dims[i]));
+        } else {
+            uint32_t op = w.ops[f >> 2];
+            const DevVD &vd = m.types[m.bond_rec[op_bond(op)].x >> 20];
+            w.spins[i] = vd.leg_states[4 * (op_vcode(op) >> 1) + (f & 3)];
+        }
+    }
+}
+
+// phases: 1 diagonal, 2 vertex list, 4 worm; phases==7 is a full sweep and
+// advances the walker's sweep counter.  Walkers with an error flag are skipped.
+__global__ void lane_sweep_kernel(DevWalker *W, const DevModel *models, DevPool *pool, int w_begin,
+                                  int w_end, int phases)
+{
+    int wi = w_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (wi >= w_end)
+        return;
+    DevWalker w = W[wi];
+    if (w.error)
+        return;
+    const DevModel &m = models[w.model];
+    bool therm = w.sweeps > w.thermalization;
+    LaneRng rng;
+    rng.init(w.key, w.ctr);
+    if (phases & 1) {
+        if (!lane_grow(w, pool)) {
+            W[wi].error = w.error;
+            flag_error(pool, w.error);
+            return;
+        }
+        lane_diagonal(w, m, rng);
+    }
+    if (phases & 2)
+        lane_vertex_list(w, m);
+    if (phases & 4)
+        lane_worm_update(w, m, rng, therm);
+    w.ctr = rng.ctr;
+    if (w.error)
+        flag_error(pool, w.error);
+    if (phases == 7) {
+        w.sweeps += 1;
+        w.acc_L += w.L;
+        w.acc_n += w.n;
+        w.acc_W += w.last_scatter;
+        w.acc_sweeps += 1;
+    }
+    W[wi] = w;
+}
+
+// Carlo.init! (sse.jl:47-60): spins, identity opstring, diagonal warm-up.
+__global__ void lane_init_kernel(DevWalker *W, const DevModel *models, DevPool *pool, int w_begin,
+                                 int w_end)
+{
+    int wi = w_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (wi >= w_end)
+        return;
+    DevWalker w = W[wi];
+    const DevModel &m = models[w.model];
+    LaneRng rng;
+    rng.init(w.key, 0);
+    for (int i = 0; i < m.n_sites; ++i)
+        w.spins[i] = (uint8_t)(1 + rng.range(m.site_dims[i]));
+    for (int64_t p = 0; p < w.L; ++p)
+        w.ops[p] = 0u;
+    w.n = 0;
+    for (int k = 0; k < w.warmup; ++k) {
+        if (!lane_grow(w, pool)) {
+            flag_error(pool, w.error);
+            break;
+        }
+        lane_diagonal(w, m, rng);
+    }
+    w.ctr = rng.ctr;
+    W[wi] = w;
+}
+
+// Carlo.measure! for one walker, sequentially (sse.jl:70-87,305-376).
+__device__ void measure_walker_serial(const DevWalker &w, const DevModel &m, double *out, uint8_t *st)
+{
+    int neg = 0;
+    for (int64_t p = 0; p < w.L; ++p) {
+        uint32_t op = w.ops[p];
+        if (op != 0u)
+            neg += m.types[m.bond_rec[op_bond(op)].x >> 20].signs[op_vcode(op) >> 1] < 0;
+    }
+    double sign = (neg & 1) ? -1.0 : 1.0;
+    double nn = (double)w.n;
+    double Nn = m.norm;
+    out[0] = sign;
+    out[1] = nn;
+    out[2] = sign * nn;
+    out[3] = sign * (nn * nn);
+    out[4] = ((-sign) * (nn * w.T + m.energy_offset)) / Nn;
+    const int N = m.n_sites, D = m.est_max_dim;
+    for (int e = 0; e < m.n_est; ++e) {
+        const double *coef = m.est_coef + (size_t)e * N;
+        const double *mv = m.est_mval + (size_t)e * N * D;
+        for (int i = 0; i < N; ++i)
+            st[i] = w.spins[i];
+        double tmp = 0.0;
+        for (int i = 0; i < N; ++i)
+            tmp += coef[i] * mv[i * D + st[i] - 1];
+        double mag = tmp, absmag = fabs(tmp), t2 = tmp * tmp, mag2 = t2, mag4 = t2 * t2, en = 1.0;
+        for (int64_t p = 0; p < w.L; ++p) {
+            uint32_t op = w.ops[p];
+            if (op == 0u)
+                continue;
+            uint32_t vc = op_vcode(op);
+            if (!(vc & 1u)) {
+                int2 br = m.bond_rec[op_bond(op)];
+                const uint8_t *ls = m.types[br.x >> 20].leg_states + 4 * (vc >> 1);
+                int sites[2] = {br.x & 0xFFFFF, br.y};
+                for (int l = 0; l < 2; ++l) {
+                    int site = sites[l];
+                    tmp += coef[site] * (mv[site * D + ls[2 + l] - 1] - mv[site * D + ls[l] - 1]);
+                    st[site] = ls[2 + l];
+                }
+            }
+            mag += tmp;
+            absmag += fabs(tmp);
+            t2 = tmp * tmp;
+            mag2 += t2;
+            mag4 += t2 * t2;
+            en += 1.0;
+        }
+        double norm = 1.0 / Nn;
+        mag *= norm;
+        absmag *= norm;
+        mag2 *= norm * norm;
+        mag4 *= (norm * norm) * (norm * norm);
+        double *o = out + 5 + 5 * e;
+        o[0] = sign * mag / en;
+        o[1] = sign * absmag / en;
+        o[2] = sign * mag2 / en;
+        o[3] = sign * mag4 / en;
+        double chi = 1.0 / w.T / (en + 1.0) / en * (mag * mag + mag2) * Nn;
+        o[4] = sign * chi;
+    }
+}
+
+// Accumulate one measurement into the per-walker sums.
+__device__ __forceinline__ void accumulate(double *sum, double *sumsq, const double *x, int width)
+{
+    for (int j = 0; j < width; ++j) {
+        sum[j] += x[j];
+        sumsq[j] += x[j] * x[j];
+    }
+}
+
+// mode 0: measure every walker (explicit Carlo.measure! call)
+// mode 1: measure thermalized walkers (Carlo measures after a sweep once
+//         sweeps > thermalization) and accumulate into sums
+__global__ void lane_measure_kernel(DevWalker *W, const DevModel *models, double *out, double *sum,
+                                    double *sumsq, int64_t *count, uint8_t *scratch_spins,
+                                    int max_sites, int n_walkers, int mode)
+{
+    int wi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (wi >= n_walkers)
+        return;
+    const DevWalker &w = W[wi];
+    if (w.error)
+        return;
+    if (mode == 1 && !(w.sweeps > w.thermalization))
+        return;
+    const DevModel &m = models[w.model];
+    double *o = out + w.meas_offset;
+    measure_walker_serial(w, m, o, scratch_spins + (size_t)wi * max_sites);
+    if (mode == 1) {
+        accumulate(sum + w.meas_offset, sumsq + w.meas_offset, o, w.meas_width);
+        count[wi] += 1;
+    }
+}

@@ -1,0 +1,33 @@
+// sse_wave.h -- one wavefront per walker (placeholder: delegates to the lane kernels)
+#pragma once
+#include <vector>
+#include "sse_common.h"
+#include "sse_lane.h"
+
+struct WaveConfig {
+    int dummy;
+};
+
+static inline int wave_configure(WaveConfig &, const std::vector<DevModel> &, int) { return 0; }
+
+static inline int wave_launch_sweep(WaveConfig &, hipStream_t s, DevWalker *W, const DevModel *M,
+                                    DevPool *pool, int n, int phases, hipEvent_t *)
+{
+    hipLaunchKernelGGL(lane_sweep_kernel, dim3((n + 63) / 64), dim3(64), 0, s, W, M, pool, 0, n, phases);
+    return 0;
+}
+static inline int wave_launch_init(WaveConfig &, hipStream_t s, DevWalker *W, const DevModel *M,
+                                   DevPool *pool, int n)
+{
+    hipLaunchKernelGGL(lane_init_kernel, dim3((n + 63) / 64), dim3(64), 0, s, W, M, pool, 0, n);
+    return 0;
+}
+static inline int wave_launch_measure(WaveConfig &, hipStream_t s, DevWalker *W, const DevModel *M,
+                                      double *out, double *sum, double *sumsq, int64_t *count, int n,
+                                      int mode)
+{
+    (void)s; (void)W; (void)M; (void)out; (void)sum; (void)sumsq; (void)count; (void)n; (void)mode;
+    return 1;
+}
+static inline int wave_dense_vertex_list(WaveConfig &, const DevWalker &, int32_t *, int64_t) { return 1; }
+static inline void wave_fix_first_last(const DevWalker &, int32_t *, int32_t *, int32_t *) {}

@@ -1,0 +1,292 @@
+"""VertexData: vertices and abstract-loop transition tables of one bond type.
+
+Host-side restatement of src/vertex_data.jl (StochasticSeriesExpansion.jl).
+Conventions follow the reference: vertex indices and states are 1-based,
+a vertex code is ``diag | idx << 1`` (opercode.jl:18-21) and ``None`` is the
+invalid vertex (opercode.jl:16).  The transition LP (vertex_data.jl:213-416)
+is solved with the HiGHS dual simplex shipped in scipy.  When an LP optimum
+is degenerate HiGHS may return a different optimal vertex than the
+reference's HiGHS.jl build; every returned table is checked for detailed
+balance and normalisation by the tests instead.
+"""
+from __future__ import annotations
+
+import warnings
+
+import numpy as np
+from scipy.optimize import linprog
+
+from .util import join_idx, site_of_leg, split_idx, worm_action, worm_count, worm_inverse
+
+
+def vertex_code(diagonal, idx):
+    """VertexCode(diagonal, idx) with 1-based idx (opercode.jl:18-21)."""
+    return int(bool(diagonal)) | (int(idx) << 1)
+
+
+def vcode_is_diagonal(v):
+    return bool(v & 1)
+
+
+def vcode_idx(v):
+    return v >> 1
+
+
+def calc_energy_offset(H, energy_offset_factor):
+    """-(hmax + (hmax-hmin)*factor) over the diagonal (vertex_data.jl:129-135)."""
+    d = np.diag(H)
+    hmin = d.min()
+    hmax = d.max()
+    epsilon = (hmax - hmin) * energy_offset_factor
+    return -(hmax + epsilon)
+
+
+def construct_vertices(dims, H, energy_offset, tolerance):
+    """Vertices = matrix elements with |w| > tolerance, w = -H (minus the offset
+    on the diagonal), in column-major order (vertex_data.jl:137-169)."""
+    n = H.shape[0]
+    rdims = tuple(reversed(dims))
+    diagonal_vertices = [None] * n
+    weights, signs, leg_states = [], [], []
+    for j in range(1, n + 1):          # CartesianIndices: first index fastest
+        for i in range(1, n + 1):
+            w = -H[i - 1, j - 1]
+            if i == j:
+                w -= energy_offset
+            if abs(w) > tolerance:
+                si = tuple(reversed(split_idx(rdims, i)))
+                sj = tuple(reversed(split_idx(rdims, j)))
+                if i == j:
+                    reversed_i = join_idx(dims, si)
+                    diagonal_vertices[reversed_i - 1] = vertex_code(True, len(weights) + 1)
+                leg_states.append(si + sj)
+                weights.append(abs(w))
+                signs.append(1 if w >= 0 else -1)
+    ls = np.array(leg_states, dtype=np.uint8).reshape(len(weights), 2 * len(dims)).T.copy()
+    return diagonal_vertices, np.array(weights, dtype=np.float64), ls, np.array(signs, dtype=np.int8)
+
+
+def wrap_vertex_idx(leg_states, vertex_idx):
+    """VertexCode with the diagonal flag computed from the leg states (vertex_data.jl:171-187)."""
+    if vertex_idx is None:
+        return None
+    leg_count = leg_states.shape[0]
+    ls = leg_states[:, vertex_idx - 1]
+    diagonal = all(ls[s] == ls[leg_count // 2 + s] for s in range(leg_count // 2))
+    return vertex_code(diagonal, vertex_idx)
+
+
+class _LegStateIndex:
+    """findfirst over leg-state columns (vertex_data.jl:210) as a dict lookup."""
+
+    def __init__(self, leg_states):
+        self.map = {}
+        for v in range(leg_states.shape[1]):
+            key = tuple(int(x) for x in leg_states[:, v])
+            self.map.setdefault(key, v + 1)
+
+    def find(self, state):
+        return self.map.get(tuple(int(x) for x in state))
+
+
+def vertex_apply_change(leg_states, dims, vertex, step_in, step_out, index=None):
+    """Vertex reached by applying step_in then step_out to `vertex`, or None
+    (vertex_data.jl:192-211)."""
+    ns = len(dims)
+    new = [int(x) for x in leg_states[:, vertex - 1]]
+    (leg_in, worm_in) = step_in
+    (leg_out, worm_out) = step_out
+    dim_in = dims[site_of_leg(leg_in, ns) - 1]
+    dim_out = dims[site_of_leg(leg_out, ns) - 1]
+    new[leg_in - 1] = worm_action(worm_in, new[leg_in - 1], dim_in)
+    new[leg_out - 1] = worm_action(worm_out, new[leg_out - 1], dim_out)
+    if index is None:
+        index = _LegStateIndex(leg_states)
+    return index.find(new)
+
+
+def _solve_lp(cost, A, b, lp_tolerance):
+    res = linprog(cost, A_eq=A, b_eq=b, bounds=[(0, None)] * len(cost), method="highs-ds",
+                  options={"primal_feasibility_tolerance": lp_tolerance,
+                           "dual_feasibility_tolerance": lp_tolerance})
+    if res.status != 0:
+        raise RuntimeError(f"transition probability optimization failed: {res.message}")
+    return res.x
+
+
+def construct_transitions(weights, leg_states, max_worm_count, dims, tolerance, lp_tolerance):
+    """Transition tables from one LP per class of vertices connected by a
+    single worm step (vertex_data.jl:213-416).
+
+    Returns (transitions, cumprobs, targets, step_outs) where transitions maps
+    (leg_in, worm_in, vertex) -> (offset, length) with a 1-based offset and
+    length = number of exits - 1, as the reference's Transition."""
+    ns = len(dims)
+    leg_count = 2 * ns
+    transitions = {}
+    cumprobs, targets, step_outs = [], [], []
+
+    steps = []
+    inv_steps = {}
+    step_idx = {}
+    for worm in range(1, max_worm_count + 1):
+        for leg in range(1, leg_count + 1):
+            dim = dims[site_of_leg(leg, ns) - 1]
+            if worm <= worm_count(dim):
+                steps.append((leg, worm))
+                step_idx[(leg, worm)] = len(steps)
+                inv_steps[(leg, worm)] = (leg, worm_inverse(worm, dim))
+
+    def inverse(vc):
+        return (inv_steps[vc[1]], inv_steps[vc[0]])
+
+    variables = []
+    var_set = set()
+    for s_in in steps:
+        for s_out in steps:
+            vc = (s_in, s_out)
+            if inverse(vc) not in var_set:
+                variables.append(vc)
+                var_set.add(vc)
+    # variable of (in, out): findfirst over variables equal to it or whose
+    # inverse is (vertex_data.jl:359-365)
+    var_lookup = {}
+    for s_in in steps:
+        for s_out in steps:
+            key = (s_in, s_out)
+            cands = [k for k, vc in enumerate(variables) if vc == key or inverse(vc) == key]
+            var_lookup[key] = cands[0]
+
+    cost = np.array([1.0 if vc == inverse(vc) else 0.0 for vc in variables])
+    A = np.zeros((len(steps), len(variables)))
+    for r, step in enumerate(steps):
+        for k, vc in enumerate(variables):
+            if vc[0] == step or inverse(vc)[0] == step:
+                A[r, k] = 1.0
+
+    index = _LegStateIndex(leg_states)
+    nv = len(weights)
+    used_inaccurate = False
+    while True:
+        v = None
+        step_in = None
+        for empty_v in range(1, nv + 1):
+            found = None
+            for s in steps:
+                if (s[0], s[1], empty_v) not in transitions:
+                    found = s
+                    break
+            if found is not None:
+                v = empty_v
+                step_in = inv_steps[found]
+                break
+        if v is None:
+            break
+
+        tmap = {}
+        constraints = np.zeros(len(steps))
+        for s_out in steps:
+            target = vertex_apply_change(leg_states, dims, v, step_in, s_out, index)
+            tmap[s_out] = target
+            constraints[step_idx[s_out] - 1] = weights[target - 1] if target is not None else 0.0
+
+        solution = _solve_lp(cost, A, constraints, lp_tolerance)
+
+        for s_in in steps:
+            if tmap[s_in] is None:
+                continue
+            c = constraints[step_idx[s_in] - 1]
+            norm = 1.0 if c == 0 else c
+            offset = len(cumprobs) + 1
+            length = -1
+            for s_out in steps:
+                var = var_lookup[(s_in, s_out)]
+                x = solution[var]
+                prob = x / norm
+                if not x >= -lp_tolerance:
+                    raise AssertionError(f"LP solution below bound: {x}")
+                if prob < 0.0:
+                    used_inaccurate = prob < -tolerance
+                    prob = 0.0
+                if not x <= norm + lp_tolerance:
+                    raise AssertionError(f"LP solution above bound: {x} > {norm}")
+                if prob > 1.0:
+                    used_inaccurate = prob > 1 + tolerance
+                    prob = 1.0
+                if prob > tolerance / len(steps):
+                    cumprobs.append(prob)
+                    vc = variables[var]
+                    in_inv = inverse(vc)[0] if vc[0] == s_in else vc[0]
+                    t = wrap_vertex_idx(leg_states, tmap[in_inv])
+                    if t is None:
+                        raise AssertionError("transition to invalid vertex")
+                    targets.append(t)
+                    step_outs.append(s_out)
+                    length += 1
+            transitions[(s_in[0], s_in[1], tmap[s_in])] = (offset, length)
+            acc = 0.0
+            for k in range(offset - 1, offset + length):
+                acc = acc + cumprobs[k]
+                cumprobs[k] = acc
+            if length >= 0 and abs(cumprobs[offset + length - 1] - 1) > tolerance:
+                warnings.warn(f"normalization error: {cumprobs[offset + length - 1]} != 1")
+
+    if used_inaccurate:
+        warnings.warn("had to truncate some probabilities in a possibly inaccurate way!")
+    return transitions, cumprobs, targets, step_outs
+
+
+class VertexData:
+    """Probability tables for one bond type (vertex_data.jl:13-88).
+
+    Fields mirror the reference: energy_offset, dims, diagonal_vertices
+    (list indexed by compound state - 1, entries vertex codes or None),
+    signs, weights, transitions {(leg, worm, v): (offset, length)},
+    transition_cumprobs, transition_targets, transition_step_outs and
+    leg_states (2*NSites x n_vertices, 1-based states)."""
+
+    def __init__(self, dims, bond_hamiltonian, energy_offset_factor=0.25, tolerance=1e-7,
+                 lp_tolerance=1e-10):
+        H = np.asarray(bond_hamiltonian, dtype=np.float64)
+        dims = tuple(int(d) for d in dims)
+        total_dim = int(np.prod(dims))
+        assert len(dims) >= 1
+        assert total_dim == H.shape[0] == H.shape[1]
+        self.dims = dims
+        self.energy_offset = calc_energy_offset(H, energy_offset_factor)
+        self.max_worm_count = max(worm_count(d) for d in dims)
+        (self.diagonal_vertices, self.weights, self.leg_states, self.signs) = \
+            construct_vertices(dims, H, self.energy_offset, tolerance)
+        assert self.leg_states.shape == (2 * len(dims), len(self.weights))
+        (self.transitions, self.transition_cumprobs, self.transition_targets,
+         self.transition_step_outs) = construct_transitions(
+            self.weights, self.leg_states, self.max_worm_count, dims, tolerance, lp_tolerance)
+
+    # accessors (vertex_data.jl:91-104)
+    def vertex_count(self):
+        return len(self.weights)
+
+    def get_diagonal_vertex(self, compound_state_idx):
+        return self.diagonal_vertices[compound_state_idx - 1]
+
+    def get_vertex_weight(self, v):
+        return 0.0 if v is None else float(self.weights[vcode_idx(v) - 1])
+
+    def get_sign(self, v):
+        return int(self.signs[vcode_idx(v) - 1])
+
+    def get_leg_state(self, v):
+        return self.leg_states[:, vcode_idx(v) - 1]
+
+    def get_transition(self, leg_in, worm_in, vidx):
+        return self.transitions.get((leg_in, worm_in, vidx))
+
+    def scatter(self, v, leg_in, worm_in, random):
+        """(leg_out, worm_out, new vertex) or (-1, -1, None) (vertex_data.jl:106-125)."""
+        t = self.transitions[(leg_in, worm_in, vcode_idx(v))]
+        offset, length = t
+        for out in range(offset, offset + length + 1):
+            if random < self.transition_cumprobs[out - 1]:
+                leg_out, worm_out = self.transition_step_outs[out - 1]
+                return leg_out, worm_out, self.transition_targets[out - 1]
+        return -1, -1, None
