@@ -1,0 +1,188 @@
+"""Bit-exact parity of the HIP path (through the C ABI) with the oracle.
+
+Every walker shares one Philox4x32-10 stream with its oracle twin; after
+init and after every sweep the operator strings, spins, cutoff, operator
+count, worm controller (FP64 bits), RNG counter and scatter count must be
+identical.  Measurements must agree to 1e-12 relative (same expressions;
+summation order of the estimator prefix sums may differ)."""
+import sys
+
+import numpy as np
+import pytest
+
+from fixtures import (isconsistent, job_bilayer, job_honeycomb, ladder_dimer, square_heisenberg,
+                      vertex_list_fixture)
+
+pytestmark = pytest.mark.gpu
+
+MODES = [0, 1]  # SSE_MODE_WAVE, SSE_MODE_LANE
+KEYS = ("cutoff", "num_operators", "num_worms", "avg_worm_length", "rng_counter",
+        "last_scatter_calls", "sweeps")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _compare(batch, oracles, tag):
+    for k, o in enumerate(oracles):
+        gs, gops, gspins = batch.get_walker(k)
+        os_, oops, ospins = o.state()
+        assert gs["cutoff"] == os_["cutoff"], f"{tag} walker {k}: cutoff"
+        assert np.array_equal(gops, oops), f"{tag} walker {k}: ops differ at " \
+            f"{np.nonzero(gops != oops)[0][:8]}"
+        assert np.array_equal(gspins, ospins), f"{tag} walker {k}: spins"
+        for key in KEYS:
+            assert gs[key] == os_[key], f"{tag} walker {k}: {key} {gs[key]} != {os_[key]}"
+
+
+def _run_parity(sse, oracle_mod, model, T, n_walkers, sweeps, mode, seed0=0, **kw):
+    flat = model.flat()
+    walkers = [sse.walker_desc(T=T, seed=seed0 + s, **kw) for s in range(n_walkers)]
+    _say(f"parity: mode={mode} N={flat.n_sites} walkers={n_walkers} sweeps={sweeps}")
+    batch = sse.Batch([flat], walkers, mode=mode)
+    oracles = [oracle_mod.OracleWalker(sse.capi, flat, w) for w in walkers]
+    batch.init()
+    for o in oracles:
+        o.init()
+    _compare(batch, oracles, "init")
+    for s in range(sweeps):
+        batch.sweep(1)
+        for o in oracles:
+            o.sweep(1)
+        _compare(batch, oracles, f"sweep {s}")
+        if s % 5 == 4 or s == sweeps - 1:
+            gm = batch.measure()
+            for k, o in enumerate(oracles):
+                om = o.measure()
+                assert np.allclose(gm[k], om, rtol=1e-12, atol=1e-13), f"measure walker {k}"
+    batch.close()
+    return flat, oracles
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_parity_square_heisenberg(sse_pkg, oracle_mod, mode):
+    _run_parity(sse_pkg, oracle_mod, square_heisenberg(sse_pkg, 4), 0.5, 8, 40, mode,
+                thermalization=20)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_parity_growth_from_zero_cutoff(sse_pkg, oracle_mod, mode):
+    _run_parity(sse_pkg, oracle_mod, square_heisenberg(sse_pkg, 6), 0.1, 4, 30, mode,
+                init_opstring_cutoff=0, thermalization=5)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_parity_mixed_spin_honeycomb(sse_pkg, oracle_mod, mode):
+    _run_parity(sse_pkg, oracle_mod, job_honeycomb(sse_pkg, 0.3), 0.3, 6, 25, mode,
+                thermalization=10)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_parity_xxz_field(sse_pkg, oracle_mod, mode):
+    model = square_heisenberg(sse_pkg, 8, d=-0.3, hz=0.1)
+    _run_parity(sse_pkg, oracle_mod, model, 0.25, 6, 25, mode, thermalization=10)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_parity_ladder_dimer_basis(sse_pkg, oracle_mod, mode):
+    _run_parity(sse_pkg, oracle_mod, ladder_dimer(sse_pkg, 16, 0.6), 0.2, 4, 25, mode,
+                thermalization=10)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_parity_bilayer_cluster(sse_pkg, oracle_mod, mode):
+    _run_parity(sse_pkg, oracle_mod, job_bilayer(sse_pkg, 0.2), 0.2, 4, 25, mode,
+                thermalization=10)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_vertex_list_golden_gpu(sse_pkg, mode):
+    """Reference golden vertex list, built by the device vertex-list phase."""
+    n_sites, bond_sites, ops, links, first, last = vertex_list_fixture()
+    _say(f"vertex list golden: mode={mode}")
+    # a 4th bond touches site 4 so every site has a dimension; no operator uses it
+    bonds = [sse_pkg.SSEBond(1, (1, 2)), sse_pkg.SSEBond(1, (2, 3)), sse_pkg.SSEBond(1, (1, 3)),
+             sse_pkg.SSEBond(1, (3, 4))]
+    sp, sz = sse_pkg.spin_operators(2)
+    H = np.kron(sz, sz) + 0.5 * (np.kron(sp, sp.T) + np.kron(sp.T, sp))
+    vd = sse_pkg.VertexData((2, 2), H)
+    flat = sse_pkg.FlatModel(sse_pkg.SSEData([vd], bonds))
+    batch = sse_pkg.Batch([flat], [sse_pkg.walker_desc(T=1.0, seed=1)], mode=mode)
+    batch.init()
+    batch.set_walker(0, {"num_operators": int((ops != 0).sum()), "num_worms": 5.0,
+                         "avg_worm_length": 1.0}, ops, np.ones(4, dtype=np.uint8))
+    batch.sweep_phases(2)
+    gl, gf, gla = batch.vertex_list(0)
+    assert np.array_equal(gl, links)
+    assert np.array_equal(gf, first)
+    assert np.array_equal(gla, last)
+    batch.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_checkpoint_roundtrip_gpu(sse_pkg, oracle_mod, mode):
+    model = square_heisenberg(sse_pkg, 6)
+    flat = model.flat()
+    wd = [sse_pkg.walker_desc(T=0.3, seed=40 + s, thermalization=5) for s in range(3)]
+    _say(f"checkpoint: mode={mode}")
+    a = sse_pkg.Batch([flat], wd, mode=mode)
+    a.init()
+    a.sweep(12)
+    saved = [a.get_walker(k) for k in range(3)]
+    b = sse_pkg.Batch([flat], wd, mode=mode)
+    b.init()
+    for k, (sc, ops, spins) in enumerate(saved):
+        b.set_walker(k, sc, ops, spins)
+    a.sweep(15)
+    b.sweep(15)
+    for k in range(3):
+        sa, oa, pa = a.get_walker(k)
+        sb, ob, pb = b.get_walker(k)
+        assert np.array_equal(oa, ob) and np.array_equal(pa, pb)
+        for key in KEYS:
+            assert sa[key] == sb[key]
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_fused_run_accumulators(sse_pkg, oracle_mod, mode):
+    """sse_run (sweep + measure once thermalized, accumulated on device) equals
+    the oracle's per-sweep measurement series summed."""
+    model = square_heisenberg(sse_pkg, 4)
+    flat = model.flat()
+    wd = [sse_pkg.walker_desc(T=0.4, seed=70 + s, thermalization=10) for s in range(4)]
+    _say(f"fused run: mode={mode}")
+    batch = sse_pkg.Batch([flat], wd, mode=mode)
+    batch.init()
+    batch.run(40, measure=True)
+    acc = batch.accumulators()
+    for k, w in enumerate(wd):
+        o = oracle_mod.OracleWalker(sse_pkg.capi, flat, w)
+        o.init()
+        rows = o.run_series(40)
+        s, s2, cnt = acc[k]
+        assert cnt == len(rows) == 40 - 10
+        assert np.allclose(s, rows.sum(axis=0), rtol=1e-12, atol=1e-12)
+        assert np.allclose(s2, (rows ** 2).sum(axis=0), rtol=1e-12, atol=1e-12)
+    batch.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_consistency_long_chain(sse_pkg, mode):
+    """Worldlines stay consistent over many sweeps of a batch (spin-1 sites too)."""
+    model = job_honeycomb(sse_pkg, 0.1)
+    flat = model.flat()
+    wd = [sse_pkg.walker_desc(T=0.1, seed=s, thermalization=50) for s in range(16)]
+    _say(f"long chain: mode={mode}")
+    batch = sse_pkg.Batch([flat], wd, mode=mode)
+    batch.init()
+    for _ in range(10):
+        batch.sweep(20)
+        for k in range(16):
+            sc, ops, spins = batch.get_walker(k)
+            assert isconsistent(flat, ops, spins)
+            assert sc["num_operators"] == int((ops != 0).sum())
+    batch.close()
