@@ -580,7 +580,8 @@ static int launch_measure(sse_batch *b, int mode)
                            b->d_scratch, b->max_sites, n, mode);
     } else {
         int rc = wave_launch_measure(b->wave, b->stream, b->d_walkers, b->d_models, b->d_out,
-                                     b->d_sum, b->d_sumsq, b->d_count, n, mode);
+                                     b->d_sum, b->d_sumsq, b->d_count, b->d_scratch, b->max_sites,
+                                     n, mode);
         if (rc)
             return rc;
     }
