@@ -523,7 +523,11 @@ int sse_init(sse_batch *b)
                            b->d_walkers, b->d_models, b->d_pool, 0, n);
         HIPCHECK(hipGetLastError());
     } else {
-        int rc = wave_launch_init(b->wave, b->stream, b->d_walkers, b->d_models, b->d_pool, n);
+        int max_warmup = 0;
+        for (const auto &wd : b->wdesc)
+            max_warmup = std::max(max_warmup, (int)wd.diagonal_warmup_sweeps);
+        int rc = wave_launch_init(b->wave, b->stream, b->d_walkers, b->d_models, b->d_pool, n,
+                                  max_warmup);
         if (rc)
             return rc;
     }
@@ -563,8 +567,8 @@ int sse_sweep_phases(sse_batch *b, int32_t phases)
         return fail(SSE_ERR_INVALID, "null batch");
     if (!b->inited)
         return fail(SSE_ERR_STATE, "sweep before init");
-    if (phases <= 0 || phases > 7)
-        return fail(SSE_ERR_INVALID, "phases must be a mask of 1|2|4");
+    if (phases <= 0 || phases > 15 || (phases & 7) == 0)
+        return fail(SSE_ERR_INVALID, "phases must be a mask of 1|2|4 (+8)");
     int rc = launch_sweep(b, phases);
     if (rc)
         return rc;
@@ -754,6 +758,16 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links, int64_t li
         int rc = wave_dense_vertex_list(b->wave, w, links, links_capacity);
         if (rc)
             return rc;
+        // the wave path writes links of operator legs only; report identity
+        // slots as unlinked like the reference's fill!(-1)
+        if (links && links_capacity >= 4 * w.L && w.L > 0) {
+            std::vector<uint32_t> ops(w.L);
+            HIPCHECK(hipMemcpy(ops.data(), w.ops, sizeof(uint32_t) * w.L, hipMemcpyDeviceToHost));
+            for (int64_t p = 0; p < w.L; ++p)
+                if (ops[p] == 0)
+                    for (int k = 0; k < 4; ++k)
+                        links[4 * p + k] = -1;
+        }
     }
     if (first)
         HIPCHECK(hipMemcpy(first, w.first, sizeof(int32_t) * w.n_sites, hipMemcpyDeviceToHost));

@@ -232,7 +232,7 @@ __global__ void lane_sweep_kernel(DevWalker *W, const DevModel *models, DevPool 
     w.ctr = rng.ctr;
     if (w.error)
         flag_error(pool, w.error);
-    if (phases == 7) {
+    if ((phases & 7) == 7) {
         w.sweeps += 1;
         w.acc_L += w.L;
         w.acc_n += w.n;

@@ -1,39 +1,877 @@
-// sse_wave.h -- one wavefront per walker (placeholder: delegates to the lane kernels)
+// sse_wave.h -- one wavefront (64 lanes) per walker.
+//
+// K1 wave_diag_vlist_kernel: diagonal update (sse.jl:137-191) and vertex-list
+//   construction (vertex_list.jl:15-54) fused into one streaming pass over the
+//   operator string, 64 slots per step:
+//   * draw offsets: slot types (identity 2 draws, diagonal 1, off-diagonal 0)
+//     are fixed before the update, so every lane's Philox words follow from a
+//     ballot prefix count; the words come from a 256-word LDS buffer that all
+//     lanes refill cooperatively.
+//   * spins at identity slots and vertex-list predecessors: every lane ORs its
+//     bit into per-site LDS lane masks (one for bond position 0, one for 1);
+//     "latest earlier operator on site s" is the highest lower bit of the mask
+//     filtered by the off-diagonal (spins) or final non-identity (links) set.
+//   * the acceptance tests depend on the running operator count n, but each
+//     is monotone in n and n moves by at most one per slot: a lane's decision
+//     is fixed if it agrees at both ends of its window [n0-lane, n0+lane];
+//     the rare ambiguous lanes are resolved in slot order with the exact n.
+//   * a Lemire rejection of the bond draw (probability Nb/2^64) would shift
+//     later draw offsets; that chunk is then processed slot by slot.
+// K2 wave_worm_kernel: worm_update (sse.jl:193-303) -- the traversal is a
+//   sequential pointer chase executed uniformly by the wave; the lanes refill
+//   the Philox buffer together (64 blocks per refill).
+// K3 wave_measure_kernel: measure (sse.jl:70-87,305-376) with the estimator
+//   running sums as wave prefix sums.
 #pragma once
 #include <vector>
+
 #include "sse_common.h"
 #include "sse_lane.h"
 
 struct WaveConfig {
-    int dummy;
+    int max_sites = 0;
+    int max_warmup = 0;
+    size_t lds_k1 = 0, lds_k2 = 0, lds_k3 = 0;
 };
 
-static inline int wave_configure(WaveConfig &, const std::vector<DevModel> &, int) { return 0; }
+__device__ __forceinline__ uint64_t below_mask(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
+__device__ __forceinline__ uint64_t above_mask(int lane) { return lane >= 63 ? 0ull : (~0ull << (lane + 1)); }
+__device__ __forceinline__ int hi_bit(uint64_t x) { return 63 - __clzll((long long)x); }
+__device__ __forceinline__ int popc(uint64_t x) { return __popcll(x); }
 
-static inline int wave_launch_sweep(WaveConfig &, hipStream_t s, DevWalker *W, const DevModel *M,
-                                    DevPool *pool, int n, int phases, hipEvent_t *)
+// Philox words of one walker staged in LDS: buf holds words [bbase, bbase+256).
+// All members are wave-uniform; every call must be made by the whole wave.
+struct WaveStream {
+    uint64_t *buf;
+    uint64_t key, ctr, bbase;
+    __device__ void init(uint64_t *b, uint64_t k, uint64_t c)
+    {
+        buf = b;
+        key = k;
+        ctr = c;
+        bbase = ~0ull;
+    }
+    __device__ void refill(uint64_t at)
+    {
+        const int l = threadIdx.x;
+        uint64_t base = at & ~1ull;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            int j = l + 64 * r;
+            uint32_t o0, o1, o2, o3;
+            philox_block((base >> 1) + (uint64_t)j, key, o0, o1, o2, o3);
+            buf[2 * j] = (uint64_t)o0 | ((uint64_t)o1 << 32);
+            buf[2 * j + 1] = (uint64_t)o2 | ((uint64_t)o3 << 32);
+        }
+        bbase = base;
+        __syncthreads();
+    }
+    __device__ __forceinline__ void ensure(uint64_t need)
+    {
+        if (bbase == ~0ull || ctr < bbase || ctr + need > bbase + 256)
+            refill(ctr);
+    }
+    __device__ __forceinline__ uint64_t at(uint64_t idx) const { return buf[idx - bbase]; }
+    __device__ __forceinline__ uint64_t next()
+    {
+        ensure(1);
+        uint64_t w = buf[ctr - bbase];
+        ++ctr;
+        return w;
+    }
+    __device__ __forceinline__ double next_float() { return word_to_float(next()); }
+    __device__ __forceinline__ uint64_t range(uint64_t s)
+    {
+        uint64_t x = next();
+        uint64_t lo = x * s, hi = __umul64hi(x, s);
+        if (lo < s) {
+            uint64_t t = (0ull - s) % s;
+            while (lo < t) {
+                x = next();
+                lo = x * s;
+                hi = __umul64hi(x, s);
+            }
+        }
+        return hi;
+    }
+};
+
+// LDS layout of K1 (per workgroup = one walker):
+//   rbuf  u64[256]        Philox words
+//   tm0   u64[N]          per-site lane mask, bond position 0
+//   tm1   u64[N]          per-site lane mask, bond position 1
+//   vfirst, vlast i32[N]  vertex-list first/last leg per site
+//   cups  u16[64]         upper-leg states of this chunk's operators
+//   sp    u8[N]           spins at the current slot
+__host__ __device__ inline size_t k1_lds_bytes(int N)
 {
-    hipLaunchKernelGGL(lane_sweep_kernel, dim3((n + 63) / 64), dim3(64), 0, s, W, M, pool, 0, n, phases);
-    return 0;
+    size_t b = 2048 + 16 * (size_t)N + 8 * (size_t)N + 128 + (size_t)N;
+    return (b + 15) & ~(size_t)15;
 }
-static inline int wave_launch_init(WaveConfig &, hipStream_t s, DevWalker *W, const DevModel *M,
-                                   DevPool *pool, int n)
+
+// Slot-by-slot processing of one chunk (reference order).  Used when a bond
+// draw of the chunk hits a Lemire rejection, and when forced for validation.
+__device__ void k1_serial_chunk(DevWalker &w, const DevModel &m, WaveStream &rs, int64_t base,
+                                int64_t &n, double pm_raw, double pr_raw, bool do_diag, bool do_vl,
+                                uint8_t *sp, int32_t *vfirst, int32_t *vlast)
 {
-    hipLaunchKernelGGL(lane_init_kernel, dim3((n + 63) / 64), dim3(64), 0, s, W, M, pool, 0, n);
-    return 0;
+    const int lane = threadIdx.x;
+    const int64_t L = w.L;
+    for (int i = 0; i < 64 && base + i < L; ++i) {
+        const int64_t p = base + i;
+        uint32_t op = w.ops[p];
+        uint32_t nop = op;
+        if (do_diag) {
+            if (op == 0u) {
+                uint32_t b = (uint32_t)rs.range((uint64_t)m.n_bonds);
+                int2 br = m.bond_rec[b];
+                const DevVD &vd = m.types[br.x >> 20];
+                int s0 = br.x & 0xFFFFF, s1 = br.y;
+                int idx = (sp[s0] - 1) + vd.dims0 * (sp[s1] - 1);
+                int32_t dv = vd.diag_vertex[idx];
+                double wgt = dv < 0 ? 0.0 : vd.weights[dv >> 1];
+                double p_make = pm_raw / (double)(L - n);
+                if (rs.next_float() < p_make * wgt) {
+                    nop = op_make(b, (uint32_t)dv);
+                    n += 1;
+                }
+            } else {
+                uint32_t vc = op_vcode(op);
+                int2 br = m.bond_rec[op_bond(op)];
+                const DevVD &vd = m.types[br.x >> 20];
+                if (vc & 1u) {
+                    double wgt = vd.weights[vc >> 1];
+                    double p_remove = (double)(L - n + 1) * pr_raw;
+                    if (rs.next_float() * wgt < p_remove) {
+                        nop = 0u;
+                        n -= 1;
+                    }
+                } else {
+                    const uint8_t *ls = vd.leg_states + 4 * (vc >> 1);
+                    __syncthreads();
+                    if (lane == 0) {
+                        sp[br.x & 0xFFFFF] = ls[2];
+                        sp[br.y] = ls[3];
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        if (do_vl && nop != 0u) {
+            int2 br = m.bond_rec[op_bond(nop)];
+            int sites[2] = {br.x & 0xFFFFF, br.y};
+            for (int k = 0; k < 2; ++k) {
+                int s = sites[k];
+                int32_t me = (int32_t)(4 * p + k);
+                int32_t lst = vlast[s];
+                __syncthreads();
+                if (lane == 0) {
+                    if (lst >= 0) {
+                        w.links[lst] = me;
+                        w.links[me] = lst;
+                    } else {
+                        vfirst[s] = me;
+                    }
+                    vlast[s] = (int32_t)(4 * p + 2 + k);
+                }
+                __syncthreads();
+            }
+        }
+        if (lane == 0 && nop != op)
+            w.ops[p] = nop;
+    }
+    __syncthreads();
 }
+
+// phases: 1 diagonal update, 2 vertex list, 8 force the slot-by-slot path.
+// warm >= 0: init warm-up round (diagonal only; walkers with fewer warm-up
+// sweeps are skipped).
+__global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const DevModel *models,
+                                                             DevPool *pool, int n_walkers,
+                                                             int phases, int warm)
+{
+    extern __shared__ __align__(16) char lds[];
+    const int wi = blockIdx.x;
+    if (wi >= n_walkers)
+        return;
+    const int lane = threadIdx.x;
+    DevWalker w = W[wi];
+    if (w.error)
+        return;
+    if (warm >= 0 && warm >= w.warmup)
+        return;
+    const DevModel &m = models[w.model];
+    const int N = m.n_sites;
+    uint64_t *rbuf = (uint64_t *)lds;
+    uint64_t *tm0 = (uint64_t *)(lds + 2048);
+    uint64_t *tm1 = tm0 + N;
+    int32_t *vfirst = (int32_t *)(tm1 + N);
+    int32_t *vlast = vfirst + N;
+    uint16_t *cups = (uint16_t *)(vlast + N);
+    uint8_t *sp = (uint8_t *)(cups + 64);
+    uint32_t *tm0w = (uint32_t *)tm0;
+    uint32_t *tm1w = (uint32_t *)tm1;
+
+    const bool do_diag = phases & 1;
+    const bool do_vl = (phases & 2) && warm < 0;
+    const bool force_serial = phases & 8;
+
+    for (int i = lane; i < N; i += 64) {
+        tm0[i] = 0ull;
+        tm1[i] = 0ull;
+        vfirst[i] = -1;
+        vlast[i] = -1;
+        sp[i] = w.spins[i];
+    }
+    // growth check, no draws (sse.jl:138-145)
+    if (do_diag && (double)w.n >= (double)w.L * 0.5) {
+        int64_t nl = grown_cutoff(w.L);
+        if (nl > w.cap) {
+            int64_t cap = slab_capacity(nl);
+            unsigned long long sp_addr = 0;
+            if (lane == 0)
+                sp_addr = (unsigned long long)pool_alloc(pool, slab_bytes(cap));
+            uint32_t lo = __shfl((uint32_t)sp_addr, 0), hi = __shfl((uint32_t)(sp_addr >> 32), 0);
+            char *s = (char *)(((unsigned long long)hi << 32) | lo);
+            if (!s) {
+                if (lane == 0) {
+                    W[wi].error = SSE_ERR_OOM;
+                    flag_error(pool, SSE_ERR_OOM);
+                }
+                return;
+            }
+            uint32_t *nops = (uint32_t *)s;
+            for (int64_t p = lane; p < w.L; p += 64)
+                nops[p] = w.ops[p];
+            w.ops = nops;
+            w.links = (int32_t *)(s + (size_t)cap * 4);
+            w.cap = cap;
+        }
+        for (int64_t p = w.L + lane; p < nl; p += 64)
+            w.ops[p] = 0u;
+        w.L = nl;
+    }
+    __syncthreads();
+
+    WaveStream rs;
+    rs.init(rbuf, w.key, w.ctr);
+    const double pm_raw = (double)m.n_bonds / w.T;
+    const double pr_raw = w.T / (double)m.n_bonds;
+    const uint64_t Nb = (uint64_t)m.n_bonds;
+    const int64_t L = w.L;
+    int64_t n0 = w.n;
+    const uint64_t below = below_mask(lane), above = above_mask(lane);
+
+    for (int64_t base = 0; base < L; base += 64) {
+        const int64_t p = base + lane;
+        const bool valid = p < L;
+        const uint32_t op = valid ? w.ops[p] : 0u;
+        const uint32_t vc = op_vcode(op);
+        const bool ident = valid && op == 0u;
+        const bool isop = valid && op != 0u;
+        const bool dia = isop && (vc & 1u);
+        const bool offd = isop && !(vc & 1u);
+        const uint64_t identm = __ballot(ident), diam = __ballot(dia), offm = __ballot(offd);
+        uint32_t b = op_bond(op);
+        double u = 0.0;
+        {
+            bool bad = false;
+            if (isop) {
+                bad = b >= (uint32_t)m.n_bonds;
+                if (!bad)
+                    bad = (int)(vc >> 1) >= m.types[m.bond_rec[b].x >> 20].nv;
+            }
+            if (__ballot(bad)) {
+                if (lane == 0) {
+                    W[wi].error = SSE_ERR_STATE;
+                    flag_error(pool, SSE_ERR_STATE);
+                }
+                return;
+            }
+        }
+
+        if (do_diag) {
+            const int D = 2 * popc(identm) + popc(diam);
+            const int off = 2 * popc(identm & below) + popc(diam & below);
+            bool reject = force_serial;
+            if (D > 0) {
+                rs.ensure((uint64_t)D);
+                if (ident) {
+                    uint64_t x = rs.at(rs.ctr + off);
+                    uint64_t lo = x * Nb;
+                    b = (uint32_t)__umul64hi(x, Nb);
+                    if (lo < Nb && lo < (0ull - Nb) % Nb)
+                        reject = true;
+                    u = word_to_float(rs.at(rs.ctr + off + 1));
+                } else if (dia) {
+                    u = word_to_float(rs.at(rs.ctr + off));
+                }
+            }
+            if (__ballot(reject)) {
+                k1_serial_chunk(w, m, rs, base, n0, pm_raw, pr_raw, do_diag, do_vl, sp, vfirst,
+                                vlast);
+                continue;
+            }
+            rs.ctr += (uint64_t)D;
+        } else if (force_serial) {
+            k1_serial_chunk(w, m, rs, base, n0, pm_raw, pr_raw, do_diag, do_vl, sp, vfirst, vlast);
+            continue;
+        }
+
+        const bool has = isop || (ident && do_diag);
+        int s0 = 0, s1 = 0, type = 0;
+        if (has) {
+            int2 br = m.bond_rec[b];
+            s0 = br.x & 0xFFFFF;
+            s1 = br.y;
+            type = br.x >> 20;
+        }
+        // per-site lane masks
+        if (has) {
+            atomicOr(&tm0w[2 * s0 + (lane >> 5)], 1u << (lane & 31));
+            atomicOr(&tm1w[2 * s1 + (lane >> 5)], 1u << (lane & 31));
+        }
+        if (offd) {
+            const uint8_t *ls = m.types[type].leg_states + 4 * (vc >> 1);
+            cups[lane] = (uint16_t)(ls[2] | (ls[3] << 8));
+        }
+        __syncthreads();
+        uint64_t a0 = 0, a1 = 0, c0 = 0, c1 = 0;  // site0: pos-0/pos-1 masks; site1: same
+        if (has) {
+            a0 = tm0[s0];
+            a1 = tm1[s0];
+            c0 = tm0[s1];
+            c1 = tm1[s1];
+        }
+
+        uint64_t plus = 0, minus = 0;
+        if (do_diag) {
+            double wgt = 0.0;
+            int32_t dv = -1;
+            if (ident) {
+                int v0, v1;
+                uint64_t cand = (a0 | a1) & offm & below;
+                if (cand) {
+                    int q = hi_bit(cand);
+                    v0 = (cups[q] >> (((a0 >> q) & 1ull) ? 0 : 8)) & 255;
+                } else {
+                    v0 = sp[s0];
+                }
+                cand = (c0 | c1) & offm & below;
+                if (cand) {
+                    int q = hi_bit(cand);
+                    v1 = (cups[q] >> (((c0 >> q) & 1ull) ? 0 : 8)) & 255;
+                } else {
+                    v1 = sp[s1];
+                }
+                const DevVD &vd = m.types[type];
+                dv = vd.diag_vertex[(v0 - 1) + vd.dims0 * (v1 - 1)];
+                wgt = dv < 0 ? 0.0 : vd.weights[dv >> 1];
+            } else if (dia) {
+                wgt = m.types[type].weights[vc >> 1];
+            }
+            bool amb = false, dec = false;
+            if (ident) {
+                int64_t lo = n0 - lane < 0 ? 0 : n0 - lane;
+                int64_t hi = n0 + lane > L - 1 ? L - 1 : n0 + lane;
+                bool d_lo = u < (pm_raw / (double)(L - lo)) * wgt;
+                bool d_hi = u < (pm_raw / (double)(L - hi)) * wgt;
+                amb = d_lo != d_hi;
+                dec = d_lo;
+            } else if (dia) {
+                int64_t lo = n0 - lane < 1 ? 1 : n0 - lane;
+                int64_t hi = n0 + lane > L ? L : n0 + lane;
+                double uw = u * wgt;
+                bool d_lo = uw < (double)(L - lo + 1) * pr_raw;
+                bool d_hi = uw < (double)(L - hi + 1) * pr_raw;
+                amb = d_lo != d_hi;
+                dec = d_lo;
+            }
+            plus = __ballot(ident && !amb && dec);
+            minus = __ballot(dia && !amb && dec);
+            uint64_t ambm = __ballot(amb);
+            while (ambm) {
+                const int a = __ffsll((long long)ambm) - 1;
+                const uint64_t ba = below_mask(a);
+                const int64_t na = n0 + popc(plus & ba) - popc(minus & ba);
+                bool d = false;
+                if (lane == a) {
+                    if (ident)
+                        d = u < (pm_raw / (double)(L - na)) * wgt;
+                    else
+                        d = u * wgt < (double)(L - na + 1) * pr_raw;
+                }
+                const uint64_t dm = __ballot(d);
+                if ((identm >> a) & 1ull)
+                    plus |= dm;
+                else
+                    minus |= dm;
+                ambm &= ambm - 1;
+            }
+            n0 += popc(plus) - popc(minus);
+            uint32_t nop = op;
+            if (ident && ((plus >> lane) & 1ull))
+                nop = op_make(b, (uint32_t)dv);
+            if (dia && ((minus >> lane) & 1ull))
+                nop = 0u;
+            if (valid && nop != op)
+                w.ops[p] = nop;
+        }
+
+        // vertex list over the final non-identity operators of the chunk
+        const uint64_t NI = do_diag ? ((identm & plus) | (diam & ~minus) | offm) : (diam | offm);
+        const bool isni = (NI >> lane) & 1ull;
+        int32_t pred0 = -1, pred1 = -1;
+        bool last0 = false, last1 = false;
+        if (do_vl && isni) {
+            uint64_t m0 = (a0 | a1) & NI, m1 = (c0 | c1) & NI;
+            uint64_t cand = m0 & below;
+            if (cand) {
+                int q = hi_bit(cand);
+                pred0 = (int32_t)(4 * (base + q) + 2 + (((a0 >> q) & 1ull) ? 0 : 1));
+            } else {
+                pred0 = vlast[s0];
+                if (pred0 < 0)
+                    pred0 = -2;  // first occurrence of the site
+            }
+            cand = m1 & below;
+            if (cand) {
+                int q = hi_bit(cand);
+                pred1 = (int32_t)(4 * (base + q) + 2 + (((c0 >> q) & 1ull) ? 0 : 1));
+            } else {
+                pred1 = vlast[s1];
+                if (pred1 < 0)
+                    pred1 = -2;
+            }
+            last0 = (m0 & above) == 0ull;
+            last1 = (m1 & above) == 0ull;
+        }
+        __syncthreads();
+        if (do_vl && isni) {
+            const int32_t me0 = (int32_t)(4 * p), me1 = (int32_t)(4 * p + 1);
+            if (pred0 >= 0) {
+                w.links[me0] = pred0;
+                w.links[pred0] = me0;
+            } else {
+                vfirst[s0] = me0;
+            }
+            if (pred1 >= 0) {
+                w.links[me1] = pred1;
+                w.links[pred1] = me1;
+            } else {
+                vfirst[s1] = me1;
+            }
+            if (last0)
+                vlast[s0] = (int32_t)(4 * p + 2);
+            if (last1)
+                vlast[s1] = (int32_t)(4 * p + 3);
+        }
+        // spins after the chunk: the last off-diagonal operator on a site wins
+        if (do_diag && offd) {
+            if ((((a0 | a1) & offm) & above) == 0ull)
+                sp[s0] = (uint8_t)(cups[lane] & 255);
+            if ((((c0 | c1) & offm) & above) == 0ull)
+                sp[s1] = (uint8_t)(cups[lane] >> 8);
+        }
+        if (has) {
+            tm0[s0] = 0ull;
+            tm1[s1] = 0ull;
+        }
+        __syncthreads();
+    }
+
+    // periodic closure (vertex_list.jl:46-51) and write-back
+    if (do_vl) {
+        for (int s = lane; s < N; s += 64) {
+            int32_t f = vfirst[s];
+            if (f >= 0) {
+                int32_t l = vlast[s];
+                w.links[f] = l;
+                w.links[l] = f;
+            }
+            w.first[s] = f;
+            w.last[s] = vlast[s];
+        }
+    }
+    if (do_diag) {
+        for (int s = lane; s < N; s += 64)
+            w.spins[s] = sp[s];
+    }
+    if (lane == 0) {
+        DevWalker &o = W[wi];
+        o.n = n0;
+        o.L = L;
+        o.cap = w.cap;
+        o.ops = w.ops;
+        o.links = w.links;
+        o.ctr = rs.ctr;
+    }
+}
+
+// ------------------------------------------------------------------ K2
+// LDS: rbuf u64[256]
+__global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevModel *models,
+                                                       DevPool *pool, int n_walkers, int full_sweep)
+{
+    __shared__ uint64_t rbuf[256];
+    const int wi = blockIdx.x;
+    if (wi >= n_walkers)
+        return;
+    const int lane = threadIdx.x;
+    DevWalker w = W[wi];
+    if (w.error)
+        return;
+    const DevModel &m = models[w.model];
+    const bool therm = w.sweeps > w.thermalization;
+    WaveStream rs;
+    rs.init(rbuf, w.key, w.ctr);
+    uint32_t *ops = w.ops;
+    const int32_t *links = w.links;
+    const int64_t L = w.L;
+    double total = 1.0;
+    const int64_t nworms = (int64_t)ceil(w.num_worms);
+    int64_t scat = 0;
+    int err = 0;
+    for (int64_t k = 0; k < nworms && !err; ++k) {
+        if (w.n == 0)
+            continue;  // worm_traverse! returns 0 without draws
+        int64_t p0;
+        int l0;
+        for (;;) {
+            p0 = (int64_t)rs.range((uint64_t)L);
+            l0 = (int)rs.range(4);
+            if (ops[p0] != 0u)  // all four legs of an operator are linked
+                break;
+        }
+        const int2 b0 = m.bond_rec[op_bond(ops[p0])];
+        const int site0 = (l0 & 1) ? b0.y : (b0.x & 0xFFFFF);
+        const int wf0 = 1 + (int)rs.range((uint64_t)(m.site_dims[site0] - 1));
+        int leg_in = l0, wf = wf0;
+        int64_t p = p0, len = 1;
+        for (;;) {
+            const uint32_t op = ops[p];
+            const uint32_t bnd = op_bond(op);
+            const int2 br = m.bond_rec[bnd];
+            const DevVD &vd = m.types[br.x >> 20];
+            const double r = rs.next_float();
+            scat += 1;
+            if (op == 0u || bnd >= (uint32_t)m.n_bonds || (int)(op_vcode(op) >> 1) >= vd.nv ||
+                wf < 1 || wf > vd.max_worm) {
+                err = SSE_ERR_STATE;
+                break;
+            }
+            const int kidx = ((int)(op_vcode(op) >> 1) * vd.max_worm + (wf - 1)) * 4 + leg_in;
+            const int32_t toff = vd.toff[kidx];
+            const int32_t tcnt = vd.tcnt[kidx];
+            int e = toff < 0 ? 0 : toff;
+            const int eend = toff < 0 ? 0 : toff + tcnt;
+            while (e < eend && !(r < vd.cum[e]))
+                ++e;
+            if (e >= eend) {
+                err = SSE_ERR_SCATTER;
+                break;
+            }
+            const int leg_out = vd.eleg[e];
+            const int wf_out = vd.ewrm[e];
+            ops[p] = op_make(bnd, (uint32_t)vd.tgt[e]);
+            const int site_out = (leg_out & 1) ? br.y : (br.x & 0xFFFFF);
+            if (p == p0 && leg_out == l0 && wf_out == m.site_dims[site_out] - wf0)
+                break;
+            len += 1;
+            wf = wf_out;
+            const int32_t nx = links[4 * p + leg_out];
+            if (nx < 0 || (int64_t)nx >= 4 * L) {
+                err = SSE_ERR_STATE;
+                break;
+            }
+            leg_in = nx & 3;
+            p = nx >> 2;
+            if (p == p0 && leg_in == l0 && wf == wf0)
+                break;
+        }
+        total += (double)len;
+    }
+    if (err) {
+        if (lane == 0) {
+            W[wi].error = err;
+            flag_error(pool, err);
+        }
+        return;
+    }
+    w.last_scatter = scat;
+    worm_controller(w, total, therm);
+
+    // state refresh (sse.jl:219-228): free sites draw in site order
+    const int N = m.n_sites;
+    __syncthreads();
+    for (int sb = 0; sb < N; sb += 64) {
+        const int s = sb + lane;
+        const bool vs = s < N;
+        const int32_t f = vs ? w.first[s] : 0;
+        const bool fr = vs && f < 0;
+        const uint64_t frm = __ballot(fr);
+        const int D = popc(frm);
+        const uint64_t dim = vs ? (uint64_t)m.site_dims[s] : 1;
+        bool reject = false;
+        uint8_t val = 0;
+        if (D > 0) {
+            rs.ensure((uint64_t)D);
+            if (fr) {
+                uint64_t x = rs.at(rs.ctr + popc(frm & below_mask(lane)));
+                uint64_t lo = x * dim;
+                if (lo < dim && lo < (0ull - dim) % dim)
+                    reject = true;
+                val = (uint8_t)(1 + __umul64hi(x, dim));
+            }
+        }
+        if (vs && !fr) {
+            const uint32_t op = (f >> 2) < L ? ops[f >> 2] : 0u;
+            const DevVD &vd = m.types[m.bond_rec[op_bond(op)].x >> 20];
+            val = vd.leg_states[4 * (op_vcode(op) >> 1) + (f & 3)];
+        }
+        if (__ballot(reject)) {
+            // slot-by-slot for this block of sites (Lemire rejection)
+            for (int i = 0; i < 64 && sb + i < N; ++i) {
+                const int si = sb + i;
+                const bool fi = (frm >> i) & 1ull;
+                uint8_t vi = 0;
+                if (fi)
+                    vi = (uint8_t)(1 + rs.range((uint64_t)m.site_dims[si]));
+                if (lane == i && fi)
+                    val = vi;
+            }
+        } else {
+            rs.ctr += (uint64_t)D;
+        }
+        if (vs)
+            w.spins[s] = val;
+        __syncthreads();
+    }
+    if (lane == 0) {
+        DevWalker &o = W[wi];
+        o.ctr = rs.ctr;
+        o.avg_worm_length = w.avg_worm_length;
+        o.num_worms = w.num_worms;
+        o.last_scatter = w.last_scatter;
+        o.last_total = w.last_total;
+        o.last_wlf = w.last_wlf;
+        if (full_sweep) {
+            o.sweeps = w.sweeps + 1;
+            o.acc_L = w.acc_L + w.L;
+            o.acc_n = w.acc_n + w.n;
+            o.acc_W = w.acc_W + scat;
+            o.acc_sweeps = w.acc_sweeps + 1;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ init
+__global__ __launch_bounds__(64) void wave_init_kernel(DevWalker *W, const DevModel *models, int n_walkers)
+{
+    __shared__ uint64_t rbuf[256];
+    const int wi = blockIdx.x;
+    if (wi >= n_walkers)
+        return;
+    const int lane = threadIdx.x;
+    DevWalker w = W[wi];
+    const DevModel &m = models[w.model];
+    WaveStream rs;
+    rs.init(rbuf, w.key, 0);
+    // random state per site (sse.jl:48), one int draw each in site order
+    for (int i = 0; i < m.n_sites; ++i) {
+        uint8_t v = (uint8_t)(1 + rs.range((uint64_t)m.site_dims[i]));
+        if (lane == 0)
+            w.spins[i] = v;
+    }
+    for (int64_t p = lane; p < w.L; p += 64)
+        w.ops[p] = 0u;
+    if (lane == 0) {
+        W[wi].n = 0;
+        W[wi].ctr = rs.ctr;
+    }
+}
+
+// ------------------------------------------------------------------ K3
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        x += __shfl_xor(x, o);
+    return x;
+}
+
+__device__ __forceinline__ double wave_incl_scan(double x)
+{
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        double y = __shfl_up(x, o);
+        if (lane >= o)
+            x += y;
+    }
+    return x;
+}
+
+// mode 0: explicit measure of every walker; mode 1: thermalized walkers only,
+// accumulated into the per-walker sums (fused sweep+measure loop).
+__global__ __launch_bounds__(64) void wave_measure_kernel(DevWalker *W, const DevModel *models,
+                                                          double *out, double *sum, double *sumsq,
+                                                          int64_t *count, int n_walkers, int mode)
+{
+    const int wi = blockIdx.x;
+    if (wi >= n_walkers)
+        return;
+    const int lane = threadIdx.x;
+    const DevWalker w = W[wi];
+    if (w.error)
+        return;
+    if (mode == 1 && !(w.sweeps > w.thermalization))
+        return;
+    const DevModel &m = models[w.model];
+    const int64_t L = w.L;
+    // sign (sse.jl:305-314)
+    int neg = 0;
+    for (int64_t base = 0; base < L; base += 64) {
+        const int64_t p = base + lane;
+        const uint32_t op = p < L ? w.ops[p] : 0u;
+        bool ng = false;
+        if (op != 0u)
+            ng = m.types[m.bond_rec[op_bond(op)].x >> 20].signs[op_vcode(op) >> 1] < 0;
+        neg += popc(__ballot(ng));
+    }
+    const double sign = (neg & 1) ? -1.0 : 1.0;
+    const double nn = (double)w.n;
+    const double Nn = m.norm;
+    double *o = out + w.meas_offset;
+    if (lane == 0) {
+        o[0] = sign;
+        o[1] = nn;
+        o[2] = sign * nn;
+        o[3] = sign * (nn * nn);
+        o[4] = ((-sign) * (nn * w.T + m.energy_offset)) / Nn;
+    }
+    const int N = m.n_sites, D = m.est_max_dim;
+    for (int e = 0; e < m.n_est; ++e) {
+        const double *coef = m.est_coef + (size_t)e * N;
+        const double *mv = m.est_mval + (size_t)e * N * D;
+        double t0 = 0.0;
+        for (int i = lane; i < N; i += 64)
+            t0 += coef[i] * mv[i * D + w.spins[i] - 1];
+        const double tmp0 = wave_sum(t0);
+        double run = tmp0;
+        double mag = 0.0, absmag = 0.0, mag2 = 0.0, mag4 = 0.0;
+        int64_t cnt = 0;
+        for (int64_t base = 0; base < L; base += 64) {
+            const int64_t p = base + lane;
+            const uint32_t op = p < L ? w.ops[p] : 0u;
+            double delta = 0.0;
+            const uint32_t vc = op_vcode(op);
+            if (op != 0u && !(vc & 1u)) {
+                const int2 br = m.bond_rec[op_bond(op)];
+                const uint8_t *ls = m.types[br.x >> 20].leg_states + 4 * (vc >> 1);
+                const int sa = br.x & 0xFFFFF, sb = br.y;
+                delta = coef[sa] * (mv[sa * D + ls[2] - 1] - mv[sa * D + ls[0] - 1]);
+                delta += coef[sb] * (mv[sb * D + ls[3] - 1] - mv[sb * D + ls[1] - 1]);
+            }
+            const double inc = wave_incl_scan(delta);
+            if (op != 0u) {
+                const double t = run + inc;
+                const double t2 = t * t;
+                mag += t;
+                absmag += fabs(t);
+                mag2 += t2;
+                mag4 += t2 * t2;
+            }
+            cnt += popc(__ballot(op != 0u));
+            run += __shfl(inc, 63);
+        }
+        double smag = tmp0 + wave_sum(mag);
+        double sabs = fabs(tmp0) + wave_sum(absmag);
+        double s2 = tmp0 * tmp0;
+        double smag2 = s2 + wave_sum(mag2);
+        double smag4 = s2 * s2 + wave_sum(mag4);
+        const double en = 1.0 + (double)cnt;
+        const double norm = 1.0 / Nn;
+        smag *= norm;
+        sabs *= norm;
+        smag2 *= norm * norm;
+        smag4 *= (norm * norm) * (norm * norm);
+        if (lane == 0) {
+            double *oe = o + 5 + 5 * e;
+            oe[0] = sign * smag / en;
+            oe[1] = sign * sabs / en;
+            oe[2] = sign * smag2 / en;
+            oe[3] = sign * smag4 / en;
+            const double chi = 1.0 / w.T / (en + 1.0) / en * (smag * smag + smag2) * Nn;
+            oe[4] = sign * chi;
+        }
+    }
+    if (mode == 1) {
+        __syncthreads();
+        for (int j = lane; j < w.meas_width; j += 64) {
+            const double x = o[j];
+            sum[w.meas_offset + j] += x;
+            sumsq[w.meas_offset + j] += x * x;
+        }
+        if (lane == 0)
+            count[wi] += 1;
+    }
+}
+
+// ------------------------------------------------------------------ host side
+static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &models, int max_sites)
+{
+    c.max_sites = max_sites;
+    c.lds_k1 = k1_lds_bytes(max_sites);
+    if (c.lds_k1 > 64 * 1024)
+        return SSE_ERR_INVALID;
+    (void)models;
+    return SSE_OK;
+}
+
+static inline int wave_launch_init(WaveConfig &c, hipStream_t s, DevWalker *W, const DevModel *M,
+                                   DevPool *pool, int n, int max_warmup)
+{
+    hipLaunchKernelGGL(wave_init_kernel, dim3(n), dim3(64), 0, s, W, M, n);
+    for (int r = 0; r < max_warmup; ++r)
+        hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, n, 1, r);
+    return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
+}
+
+static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, const DevModel *M,
+                                    DevPool *pool, int n, int phases, hipEvent_t *ev)
+{
+    const int k1 = phases & (1 | 2 | 8);
+    if (k1 & 3) {
+        hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, n, k1, -1);
+    }
+    if (ev)
+        (void)hipEventRecord(ev[1], s);
+    if (phases & 4)
+        hipLaunchKernelGGL(wave_worm_kernel, dim3(n), dim3(64), 0, s, W, M, pool, n,
+                           (phases & 7) == 7 ? 1 : 0);
+    if (ev)
+        (void)hipEventRecord(ev[2], s);
+    return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
+}
+
 static inline int wave_launch_measure(WaveConfig &, hipStream_t s, DevWalker *W, const DevModel *M,
                                       double *out, double *sum, double *sumsq, int64_t *count,
-                                      uint8_t *scratch, int max_sites, int n, int mode)
+                                      uint8_t *, int, int n, int mode)
 {
-    hipLaunchKernelGGL(lane_measure_kernel, dim3((n + 63) / 64), dim3(64), 0, s, W, M, out, sum,
-                       sumsq, count, scratch, max_sites, n, mode);
-    return 0;
+    hipLaunchKernelGGL(wave_measure_kernel, dim3(n), dim3(64), 0, s, W, M, out, sum, sumsq, count, n,
+                       mode);
+    return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
+
 static inline int wave_dense_vertex_list(WaveConfig &, const DevWalker &w, int32_t *links, int64_t cap)
 {
     if (links && cap >= 4 * w.L && w.L > 0)
-        return hipMemcpy(links, w.links, sizeof(int32_t) * 4 * w.L, hipMemcpyDeviceToHost) == hipSuccess ? 0 : 2;
-    return 0;
+        return hipMemcpy(links, w.links, sizeof(int32_t) * 4 * w.L, hipMemcpyDeviceToHost) == hipSuccess
+                   ? SSE_OK
+                   : SSE_ERR_HIP;
+    return SSE_OK;
 }
+
 static inline void wave_fix_first_last(const DevWalker &, int32_t *, int32_t *, int32_t *) {}

@@ -186,3 +186,46 @@ def test_consistency_long_chain(sse_pkg, mode):
             assert isconsistent(flat, ops, spins)
             assert sc["num_operators"] == int((ops != 0).sum())
     batch.close()
+
+
+def test_parity_forced_serial_chunks(sse_pkg, oracle_mod):
+    """The slot-by-slot chunk path of the wave kernel (taken on a Lemire
+    rejection) is forced for every chunk and must stay bit-exact."""
+    model = square_heisenberg(sse_pkg, 6)
+    flat = model.flat()
+    walkers = [sse_pkg.walker_desc(T=0.3, seed=90 + s, thermalization=8) for s in range(4)]
+    _say("forced serial chunks: wave mode")
+    batch = sse_pkg.Batch([flat], walkers, mode=0)
+    oracles = [oracle_mod.OracleWalker(sse_pkg.capi, flat, w) for w in walkers]
+    batch.init()
+    for o in oracles:
+        o.init()
+    for s in range(15):
+        batch.sweep_phases(1 | 2 | 4 | 8)
+        for o in oracles:
+            o.sweep(1)
+        _compare(batch, oracles, f"serial sweep {s}")
+    batch.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_parity_many_walkers_c2_shape(sse_pkg, oracle_mod, mode):
+    """16x16 beta=8 Heisenberg (the batched workload's model) for a few sweeps
+    from init, 32 walkers with seeds 0..31."""
+    model = square_heisenberg(sse_pkg, 16)
+    flat = model.flat()
+    walkers = [sse_pkg.walker_desc(T=0.125, seed=s, thermalization=2000) for s in range(32)]
+    _say(f"c2 shape: mode={mode}")
+    batch = sse_pkg.Batch([flat], walkers, mode=mode)
+    batch.init()
+    batch.sweep(12)
+    for k in (0, 7, 31):
+        o = oracle_mod.OracleWalker(sse_pkg.capi, flat, walkers[k])
+        o.init()
+        o.sweep(12)
+        gs, gops, gspins = batch.get_walker(k)
+        os_, oops, ospins = o.state()
+        assert np.array_equal(gops, oops) and np.array_equal(gspins, ospins)
+        for key in KEYS:
+            assert gs[key] == os_[key]
+    batch.close()
