@@ -66,6 +66,7 @@ def lib(capi):
     L.so_run_timed.restype = C.c_double
     L.so_run_series.argtypes = [vp, C.c_int64, C.POINTER(C.c_double)]
     L.so_run_series.restype = C.c_int64
+    L.so_work_counters.argtypes = [vp, C.POINTER(C.c_int64)]
     _lib = L
     return L
 
@@ -153,6 +154,11 @@ class OracleWalker:
         if rows < 0:
             raise RuntimeError("oracle sweep error")
         return out[:rows]
+
+    def work_counters(self):
+        out = np.zeros(4, dtype=np.int64)
+        self.L.so_work_counters(self.h, _p(out, C.c_int64))
+        return tuple(int(x) for x in out)
 
     def run_timed(self, n_sweeps, measure=True):
         sums = np.zeros(self.flat.measure_width)

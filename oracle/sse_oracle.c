@@ -136,6 +136,7 @@ struct so_walker {
     int64_t last_scatter;
     double last_total, last_wlf;
     int error;
+    int64_t acc_L, acc_n, acc_W, acc_sweeps;
 };
 
 static inline uint32_t op_bond(uint32_t op) { return op >> SSE_OP_BOND_SHIFT; }
@@ -404,8 +405,13 @@ int so_sweep_phases(so_walker *w, int phases)
     }
     if (phases & 4)
         worm_update(w, thermalized);
-    if (phases == 7)
+    if (phases == 7) {
         w->sweeps += 1;
+        w->acc_L += w->L;
+        w->acc_n += w->n;
+        w->acc_W += w->last_scatter;
+        w->acc_sweeps += 1;
+    }
     return w->error;
 }
 
@@ -564,4 +570,15 @@ int64_t so_run_series(so_walker *w, int64_t n_sweeps, double *out)
         }
     }
     return rows;
+}
+
+/* Sums of cutoff L, operator count n and scatter calls W over the sweeps
+ * since the last call (counters from the chain itself); resets them.     */
+void so_work_counters(so_walker *w, int64_t *out4)
+{
+    out4[0] = w->acc_L;
+    out4[1] = w->acc_n;
+    out4[2] = w->acc_W;
+    out4[3] = w->acc_sweeps;
+    w->acc_L = w->acc_n = w->acc_W = w->acc_sweeps = 0;
 }

@@ -47,6 +47,7 @@ void so_make_vertex_list(int32_t n_sites, const int32_t *bond_sites, const uint3
  * accumulates measurements into sums (width doubles) when non-NULL.        */
 double so_run_timed(so_walker *w, int64_t n_sweeps, int measure, double *sums);
 int64_t so_run_series(so_walker *w, int64_t n_sweeps, double *out);
+void so_work_counters(so_walker *w, int64_t *out4);
 
 #ifdef __cplusplus
 }

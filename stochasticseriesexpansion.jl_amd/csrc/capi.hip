@@ -63,7 +63,9 @@ struct sse_batch {
     int64_t max_records = 0;
     bool inited = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> evpool;  // 4 per sweep of the last timed call
     double phase_ms[3] = {0, 0, 0};
+    int64_t phase_launches = 0;
     WaveConfig wave;
 };
 
@@ -489,13 +491,17 @@ int sse_batch_destroy(sse_batch *b)
     for (int i = 0; i < 4; ++i)
         if (b->ev[i])
             (void)hipEventDestroy(b->ev[i]);
+    for (hipEvent_t e : b->evpool)
+        (void)hipEventDestroy(e);
     if (b->stream)
         (void)hipStreamDestroy(b->stream);
     delete b;
     return SSE_OK;
 }
 
-static int launch_sweep(sse_batch *b, int phases)
+// ev (optional): 3 events recorded after the diagonal/vertex-list kernel,
+// after the worm kernel (lane mode: both after the single sweep kernel)
+static int launch_sweep(sse_batch *b, int phases, hipEvent_t *ev = nullptr)
 {
     int n = b->n_walkers;
     if (b->mode == SSE_MODE_LANE) {
@@ -503,14 +509,48 @@ static int launch_sweep(sse_batch *b, int phases)
         int blocks = (n + threads - 1) / threads;
         hipLaunchKernelGGL(lane_sweep_kernel, dim3(blocks), dim3(threads), 0, b->stream,
                            b->d_walkers, b->d_models, b->d_pool, 0, n, phases);
+        if (ev) {
+            HIPCHECK(hipEventRecord(ev[1], b->stream));
+            HIPCHECK(hipEventRecord(ev[2], b->stream));
+        }
     } else {
         int rc = wave_launch_sweep(b->wave, b->stream, b->d_walkers, b->d_models, b->d_pool, n,
-                                   phases, b->ev);
+                                   phases, ev);
         if (rc)
             return rc;
     }
     HIPCHECK(hipGetLastError());
     return SSE_OK;
+}
+
+static int ensure_events(sse_batch *b, size_t count)
+{
+    while (b->evpool.size() < count) {
+        hipEvent_t e;
+        HIPCHECK(hipEventCreate(&e));
+        b->evpool.push_back(e);
+    }
+    return SSE_OK;
+}
+
+// per-phase device time summed over the sweeps of the last call
+static void collect_phase_times(sse_batch *b, int n_sweeps, bool with_measure)
+{
+    double t[3] = {0, 0, 0};
+    for (int k = 0; k < n_sweeps; ++k) {
+        hipEvent_t *e = &b->evpool[4 * (size_t)k];
+        float a = 0, c = 0, d = 0;
+        (void)hipEventElapsedTime(&a, e[0], e[1]);
+        (void)hipEventElapsedTime(&c, e[1], e[2]);
+        if (with_measure)
+            (void)hipEventElapsedTime(&d, e[2], e[3]);
+        t[0] += a;
+        t[1] += c;
+        t[2] += d;
+    }
+    for (int i = 0; i < 3; ++i)
+        b->phase_ms[i] = t[i];
+    b->phase_launches = n_sweeps;
 }
 
 int sse_init(sse_batch *b)
@@ -546,18 +586,23 @@ int sse_sweep(sse_batch *b, int32_t n_sweeps)
         return fail(SSE_ERR_STATE, "sse_sweep before sse_init");
     if (n_sweeps < 0)
         return fail(SSE_ERR_INVALID, "negative sweep count");
-    HIPCHECK(hipEventRecord(b->ev[0], b->stream));
-    for (int k = 0; k < n_sweeps; ++k) {
-        int rc = launch_sweep(b, 7);
+    const bool timed = n_sweeps <= 256;
+    if (timed) {
+        int rc = ensure_events(b, 4 * (size_t)n_sweeps);
         if (rc)
             return rc;
     }
-    HIPCHECK(hipEventRecord(b->ev[3], b->stream));
+    for (int k = 0; k < n_sweeps; ++k) {
+        hipEvent_t *e = timed ? &b->evpool[4 * (size_t)k] : nullptr;
+        if (e)
+            HIPCHECK(hipEventRecord(e[0], b->stream));
+        int rc = launch_sweep(b, 7, e);
+        if (rc)
+            return rc;
+    }
     int rc = check_walker_errors(b);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[3]);
-    b->phase_ms[0] = ms;
-    b->phase_ms[1] = b->phase_ms[2] = 0;
+    if (timed)
+        collect_phase_times(b, n_sweeps, false);
     return rc;
 }
 
@@ -622,19 +667,27 @@ int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure)
         return fail(SSE_ERR_INVALID, "null batch");
     if (!b->inited)
         return fail(SSE_ERR_STATE, "run before init");
-    HIPCHECK(hipEventRecord(b->ev[0], b->stream));
+    const bool timed = n_sweeps <= 256;
+    if (timed) {
+        int rc = ensure_events(b, 4 * (size_t)n_sweeps);
+        if (rc)
+            return rc;
+    }
     for (int k = 0; k < n_sweeps; ++k) {
-        int rc = launch_sweep(b, 7);
+        hipEvent_t *e = timed ? &b->evpool[4 * (size_t)k] : nullptr;
+        if (e)
+            HIPCHECK(hipEventRecord(e[0], b->stream));
+        int rc = launch_sweep(b, 7, e);
         if (!rc && measure)
             rc = launch_measure(b, 1);
         if (rc)
             return rc;
+        if (e)
+            HIPCHECK(hipEventRecord(e[3], b->stream));
     }
-    HIPCHECK(hipEventRecord(b->ev[3], b->stream));
     int rc = check_walker_errors(b);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[3]);
-    b->phase_ms[0] = ms;
+    if (timed)
+        collect_phase_times(b, n_sweeps, measure != 0);
     return rc;
 }
 
@@ -661,6 +714,9 @@ int sse_reset_accumulators(sse_batch *b)
     HIPCHECK(hipMemsetAsync(b->d_sum, 0, sizeof(double) * b->total_width, b->stream));
     HIPCHECK(hipMemsetAsync(b->d_sumsq, 0, sizeof(double) * b->total_width, b->stream));
     HIPCHECK(hipMemsetAsync(b->d_count, 0, sizeof(int64_t) * b->n_walkers, b->stream));
+    hipLaunchKernelGGL(reset_work_counters_kernel, dim3((b->n_walkers + 63) / 64), dim3(64), 0,
+                       b->stream, b->d_walkers, b->n_walkers);
+    HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(b->stream));
     return SSE_OK;
 }

@@ -365,3 +365,15 @@ __global__ void lane_measure_kernel(DevWalker *W, const DevModel *models, double
         count[wi] += 1;
     }
 }
+
+// zero the per-walker work counters (L, n, W sums and sweep count)
+__global__ void reset_work_counters_kernel(DevWalker *W, int n_walkers)
+{
+    int wi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (wi >= n_walkers)
+        return;
+    W[wi].acc_L = 0;
+    W[wi].acc_n = 0;
+    W[wi].acc_W = 0;
+    W[wi].acc_sweeps = 0;
+}
