@@ -195,6 +195,48 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
     size_t o_ec = put(m.est_coef, sizeof(double) * ne * m.n_sites);
     size_t o_em = put(m.est_mval, sizeof(double) * ne * m.n_sites * (m.n_estimators ? m.est_max_dim : 0));
     size_t o_types = put(nullptr, sizeof(DevVD) * m.n_types);
+    // packed worm tables: hdr | trans | eout | cum
+    std::vector<uint32_t> hdr, trans, eout;
+    std::vector<double> cum;
+    int total_exits = 0;
+    for (int t = 0; t < m.n_types; ++t) {
+        const sse_vertex_data_desc &vd = m.types[t];
+        hdr.push_back((uint32_t)trans.size());
+        hdr.push_back((uint32_t)vd.n_vertices);
+        hdr.push_back((uint32_t)vd.max_worm_count);
+        int nt = vd.n_vertices * vd.max_worm_count * 4;
+        for (int i = 0; i < nt; ++i) {
+            int off = vd.transition_offset[i], cnt = vd.transition_count[i];
+            if (off < 0 || cnt <= 0)
+                trans.push_back(0xFFFFFFFFu);
+            else
+                trans.push_back((uint32_t)(total_exits + off) | ((uint32_t)cnt << 20));
+            if (cnt >= 4096)
+                return fail(SSE_ERR_INVALID, "transition with more than 4095 exits");
+        }
+        for (int e = 0; e < vd.n_exits; ++e) {
+            eout.push_back((uint32_t)vd.exit_target[e] | ((uint32_t)vd.exit_leg[e] << 10) |
+                           ((uint32_t)vd.exit_worm[e] << 12));
+            cum.push_back(vd.exit_cumprob[e]);
+        }
+        total_exits += vd.n_exits;
+    }
+    if (total_exits >= (1 << 20))
+        return fail(SSE_ERR_INVALID, "too many transition exits");
+    std::vector<char> wt;
+    size_t w_trans = hdr.size() * 4;
+    size_t w_eout = w_trans + trans.size() * 4;
+    size_t w_cum = (w_eout + eout.size() * 4 + 7) & ~(size_t)7;
+    size_t w_bytes = (w_cum + cum.size() * 8 + 15) & ~(size_t)15;
+    wt.assign(w_bytes, 0);
+    memcpy(wt.data(), hdr.data(), hdr.size() * 4);
+    if (!trans.empty())
+        memcpy(wt.data() + w_trans, trans.data(), trans.size() * 4);
+    if (!eout.empty())
+        memcpy(wt.data() + w_eout, eout.data(), eout.size() * 4);
+    if (!cum.empty())
+        memcpy(wt.data() + w_cum, cum.data(), cum.size() * 8);
+    size_t o_wtab = put(wt.data(), wt.size());
 
     char *d = nullptr;
     int rc = dev_alloc(b, &d, blob.size() + 16);
@@ -239,6 +281,11 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
     dm.est_coef = (const double *)(d + o_ec);
     dm.est_mval = (const double *)(d + o_em);
     dm.types = (const DevVD *)(d + o_types);
+    dm.wtab = (const uint32_t *)(d + o_wtab);
+    dm.wtab_bytes = (int32_t)w_bytes;
+    dm.wtab_trans_off = (int32_t)w_trans;
+    dm.wtab_eout_off = (int32_t)w_eout;
+    dm.wtab_cum_off = (int32_t)w_cum;
     return SSE_OK;
 }
 
@@ -269,6 +316,8 @@ static int sync_pool_to_device(sse_batch *b)
     p.base = b->pool_base;
     p.overflow = 0;
     p.status = 0;
+    p.max_n = 0;
+    p.max_L = 0;
     HIPCHECK(hipMemcpy(b->d_pool, &p, sizeof p, hipMemcpyHostToDevice));
     return SSE_OK;
 }
@@ -290,6 +339,8 @@ static int check_walker_errors(sse_batch *b)
     HIPCHECK(hipMemcpy(&p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));
     if (p.overflow)
         return fail(SSE_ERR_OOM, "slab pool exhausted; raise SSE_POOL_MB");
+    if (b->mode == SSE_MODE_WAVE)
+        wave_resize_lds(b->wave, p.max_n, p.max_L);
     if (p.status) {
         if (p.status & (1 << SSE_ERR_SCATTER))
             return fail(SSE_ERR_SCATTER, "scatter fell through (r >= last cumprob) or hit an "
@@ -810,27 +861,52 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links, int64_t li
     if (b->mode == SSE_MODE_LANE) {
         if (links && links_capacity >= 4 * w.L && w.L > 0)
             HIPCHECK(hipMemcpy(links, w.links, sizeof(int32_t) * 4 * w.L, hipMemcpyDeviceToHost));
-    } else {
-        int rc = wave_dense_vertex_list(b->wave, w, links, links_capacity);
-        if (rc)
-            return rc;
-        // the wave path writes links of operator legs only; report identity
-        // slots as unlinked like the reference's fill!(-1)
-        if (links && links_capacity >= 4 * w.L && w.L > 0) {
-            std::vector<uint32_t> ops(w.L);
-            HIPCHECK(hipMemcpy(ops.data(), w.ops, sizeof(uint32_t) * w.L, hipMemcpyDeviceToHost));
-            for (int64_t p = 0; p < w.L; ++p)
-                if (ops[p] == 0)
-                    for (int k = 0; k < 4; ++k)
-                        links[4 * p + k] = -1;
+    }
+    std::vector<int64_t> pos;  // wave path: operator rank -> slot
+    if (b->mode != SSE_MODE_LANE) {
+        // compact records (rank-indexed) -> the reference's dense layout
+        int64_t nch = (w.L + 63) / 64;
+        const uint32_t *vt = (const uint32_t *)w.links;
+        std::vector<uint64_t> mask(nch > 0 ? nch : 1);
+        std::vector<uint32_t> lk(4 * (w.n > 0 ? w.n : 1));
+        if (nch > 0)
+            HIPCHECK(hipMemcpy(mask.data(), (const uint64_t *)(vt + 5 * w.cap), 8 * nch,
+                               hipMemcpyDeviceToHost));
+        if (w.n > 0)
+            HIPCHECK(hipMemcpy(lk.data(), vt + w.cap, 16 * w.n, hipMemcpyDeviceToHost));
+        for (int64_t c = 0; c < nch; ++c)
+            for (int i = 0; i < 64; ++i)
+                if ((mask[c] >> i) & 1ull)
+                    pos.push_back(64 * c + i);
+        if ((int64_t)pos.size() != w.n)
+            return fail(SSE_ERR_STATE, "vertex list not built for the current operator string");
+        auto dense = [&](int32_t leg) -> int32_t {
+            return leg < 0 ? leg : (int32_t)(4 * pos[leg >> 2] + (leg & 3));
+        };
+        if (links && links_capacity >= 4 * w.L) {
+            for (int64_t k = 0; k < 4 * w.L; ++k)
+                links[k] = -1;
+            for (int64_t r = 0; r < w.n; ++r)
+                for (int k = 0; k < 4; ++k)
+                    links[4 * pos[r] + k] = dense((int32_t)lk[4 * r + k]);
         }
+        if (first || last) {
+            std::vector<int32_t> f(w.n_sites), l(w.n_sites);
+            HIPCHECK(hipMemcpy(f.data(), w.first, 4 * w.n_sites, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(l.data(), w.last, 4 * w.n_sites, hipMemcpyDeviceToHost));
+            for (int i = 0; i < w.n_sites; ++i) {
+                if (first)
+                    first[i] = dense(f[i] < 0 ? -1 : f[i]);
+                if (last)
+                    last[i] = dense(l[i] < 0 ? -1 : l[i]);
+            }
+        }
+        return SSE_OK;
     }
     if (first)
         HIPCHECK(hipMemcpy(first, w.first, sizeof(int32_t) * w.n_sites, hipMemcpyDeviceToHost));
     if (last)
         HIPCHECK(hipMemcpy(last, w.last, sizeof(int32_t) * w.n_sites, hipMemcpyDeviceToHost));
-    if (b->mode != SSE_MODE_LANE)
-        wave_fix_first_last(w, links, first, last);
     return SSE_OK;
 }
 

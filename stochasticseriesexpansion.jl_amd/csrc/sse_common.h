@@ -3,9 +3,17 @@
 // Layout in HBM (one allocation per batch unless noted):
 //   DevModel[n_models]      model graph + per-bond-type VertexData tables
 //   DevWalker[n_walkers]    per-walker scalars and slab pointers
-//   slab pool               per walker one slab of  ops[cap] (u32) followed by
-//                           links[4*cap] (i32), carved from a device bump
-//                           allocator so the cutoff L grows on device
+//   slab pool               per walker one slab: ops[cap] (u32), then a record
+//                           region used as dense links[4*cap] (i32) by the lane
+//                           kernels, or by the wave kernels as compact records
+//                           indexed by operator rank r (r-th non-identity slot):
+//                             vt[cap]      vcode | bond type << 10
+//                             link[4*cap]  linked leg id 4r'+leg'
+//                             mask[cap/64+1], base[cap/64+1]  per 64-slot chunk:
+//                                          non-identity bit mask, rank of its
+//                                          first operator
+//                           carved from a device bump allocator, so the cutoff
+//                           L grows inside the kernels
 //   spins/first/last        [n_walkers][n_sites] u8 / i32 / i32
 //
 // Operator code (u32): 0 = identity, else 1 | vcode<<1 | bond<<11,
@@ -42,6 +50,13 @@ struct DevModel {
     const double *est_mval;    // [n_est][n_sites][est_max_dim]
     // packed per-bond record for the kernels: sites and type in one int2
     const int2 *bond_rec;      // [n_bonds] (site0 | type<<20, site1)
+    // packed worm tables (staged in LDS by the worm kernel):
+    //   u32 hdr[3*n_types]   (trans offset, nv, max_worm) per type
+    //   u32 trans[...]       exit index | count << 20, or 0xFFFFFFFF
+    //   u32 eout[n_exits]    target vcode | leg << 10 | worm << 12
+    //   f64 cum[n_exits]     at byte offset wtab_cum_off
+    const uint32_t *wtab;
+    int32_t wtab_bytes, wtab_cum_off, wtab_eout_off, wtab_trans_off;
 };
 
 struct DevWalker {
@@ -76,6 +91,8 @@ struct DevPool {
     char *base;
     int32_t overflow;
     int32_t status;           // bit (1 << code) per walker error code seen
+    unsigned long long max_n; // largest operator count after a diagonal update
+    unsigned long long max_L; // largest cutoff
 };
 
 __device__ __forceinline__ void flag_error(DevPool *pool, int code)
@@ -234,8 +251,20 @@ __device__ __host__ __forceinline__ int64_t slab_capacity(int64_t L)
 
 __device__ __host__ __forceinline__ uint64_t slab_bytes(int64_t cap)
 {
-    uint64_t b = (uint64_t)cap * 4u + (uint64_t)cap * 16u;
+    uint64_t b = (uint64_t)cap * 4u + (uint64_t)cap * 20u + ((uint64_t)cap / 64u + 1u) * 16u;
     return (b + 255u) & ~(uint64_t)255u;
+}
+
+// compact-record views of a walker's record region (wave kernels)
+__device__ __forceinline__ uint32_t *rec_vt(const DevWalker &w) { return (uint32_t *)w.links; }
+__device__ __forceinline__ uint32_t *rec_link(const DevWalker &w) { return (uint32_t *)w.links + w.cap; }
+__device__ __forceinline__ uint64_t *rec_mask(const DevWalker &w)
+{
+    return (uint64_t *)((uint32_t *)w.links + 5 * w.cap);
+}
+__device__ __forceinline__ uint32_t *rec_base(const DevWalker &w)
+{
+    return (uint32_t *)(rec_mask(w) + (w.cap / 64 + 1));
 }
 
 // Bump allocation from the batch's slab pool; returns nullptr on exhaustion.

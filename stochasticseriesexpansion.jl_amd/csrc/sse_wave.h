@@ -31,7 +31,9 @@
 struct WaveConfig {
     int max_sites = 0;
     int max_warmup = 0;
+    int max_wtab = 0;
     size_t lds_k1 = 0, lds_k2 = 0, lds_k3 = 0;
+    size_t lds_limit = 65536;  // device limit on dynamic LDS per workgroup
 };
 
 __device__ __forceinline__ uint64_t below_mask(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
@@ -112,12 +114,18 @@ __host__ __device__ inline size_t k1_lds_bytes(int N)
 
 // Slot-by-slot processing of one chunk (reference order).  Used when a bond
 // draw of the chunk hits a Lemire rejection, and when forced for validation.
+// Emits the same compact records as the parallel path.
 __device__ void k1_serial_chunk(DevWalker &w, const DevModel &m, WaveStream &rs, int64_t base,
-                                int64_t &n, double pm_raw, double pr_raw, bool do_diag, bool do_vl,
-                                uint8_t *sp, int32_t *vfirst, int32_t *vlast)
+                                int64_t &n, int64_t &nr, double pm_raw, double pr_raw,
+                                bool do_diag, bool do_vl, uint8_t *sp, int32_t *vfirst,
+                                int32_t *vlast)
 {
     const int lane = threadIdx.x;
     const int64_t L = w.L;
+    uint32_t *gvt = rec_vt(w);
+    uint32_t *glk = rec_link(w);
+    const int64_t nr0 = nr;
+    uint64_t cmask = 0ull;
     for (int i = 0; i < 64 && base + i < L; ++i) {
         const int64_t p = base + i;
         uint32_t op = w.ops[p];
@@ -161,25 +169,34 @@ __device__ void k1_serial_chunk(DevWalker &w, const DevModel &m, WaveStream &rs,
         if (do_vl && nop != 0u) {
             int2 br = m.bond_rec[op_bond(nop)];
             int sites[2] = {br.x & 0xFFFFF, br.y};
+            const int32_t rk = (int32_t)nr;
             for (int k = 0; k < 2; ++k) {
                 int s = sites[k];
-                int32_t me = (int32_t)(4 * p + k);
+                int32_t me = 4 * rk + k;
                 int32_t lst = vlast[s];
                 __syncthreads();
                 if (lane == 0) {
                     if (lst >= 0) {
-                        w.links[lst] = me;
-                        w.links[me] = lst;
+                        glk[lst] = (uint32_t)me;
+                        glk[me] = (uint32_t)lst;
                     } else {
                         vfirst[s] = me;
                     }
-                    vlast[s] = (int32_t)(4 * p + 2 + k);
+                    vlast[s] = 4 * rk + 2 + k;
                 }
                 __syncthreads();
             }
+            if (lane == 0)
+                gvt[rk] = op_vcode(nop) | ((uint32_t)(br.x >> 20) << 10);
+            cmask |= 1ull << i;
+            nr += 1;
         }
         if (lane == 0 && nop != op)
             w.ops[p] = nop;
+    }
+    if (do_vl && lane == 0) {
+        rec_mask(w)[base >> 6] = cmask;
+        rec_base(w)[base >> 6] = (uint32_t)nr0;
     }
     __syncthreads();
 }
@@ -261,7 +278,12 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
     const uint64_t Nb = (uint64_t)m.n_bonds;
     const int64_t L = w.L;
     int64_t n0 = w.n;
+    int64_t nr = 0;  // operators (ranks) emitted so far
     const uint64_t below = below_mask(lane), above = above_mask(lane);
+    uint32_t *gvt = rec_vt(w);
+    uint32_t *glk = rec_link(w);
+    uint64_t *gmask = rec_mask(w);
+    uint32_t *gbase = rec_base(w);
 
     for (int64_t base = 0; base < L; base += 64) {
         const int64_t p = base + lane;
@@ -309,13 +331,14 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
                 }
             }
             if (__ballot(reject)) {
-                k1_serial_chunk(w, m, rs, base, n0, pm_raw, pr_raw, do_diag, do_vl, sp, vfirst,
+                k1_serial_chunk(w, m, rs, base, n0, nr, pm_raw, pr_raw, do_diag, do_vl, sp, vfirst,
                                 vlast);
                 continue;
             }
             rs.ctr += (uint64_t)D;
         } else if (force_serial) {
-            k1_serial_chunk(w, m, rs, base, n0, pm_raw, pr_raw, do_diag, do_vl, sp, vfirst, vlast);
+            k1_serial_chunk(w, m, rs, base, n0, nr, pm_raw, pr_raw, do_diag, do_vl, sp, vfirst,
+                            vlast);
             continue;
         }
 
@@ -346,6 +369,7 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
         }
 
         uint64_t plus = 0, minus = 0;
+        uint32_t fop = op;
         if (do_diag) {
             double wgt = 0.0;
             int32_t dv = -1;
@@ -410,18 +434,19 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
                 ambm &= ambm - 1;
             }
             n0 += popc(plus) - popc(minus);
-            uint32_t nop = op;
             if (ident && ((plus >> lane) & 1ull))
-                nop = op_make(b, (uint32_t)dv);
+                fop = op_make(b, (uint32_t)dv);
             if (dia && ((minus >> lane) & 1ull))
-                nop = 0u;
-            if (valid && nop != op)
-                w.ops[p] = nop;
+                fop = 0u;
+            if (valid && fop != op)
+                w.ops[p] = fop;
         }
 
-        // vertex list over the final non-identity operators of the chunk
+        // vertex list over the final non-identity operators of the chunk, in
+        // compact form: operator rank r = nr + (non-identity lanes below)
         const uint64_t NI = do_diag ? ((identm & plus) | (diam & ~minus) | offm) : (diam | offm);
         const bool isni = (NI >> lane) & 1ull;
+        const int32_t rank = (int32_t)(nr + popc(NI & below));
         int32_t pred0 = -1, pred1 = -1;
         bool last0 = false, last1 = false;
         if (do_vl && isni) {
@@ -429,7 +454,7 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
             uint64_t cand = m0 & below;
             if (cand) {
                 int q = hi_bit(cand);
-                pred0 = (int32_t)(4 * (base + q) + 2 + (((a0 >> q) & 1ull) ? 0 : 1));
+                pred0 = 4 * (int32_t)(nr + popc(NI & below_mask(q))) + 2 + (((a0 >> q) & 1ull) ? 0 : 1);
             } else {
                 pred0 = vlast[s0];
                 if (pred0 < 0)
@@ -438,7 +463,7 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
             cand = m1 & below;
             if (cand) {
                 int q = hi_bit(cand);
-                pred1 = (int32_t)(4 * (base + q) + 2 + (((c0 >> q) & 1ull) ? 0 : 1));
+                pred1 = 4 * (int32_t)(nr + popc(NI & below_mask(q))) + 2 + (((c0 >> q) & 1ull) ? 0 : 1);
             } else {
                 pred1 = vlast[s1];
                 if (pred1 < 0)
@@ -449,24 +474,30 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
         }
         __syncthreads();
         if (do_vl && isni) {
-            const int32_t me0 = (int32_t)(4 * p), me1 = (int32_t)(4 * p + 1);
+            const int32_t me0 = 4 * rank, me1 = 4 * rank + 1;
             if (pred0 >= 0) {
-                w.links[me0] = pred0;
-                w.links[pred0] = me0;
+                glk[me0] = (uint32_t)pred0;
+                glk[pred0] = (uint32_t)me0;
             } else {
                 vfirst[s0] = me0;
             }
             if (pred1 >= 0) {
-                w.links[me1] = pred1;
-                w.links[pred1] = me1;
+                glk[me1] = (uint32_t)pred1;
+                glk[pred1] = (uint32_t)me1;
             } else {
                 vfirst[s1] = me1;
             }
             if (last0)
-                vlast[s0] = (int32_t)(4 * p + 2);
+                vlast[s0] = 4 * rank + 2;
             if (last1)
-                vlast[s1] = (int32_t)(4 * p + 3);
+                vlast[s1] = 4 * rank + 3;
+            gvt[rank] = op_vcode(fop) | ((uint32_t)type << 10);
         }
+        if (do_vl && lane == 0) {
+            gmask[base >> 6] = NI;
+            gbase[base >> 6] = (uint32_t)nr;
+        }
+        nr += popc(NI);
         // spins after the chunk: the last off-diagonal operator on a site wins
         if (do_diag && offd) {
             if ((((a0 | a1) & offm) & above) == 0ull)
@@ -487,8 +518,8 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
             int32_t f = vfirst[s];
             if (f >= 0) {
                 int32_t l = vlast[s];
-                w.links[f] = l;
-                w.links[l] = f;
+                glk[f] = (uint32_t)l;
+                glk[l] = (uint32_t)f;
             }
             w.first[s] = f;
             w.last[s] = vlast[s];
@@ -506,105 +537,111 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
         o.ops = w.ops;
         o.links = w.links;
         o.ctr = rs.ctr;
+        atomicMax(&pool->max_n, (unsigned long long)n0);
+        atomicMax(&pool->max_L, (unsigned long long)L);
     }
 }
 
 // ------------------------------------------------------------------ K2
-// LDS: rbuf u64[256]
-__global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevModel *models,
-                                                       DevPool *pool, int n_walkers, int full_sweep)
+// Worm tables view (LDS copy of DevModel::wtab, or the global original).
+struct WormTab {
+    const uint32_t *hdr, *trans, *eout;
+    const double *cum;
+};
+
+__device__ __forceinline__ WormTab worm_tab(const char *base, const DevModel &m)
 {
-    __shared__ uint64_t rbuf[256];
-    const int wi = blockIdx.x;
-    if (wi >= n_walkers)
-        return;
+    WormTab t;
+    t.hdr = (const uint32_t *)base;
+    t.trans = (const uint32_t *)(base + m.wtab_trans_off);
+    t.eout = (const uint32_t *)(base + m.wtab_eout_off);
+    t.cum = (const double *)(base + m.wtab_cum_off);
+    return t;
+}
+
+// The worm update of one walker over compact records (VT: vcode | type<<10,
+// LK: 4 links per rank), executed uniformly by the whole wave.
+// Returns an error code (0 = ok).
+template <class VTt, class LKt>
+__device__ int worm_body(DevWalker &w, const DevModel &m, WaveStream &rs, VTt *VT, const LKt *LK,
+                         const uint64_t *MASK, const uint32_t *BASE, const WormTab &tab,
+                         double &total, int64_t &scat)
+{
     const int lane = threadIdx.x;
-    DevWalker w = W[wi];
-    if (w.error)
-        return;
-    const DevModel &m = models[w.model];
-    const bool therm = w.sweeps > w.thermalization;
-    WaveStream rs;
-    rs.init(rbuf, w.key, w.ctr);
-    uint32_t *ops = w.ops;
-    const int32_t *links = w.links;
     const int64_t L = w.L;
-    double total = 1.0;
     const int64_t nworms = (int64_t)ceil(w.num_worms);
-    int64_t scat = 0;
-    int err = 0;
-    for (int64_t k = 0; k < nworms && !err; ++k) {
+    total = 1.0;
+    scat = 0;
+    for (int64_t k = 0; k < nworms; ++k) {
         if (w.n == 0)
-            continue;  // worm_traverse! returns 0 without draws
+            continue;  // worm_traverse! returns 0 without draws (sse.jl:234-236)
         int64_t p0;
-        int l0;
-        for (;;) {
+        int l0, r0;
+        for (;;) {  // sse.jl:241-247: an operator's four legs are all linked
             p0 = (int64_t)rs.range((uint64_t)L);
             l0 = (int)rs.range(4);
-            if (ops[p0] != 0u)  // all four legs of an operator are linked
+            const uint64_t mk = MASK[p0 >> 6];
+            const int bit = (int)(p0 & 63);
+            if ((mk >> bit) & 1ull) {
+                r0 = (int)(BASE[p0 >> 6] + popc(mk & below_mask(bit)));
                 break;
+            }
         }
-        const int2 b0 = m.bond_rec[op_bond(ops[p0])];
+        const int2 b0 = m.bond_rec[op_bond(w.ops[p0])];
         const int site0 = (l0 & 1) ? b0.y : (b0.x & 0xFFFFF);
-        const int wf0 = 1 + (int)rs.range((uint64_t)(m.site_dims[site0] - 1));
-        int leg_in = l0, wf = wf0;
-        int64_t p = p0, len = 1;
-        for (;;) {
-            const uint32_t op = ops[p];
-            const uint32_t bnd = op_bond(op);
-            const int2 br = m.bond_rec[bnd];
-            const DevVD &vd = m.types[br.x >> 20];
-            const double r = rs.next_float();
+        const int dim0 = m.site_dims[site0];
+        const int wf0 = 1 + (int)rs.range((uint64_t)(dim0 - 1));
+        int leg_in = l0, wf = wf0, r = r0;
+        int64_t len = 1;
+        for (;;) {  // sse.jl:274-300
+            const uint32_t vt = VT[r];
+            const uint32_t vc = vt & 1023u, t = vt >> 10;
+            const uint32_t toff = tab.hdr[3 * t], nv = tab.hdr[3 * t + 1], mw = tab.hdr[3 * t + 2];
+            if ((vc >> 1) >= nv || wf < 1 || (uint32_t)wf > mw)
+                return SSE_ERR_STATE;
+            const uint32_t tr = tab.trans[toff + ((vc >> 1) * mw + (uint32_t)(wf - 1)) * 4u + leg_in];
+            const double x = rs.next_float();
             scat += 1;
-            if (op == 0u || bnd >= (uint32_t)m.n_bonds || (int)(op_vcode(op) >> 1) >= vd.nv ||
-                wf < 1 || wf > vd.max_worm) {
-                err = SSE_ERR_STATE;
-                break;
-            }
-            const int kidx = ((int)(op_vcode(op) >> 1) * vd.max_worm + (wf - 1)) * 4 + leg_in;
-            const int32_t toff = vd.toff[kidx];
-            const int32_t tcnt = vd.tcnt[kidx];
-            int e = toff < 0 ? 0 : toff;
-            const int eend = toff < 0 ? 0 : toff + tcnt;
-            while (e < eend && !(r < vd.cum[e]))
+            if (tr == 0xFFFFFFFFu)
+                return SSE_ERR_SCATTER;
+            uint32_t e = tr & 0xFFFFFu;
+            const uint32_t eend = e + (tr >> 20);
+            while (e < eend && !(x < tab.cum[e]))
                 ++e;
-            if (e >= eend) {
-                err = SSE_ERR_SCATTER;
-                break;
-            }
-            const int leg_out = vd.eleg[e];
-            const int wf_out = vd.ewrm[e];
-            ops[p] = op_make(bnd, (uint32_t)vd.tgt[e]);
-            const int site_out = (leg_out & 1) ? br.y : (br.x & 0xFFFFF);
-            if (p == p0 && leg_out == l0 && wf_out == m.site_dims[site_out] - wf0)
+            if (e >= eend)
+                return SSE_ERR_SCATTER;  // fall-through (vertex_data.jl:124)
+            const uint32_t eo = tab.eout[e];
+            const int leg_out = (int)((eo >> 10) & 3u);
+            const int wf_out = (int)(eo >> 12);
+            // LDS: one lane writes (the wave's LDS ops execute in order);
+            // global fallback: every lane writes, so each lane's own later
+            // load of the record is ordered after its store
+            if (sizeof(VTt) == 4 || lane == 0)
+                VT[r] = (VTt)((eo & 1023u) | (t << 10));
+            if (r == r0 && leg_out == l0 && wf_out == dim0 - wf0)
                 break;
             len += 1;
             wf = wf_out;
-            const int32_t nx = links[4 * p + leg_out];
-            if (nx < 0 || (int64_t)nx >= 4 * L) {
-                err = SSE_ERR_STATE;
-                break;
-            }
-            leg_in = nx & 3;
-            p = nx >> 2;
-            if (p == p0 && leg_in == l0 && wf == wf0)
+            const uint32_t nx = (uint32_t)LK[4 * r + leg_out];
+            leg_in = (int)(nx & 3u);
+            r = (int)(nx >> 2);
+            if ((int64_t)r >= w.n)
+                return SSE_ERR_STATE;
+            if (r == r0 && leg_in == l0 && wf == wf0)
                 break;
         }
         total += (double)len;
     }
-    if (err) {
-        if (lane == 0) {
-            W[wi].error = err;
-            flag_error(pool, err);
-        }
-        return;
-    }
-    w.last_scatter = scat;
-    worm_controller(w, total, therm);
+    return 0;
+}
 
-    // state refresh (sse.jl:219-228): free sites draw in site order
+// state refresh (sse.jl:219-228) and operator write-back after the worms
+template <class VTt>
+__device__ void worm_finish(DevWalker &w, const DevModel &m, WaveStream &rs, const VTt *VT,
+                            const uint64_t *MASK, const uint32_t *BASE)
+{
+    const int lane = threadIdx.x;
     const int N = m.n_sites;
-    __syncthreads();
     for (int sb = 0; sb < N; sb += 64) {
         const int s = sb + lane;
         const bool vs = s < N;
@@ -625,19 +662,17 @@ __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevMo
                 val = (uint8_t)(1 + __umul64hi(x, dim));
             }
         }
-        if (vs && !fr) {
-            const uint32_t op = (f >> 2) < L ? ops[f >> 2] : 0u;
-            const DevVD &vd = m.types[m.bond_rec[op_bond(op)].x >> 20];
-            val = vd.leg_states[4 * (op_vcode(op) >> 1) + (f & 3)];
+        if (vs && !fr && (int64_t)(f >> 2) < w.n) {
+            const uint32_t vt = VT[f >> 2];
+            const DevVD &vd = m.types[vt >> 10];
+            val = vd.leg_states[4 * ((vt & 1023u) >> 1) + (f & 3)];
         }
         if (__ballot(reject)) {
-            // slot-by-slot for this block of sites (Lemire rejection)
             for (int i = 0; i < 64 && sb + i < N; ++i) {
-                const int si = sb + i;
                 const bool fi = (frm >> i) & 1ull;
                 uint8_t vi = 0;
                 if (fi)
-                    vi = (uint8_t)(1 + rs.range((uint64_t)m.site_dims[si]));
+                    vi = (uint8_t)(1 + rs.range((uint64_t)m.site_dims[sb + i]));
                 if (lane == i && fi)
                     val = vi;
             }
@@ -646,8 +681,106 @@ __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevMo
         }
         if (vs)
             w.spins[s] = val;
-        __syncthreads();
     }
+    // write the worm-updated vertex codes back into the operator string
+    for (int64_t base = 0; base < w.L; base += 64) {
+        const int64_t p = base + lane;
+        const uint64_t mk = MASK[base >> 6];
+        if ((mk >> lane) & 1ull) {
+            const int r = (int)(BASE[base >> 6] + popc(mk & below_mask(lane)));
+            const uint32_t op = w.ops[p];
+            const uint32_t nop = (op & ~(1023u << 1)) | ((VT[r] & 1023u) << 1);
+            if (nop != op)
+                w.ops[p] = nop;
+        }
+    }
+}
+
+// LDS budget of the worm kernel for a walker with n operators and cutoff L
+__host__ __device__ inline size_t k2_lds_need(int64_t n, int64_t L, int wtab_bytes)
+{
+    size_t nch = (size_t)((L + 63) / 64);
+    size_t b = 2048 + (((size_t)wtab_bytes + 15) & ~(size_t)15) + nch * 8 + ((nch * 4 + 15) & ~(size_t)15)
+               + (((size_t)n * 2 + 15) & ~(size_t)15) + (size_t)n * 8;
+    return b;
+}
+
+// K2: LDS layout  rbuf u64[256] | wtab copy | mask u64[nch] | base u32[nch] |
+//                 vt u16[n] | link u16[4n]
+// Walkers whose records do not fit (or n >= 16384, or > 64 bond types) chase
+// their records in global memory instead.
+__global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevModel *models,
+                                                       DevPool *pool, int n_walkers, int full_sweep,
+                                                       int lds_bytes)
+{
+    extern __shared__ __align__(16) char lds[];
+    const int wi = blockIdx.x;
+    if (wi >= n_walkers)
+        return;
+    const int lane = threadIdx.x;
+    DevWalker w = W[wi];
+    if (w.error)
+        return;
+    const DevModel &m = models[w.model];
+    const bool therm = w.sweeps > w.thermalization;
+    uint64_t *rbuf = (uint64_t *)lds;
+    WaveStream rs;
+    rs.init(rbuf, w.key, w.ctr);
+    const int64_t nch = (w.L + 63) / 64;
+    const bool fits = w.n < 16384 && m.n_types <= 64 &&
+                      k2_lds_need(w.n, w.L, m.wtab_bytes) <= (size_t)lds_bytes;
+    double total = 1.0;
+    int64_t scat = 0;
+    int err;
+    if (fits) {
+        char *q = lds + 2048;
+        uint32_t *ltab = (uint32_t *)q;
+        q += (m.wtab_bytes + 15) & ~15;
+        uint64_t *lmask = (uint64_t *)q;
+        q += nch * 8;
+        uint32_t *lbase = (uint32_t *)q;
+        q += (nch * 4 + 15) & ~(int64_t)15;
+        uint16_t *lvt = (uint16_t *)q;
+        q += (w.n * 2 + 15) & ~(int64_t)15;
+        uint16_t *llk = (uint16_t *)q;
+        for (int i = lane; i < m.wtab_bytes / 4; i += 64)
+            ltab[i] = m.wtab[i];
+        const uint64_t *gmask = rec_mask(w);
+        const uint32_t *gbase = rec_base(w);
+        for (int64_t i = lane; i < nch; i += 64) {
+            lmask[i] = gmask[i];
+            lbase[i] = gbase[i];
+        }
+        const uint32_t *gvt = rec_vt(w);
+        const uint32_t *glk = rec_link(w);
+        for (int64_t i = lane; i < w.n; i += 64)
+            lvt[i] = (uint16_t)gvt[i];
+        for (int64_t i = lane; i < 4 * w.n; i += 64)
+            llk[i] = (uint16_t)glk[i];
+        __syncthreads();
+        const WormTab tab = worm_tab((const char *)ltab, m);
+        err = worm_body<uint16_t, uint16_t>(w, m, rs, lvt, llk, lmask, lbase, tab, total, scat);
+        __syncthreads();
+        if (!err)
+            worm_finish<uint16_t>(w, m, rs, lvt, lmask, lbase);
+    } else {
+        const WormTab tab = worm_tab((const char *)m.wtab, m);
+        uint32_t *gvt = rec_vt(w);
+        err = worm_body<uint32_t, uint32_t>(w, m, rs, gvt, rec_link(w), rec_mask(w), rec_base(w),
+                                            tab, total, scat);
+        __syncthreads();
+        if (!err)
+            worm_finish<uint32_t>(w, m, rs, gvt, rec_mask(w), rec_base(w));
+    }
+    if (err) {
+        if (lane == 0) {
+            W[wi].error = err;
+            flag_error(pool, err);
+        }
+        return;
+    }
+    w.last_scatter = scat;
+    worm_controller(w, total, therm);
     if (lane == 0) {
         DevWalker &o = W[wi];
         o.ctr = rs.ctr;
@@ -824,10 +957,35 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
 {
     c.max_sites = max_sites;
     c.lds_k1 = k1_lds_bytes(max_sites);
-    if (c.lds_k1 > 64 * 1024)
+    int dev = 0, lim = 65536;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        lim > 0)
+        c.lds_limit = (size_t)lim;
+    if (c.lds_k1 > c.lds_limit)
         return SSE_ERR_INVALID;
-    (void)models;
+    for (const DevModel &m : models)
+        c.max_wtab = std::max(c.max_wtab, (int)m.wtab_bytes);
+    c.lds_k2 = 2048 + 4096;
+    if (c.lds_limit > 65536) {
+        (void)hipFuncSetAttribute((const void *)wave_worm_kernel,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_limit);
+        (void)hipFuncSetAttribute((const void *)wave_diag_vlist_kernel,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_limit);
+    }
     return SSE_OK;
+}
+
+// Size the worm kernel's LDS for the largest walker seen (max n, max L), with
+// headroom; walkers that still do not fit use the global-memory chase.
+static inline void wave_resize_lds(WaveConfig &c, unsigned long long max_n, unsigned long long max_L)
+{
+    size_t need = k2_lds_need((int64_t)(max_n + max_n / 8 + 64), (int64_t)max_L, c.max_wtab);
+    need = (need + 1023) & ~(size_t)1023;
+    if (need > c.lds_limit)
+        need = c.lds_limit;
+    if (need > c.lds_k2)
+        c.lds_k2 = need;
 }
 
 static inline int wave_launch_init(WaveConfig &c, hipStream_t s, DevWalker *W, const DevModel *M,
@@ -849,8 +1007,8 @@ static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, 
     if (ev)
         (void)hipEventRecord(ev[1], s);
     if (phases & 4)
-        hipLaunchKernelGGL(wave_worm_kernel, dim3(n), dim3(64), 0, s, W, M, pool, n,
-                           (phases & 7) == 7 ? 1 : 0);
+        hipLaunchKernelGGL(wave_worm_kernel, dim3(n), dim3(64), c.lds_k2, s, W, M, pool, n,
+                           (phases & 7) == 7 ? 1 : 0, (int)c.lds_k2);
     if (ev)
         (void)hipEventRecord(ev[2], s);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
@@ -865,13 +1023,3 @@ static inline int wave_launch_measure(WaveConfig &, hipStream_t s, DevWalker *W,
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
 
-static inline int wave_dense_vertex_list(WaveConfig &, const DevWalker &w, int32_t *links, int64_t cap)
-{
-    if (links && cap >= 4 * w.L && w.L > 0)
-        return hipMemcpy(links, w.links, sizeof(int32_t) * 4 * w.L, hipMemcpyDeviceToHost) == hipSuccess
-                   ? SSE_OK
-                   : SSE_ERR_HIP;
-    return SSE_OK;
-}
-
-static inline void wave_fix_first_last(const DevWalker &, int32_t *, int32_t *, int32_t *) {}
