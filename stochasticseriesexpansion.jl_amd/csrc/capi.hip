@@ -237,6 +237,47 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
     if (!cum.empty())
         memcpy(wt.data() + w_cum, cum.data(), cum.size() * 8);
     size_t o_wtab = put(wt.data(), wt.size());
+    // direct table
+    int dnv = 1, dmw = 1;
+    for (int t = 0; t < m.n_types; ++t) {
+        dnv = std::max(dnv, (int)m.types[t].n_vertices);
+        dmw = std::max(dmw, (int)m.types[t].max_worm_count);
+    }
+    std::vector<DirectEntry> dt((size_t)m.n_types * dnv * dmw * 4);
+    for (auto &x : dt) {
+        for (int j = 0; j < 4; ++j) {
+            x.cum[j] = -1.0;
+            x.out[j] = 0;
+        }
+        x.count = 0;
+        x.trans = 0xFFFFFFFFu;
+        x.pad[0] = x.pad[1] = 0;
+    }
+    {
+        int ebase = 0;
+        for (int t = 0; t < m.n_types; ++t) {
+            const sse_vertex_data_desc &vd = m.types[t];
+            for (int v = 0; v < vd.n_vertices; ++v)
+                for (int wv = 0; wv < vd.max_worm_count; ++wv)
+                    for (int l = 0; l < 4; ++l) {
+                        int src = (v * vd.max_worm_count + wv) * 4 + l;
+                        DirectEntry &x = dt[(((size_t)t * dnv + v) * dmw + wv) * 4 + l];
+                        int off = vd.transition_offset[src], cnt = vd.transition_count[src];
+                        if (off < 0 || cnt <= 0)
+                            continue;
+                        x.count = (uint32_t)cnt;
+                        x.trans = (uint32_t)(ebase + off) | ((uint32_t)cnt << 20);
+                        for (int j = 0; j < cnt && j < 4; ++j) {
+                            x.cum[j] = vd.exit_cumprob[off + j];
+                            x.out[j] = (uint32_t)vd.exit_target[off + j] |
+                                       ((uint32_t)vd.exit_leg[off + j] << 10) |
+                                       ((uint32_t)vd.exit_worm[off + j] << 12);
+                        }
+                    }
+            ebase += vd.n_exits;
+        }
+    }
+    size_t o_dtab = put(dt.data(), sizeof(DirectEntry) * dt.size());
 
     char *d = nullptr;
     int rc = dev_alloc(b, &d, blob.size() + 16);
@@ -286,6 +327,10 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
     dm.wtab_trans_off = (int32_t)w_trans;
     dm.wtab_eout_off = (int32_t)w_eout;
     dm.wtab_cum_off = (int32_t)w_cum;
+    dm.dtab = (const DirectEntry *)(d + o_dtab);
+    dm.dt_nv = dnv;
+    dm.dt_mw = dmw;
+    dm.dtab_bytes = (int32_t)(sizeof(DirectEntry) * dt.size());
     return SSE_OK;
 }
 

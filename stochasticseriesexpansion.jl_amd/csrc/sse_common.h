@@ -57,6 +57,20 @@ struct DevModel {
     //   f64 cum[n_exits]     at byte offset wtab_cum_off
     const uint32_t *wtab;
     int32_t wtab_bytes, wtab_cum_off, wtab_eout_off, wtab_trans_off;
+    // direct worm table: one 64-byte entry per (type, vertex, worm, leg_in)
+    // at index ((type*dt_nv + vidx)*dt_mw + worm-1)*4 + leg_in; up to 4 exits
+    // inline (DirectEntry), count 0 = invalid transition, count > 4 = look
+    // the exits up in wtab
+    const struct DirectEntry *dtab;
+    int32_t dt_nv, dt_mw, dtab_bytes, pad_dt;
+};
+
+struct __align__(16) DirectEntry {
+    double cum[4];     // cumulative probabilities; unused slots -1 (never taken)
+    uint32_t out[4];   // target vcode | leg_out << 10 | worm_out << 12
+    uint32_t count;    // number of exits
+    uint32_t trans;    // wtab trans word (exit index | count << 20)
+    uint32_t pad[2];
 };
 
 struct DevWalker {

@@ -41,45 +41,59 @@ __device__ __forceinline__ uint64_t above_mask(int lane) { return lane >= 63 ? 0
 __device__ __forceinline__ int hi_bit(uint64_t x) { return 63 - __clzll((long long)x); }
 __device__ __forceinline__ int popc(uint64_t x) { return __popcll(x); }
 
-// Philox words of one walker staged in LDS: buf holds words [bbase, bbase+256).
-// All members are wave-uniform; every call must be made by the whole wave.
+// Philox words of one walker staged in LDS: buf holds words
+// [bbase, bbase+256) and the next word to draw is bbase+off.  All members are
+// wave-uniform; every call must be made by the whole wave.
+__device__ __noinline__ void wave_refill(uint64_t *buf, uint64_t key, uint64_t base)
+{
+    const int l = threadIdx.x;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        int j = l + 64 * r;
+        uint32_t o0, o1, o2, o3;
+        philox_block((base >> 1) + (uint64_t)j, key, o0, o1, o2, o3);
+        buf[2 * j] = (uint64_t)o0 | ((uint64_t)o1 << 32);
+        buf[2 * j + 1] = (uint64_t)o2 | ((uint64_t)o3 << 32);
+    }
+    __syncthreads();
+}
+
 struct WaveStream {
     uint64_t *buf;
-    uint64_t key, ctr, bbase;
+    uint64_t key, bbase;
+    uint32_t off;  // next word is bbase + off; off > 256 marks an empty buffer
     __device__ void init(uint64_t *b, uint64_t k, uint64_t c)
     {
         buf = b;
         key = k;
-        ctr = c;
-        bbase = ~0ull;
+        bbase = c;
+        off = 1024;  // empty: the first ensure() refills at c
     }
-    __device__ void refill(uint64_t at)
+    __device__ __forceinline__ uint64_t ctr() const { return bbase + off - (off >= 1024 ? 1024 : 0); }
+    // make words [ctr, ctr+need) resident (need <= 255)
+    __device__ __forceinline__ void ensure(uint32_t need)
     {
-        const int l = threadIdx.x;
-        uint64_t base = at & ~1ull;
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            int j = l + 64 * r;
-            uint32_t o0, o1, o2, o3;
-            philox_block((base >> 1) + (uint64_t)j, key, o0, o1, o2, o3);
-            buf[2 * j] = (uint64_t)o0 | ((uint64_t)o1 << 32);
-            buf[2 * j + 1] = (uint64_t)o2 | ((uint64_t)o3 << 32);
+        if (off >= 1024) {  // empty: start at the pending counter bbase
+            const uint64_t c = bbase;
+            bbase = c & ~1ull;
+            off = (uint32_t)(c - bbase);
+            wave_refill(buf, key, bbase);
+        } else if (off + need > 256) {
+            const uint64_t c = bbase + off;
+            bbase = c & ~1ull;
+            off = (uint32_t)(c - bbase);
+            wave_refill(buf, key, bbase);
         }
-        bbase = base;
-        __syncthreads();
     }
-    __device__ __forceinline__ void ensure(uint64_t need)
-    {
-        if (bbase == ~0ull || ctr < bbase || ctr + need > bbase + 256)
-            refill(ctr);
-    }
-    __device__ __forceinline__ uint64_t at(uint64_t idx) const { return buf[idx - bbase]; }
+    // word k positions after the next one (after ensure)
+    __device__ __forceinline__ uint64_t peek(uint32_t k) const { return buf[off + k]; }
+    __device__ __forceinline__ void advance(uint32_t k) { off += k; }
     __device__ __forceinline__ uint64_t next()
     {
         ensure(1);
-        uint64_t w = buf[ctr - bbase];
-        ++ctr;
+        uint64_t w = buf[off];
+        ++off;
         return w;
     }
     __device__ __forceinline__ double next_float() { return word_to_float(next()); }
@@ -98,6 +112,12 @@ struct WaveStream {
         return hi;
     }
 };
+
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uni64(uint64_t x)
+{
+    return (uint64_t)uni((uint32_t)x) | ((uint64_t)uni((uint32_t)(x >> 32)) << 32);
+}
 
 // LDS layout of K1 (per workgroup = one walker):
 //   rbuf  u64[256]        Philox words
@@ -318,16 +338,16 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
             const int off = 2 * popc(identm & below) + popc(diam & below);
             bool reject = force_serial;
             if (D > 0) {
-                rs.ensure((uint64_t)D);
+                rs.ensure((uint32_t)D);
                 if (ident) {
-                    uint64_t x = rs.at(rs.ctr + off);
+                    uint64_t x = rs.peek((uint32_t)off);
                     uint64_t lo = x * Nb;
                     b = (uint32_t)__umul64hi(x, Nb);
                     if (lo < Nb && lo < (0ull - Nb) % Nb)
                         reject = true;
-                    u = word_to_float(rs.at(rs.ctr + off + 1));
+                    u = word_to_float(rs.peek((uint32_t)off + 1));
                 } else if (dia) {
-                    u = word_to_float(rs.at(rs.ctr + off));
+                    u = word_to_float(rs.peek((uint32_t)off));
                 }
             }
             if (__ballot(reject)) {
@@ -335,7 +355,7 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
                                 vlast);
                 continue;
             }
-            rs.ctr += (uint64_t)D;
+            rs.advance((uint32_t)D);
         } else if (force_serial) {
             k1_serial_chunk(w, m, rs, base, n0, nr, pm_raw, pr_raw, do_diag, do_vl, sp, vfirst,
                             vlast);
@@ -536,14 +556,15 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
         o.cap = w.cap;
         o.ops = w.ops;
         o.links = w.links;
-        o.ctr = rs.ctr;
+        o.ctr = rs.ctr();
         atomicMax(&pool->max_n, (unsigned long long)n0);
         atomicMax(&pool->max_L, (unsigned long long)L);
     }
 }
 
 // ------------------------------------------------------------------ K2
-// Worm tables view (LDS copy of DevModel::wtab, or the global original).
+// Exit lists of the packed worm tables (DevModel::wtab) for transitions with
+// more than four exits.
 struct WormTab {
     const uint32_t *hdr, *trans, *eout;
     const double *cum;
@@ -559,85 +580,152 @@ __device__ __forceinline__ WormTab worm_tab(const char *base, const DevModel &m)
     return t;
 }
 
-// The worm update of one walker over compact records (VT: vcode | type<<10,
-// LK: 4 links per rank), executed uniformly by the whole wave.
-// Returns an error code (0 = ok).
-template <class VTt, class LKt>
-__device__ int worm_body(DevWalker &w, const DevModel &m, WaveStream &rs, VTt *VT, const LKt *LK,
-                         const uint64_t *MASK, const uint32_t *BASE, const WormTab &tab,
+// Compact worm records of one walker: vt[r] = vcode | type << 10 and the four
+// links of rank r (leg ids 4r'+leg').  LDS form: u16 vt, four u16 links in
+// one u64; global form: u32 vt, four u32 links in one uint4.
+struct RecLds {
+    uint16_t *vt;
+    const uint64_t *lk;
+    __device__ __forceinline__ void load(int r, uint32_t &v, uint64_t &l) const
+    {
+        v = vt[r];
+        l = lk[r];
+    }
+    __device__ __forceinline__ static uint32_t link(uint64_t l, int leg)
+    {
+        return uni((uint32_t)(l >> (16 * leg)) & 0xFFFFu);
+    }
+    __device__ __forceinline__ void store(int r, uint32_t v, int lane) const
+    {
+        if (lane == 0)
+            vt[r] = (uint16_t)v;
+    }
+    __device__ __forceinline__ uint32_t get_vt(int r) const { return vt[r]; }
+};
+
+struct RecGlobal {
+    uint32_t *vt;
+    const uint4 *lk;
+    struct L4 {
+        uint32_t x[4];
+    };
+    __device__ __forceinline__ void load(int r, uint32_t &v, uint4 &l) const
+    {
+        v = vt[r];
+        l = lk[r];
+    }
+    __device__ __forceinline__ static uint32_t link(const uint4 &l, int leg)
+    {
+        return uni(leg == 0 ? l.x : leg == 1 ? l.y : leg == 2 ? l.z : l.w);
+    }
+    // every lane stores, so each lane's own later load is ordered after it
+    __device__ __forceinline__ void store(int r, uint32_t v, int) const { vt[r] = v; }
+    __device__ __forceinline__ uint32_t get_vt(int r) const { return vt[r]; }
+};
+
+template <class R>
+struct LinkType;
+template <>
+struct LinkType<RecLds> {
+    typedef uint64_t T;
+};
+template <>
+struct LinkType<RecGlobal> {
+    typedef uint4 T;
+};
+
+// The worm update of one walker (sse.jl:193-303), executed uniformly by the
+// whole wave: per chase step one record load (vcode + links) and one direct
+// table entry (up to 4 exits inline).  The chase state is made scalar with
+// readfirstlane (every lane holds the same values), so the integer work and
+// the branches run on the scalar unit.  Returns an error code (0 = ok).
+template <class R>
+__device__ int worm_body(DevWalker &w, const DevModel &m, WaveStream &rs, const R &rec,
+                         const uint64_t *MASK, const uint32_t *BASE, const DirectEntry *DT,
                          double &total, int64_t &scat)
 {
     const int lane = threadIdx.x;
     const int64_t L = w.L;
+    const int n = (int)w.n;
     const int64_t nworms = (int64_t)ceil(w.num_worms);
+    const uint32_t dnv = (uint32_t)m.dt_nv, dmw = (uint32_t)m.dt_mw;
     total = 1.0;
-    scat = 0;
+    int64_t sc = 0;
     for (int64_t k = 0; k < nworms; ++k) {
-        if (w.n == 0)
+        if (n == 0)
             continue;  // worm_traverse! returns 0 without draws (sse.jl:234-236)
         int64_t p0;
         int l0, r0;
         for (;;) {  // sse.jl:241-247: an operator's four legs are all linked
             p0 = (int64_t)rs.range((uint64_t)L);
             l0 = (int)rs.range(4);
-            const uint64_t mk = MASK[p0 >> 6];
+            const uint64_t mk = uni64(MASK[p0 >> 6]);
             const int bit = (int)(p0 & 63);
             if ((mk >> bit) & 1ull) {
-                r0 = (int)(BASE[p0 >> 6] + popc(mk & below_mask(bit)));
+                r0 = (int)uni(BASE[p0 >> 6] + (uint32_t)popc(mk & below_mask(bit)));
                 break;
             }
         }
         const int2 b0 = m.bond_rec[op_bond(w.ops[p0])];
         const int site0 = (l0 & 1) ? b0.y : (b0.x & 0xFFFFF);
-        const int dim0 = m.site_dims[site0];
+        const int dim0 = (int)uni(m.site_dims[site0]);
         const int wf0 = 1 + (int)rs.range((uint64_t)(dim0 - 1));
+        const int winv0 = dim0 - wf0;
         int leg_in = l0, wf = wf0, r = r0;
         int64_t len = 1;
         for (;;) {  // sse.jl:274-300
-            const uint32_t vt = VT[r];
-            const uint32_t vc = vt & 1023u, t = vt >> 10;
-            const uint32_t toff = tab.hdr[3 * t], nv = tab.hdr[3 * t + 1], mw = tab.hdr[3 * t + 2];
-            if ((vc >> 1) >= nv || wf < 1 || (uint32_t)wf > mw)
+            uint32_t vt;
+            typename LinkType<R>::T lk;
+            rec.load(r, vt, lk);
+            vt = uni(vt);
+            const uint32_t vidx = (vt & 1023u) >> 1, t = vt >> 10;
+            if (vidx >= dnv || (uint32_t)(wf - 1) >= dmw)
                 return SSE_ERR_STATE;
-            const uint32_t tr = tab.trans[toff + ((vc >> 1) * mw + (uint32_t)(wf - 1)) * 4u + leg_in];
-            const double x = rs.next_float();
-            scat += 1;
-            if (tr == 0xFFFFFFFFu)
-                return SSE_ERR_SCATTER;
-            uint32_t e = tr & 0xFFFFFu;
-            const uint32_t eend = e + (tr >> 20);
-            while (e < eend && !(x < tab.cum[e]))
-                ++e;
-            if (e >= eend)
-                return SSE_ERR_SCATTER;  // fall-through (vertex_data.jl:124)
-            const uint32_t eo = tab.eout[e];
+            const DirectEntry &de = DT[((t * dnv + vidx) * dmw + (uint32_t)(wf - 1)) * 4u + leg_in];
+            const double c0 = de.cum[0], c1 = de.cum[1], c2 = de.cum[2], c3 = de.cum[3];
+            const uint32_t o0 = de.out[0], o1 = de.out[1], o2 = de.out[2], o3 = de.out[3];
+            const uint32_t cnt = uni(de.count);
+            const double x = word_to_float(uni64(rs.next()));
+            sc += 1;
+            uint32_t eo = (x < c0) ? o0 : (x < c1) ? o1 : (x < c2) ? o2 : (x < c3) ? o3 : 0xFFFFFFFFu;
+            eo = uni(eo);
+            if (cnt > 4u) {
+                const WormTab tab = worm_tab((const char *)m.wtab, m);
+                const uint32_t tr = uni(de.trans);
+                uint32_t e = tr & 0xFFFFFu;
+                const uint32_t eend = e + (tr >> 20);
+                while (e < eend && !(x < tab.cum[e]))
+                    ++e;
+                if (e >= eend)
+                    return SSE_ERR_SCATTER;
+                eo = uni(tab.eout[e]);
+            } else if (eo == 0xFFFFFFFFu) {
+                return SSE_ERR_SCATTER;  // fall-through (vertex_data.jl:124) or invalid
+            }
             const int leg_out = (int)((eo >> 10) & 3u);
             const int wf_out = (int)(eo >> 12);
-            // LDS: one lane writes (the wave's LDS ops execute in order);
-            // global fallback: every lane writes, so each lane's own later
-            // load of the record is ordered after its store
-            if (sizeof(VTt) == 4 || lane == 0)
-                VT[r] = (VTt)((eo & 1023u) | (t << 10));
-            if (r == r0 && leg_out == l0 && wf_out == dim0 - wf0)
+            rec.store(r, (eo & 1023u) | (t << 10), lane);
+            if (r == r0 && leg_out == l0 && wf_out == winv0)
                 break;
             len += 1;
             wf = wf_out;
-            const uint32_t nx = (uint32_t)LK[4 * r + leg_out];
+            const uint32_t nx = R::link(lk, leg_out);
             leg_in = (int)(nx & 3u);
             r = (int)(nx >> 2);
-            if ((int64_t)r >= w.n)
+            if (r >= n)
                 return SSE_ERR_STATE;
             if (r == r0 && leg_in == l0 && wf == wf0)
                 break;
         }
         total += (double)len;
     }
+    scat = sc;
     return 0;
 }
 
 // state refresh (sse.jl:219-228) and operator write-back after the worms
-template <class VTt>
-__device__ void worm_finish(DevWalker &w, const DevModel &m, WaveStream &rs, const VTt *VT,
+template <class R>
+__device__ void worm_finish(DevWalker &w, const DevModel &m, WaveStream &rs, const R &rec,
                             const uint64_t *MASK, const uint32_t *BASE)
 {
     const int lane = threadIdx.x;
@@ -653,9 +741,9 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, WaveStream &rs, con
         bool reject = false;
         uint8_t val = 0;
         if (D > 0) {
-            rs.ensure((uint64_t)D);
+            rs.ensure((uint32_t)D);
             if (fr) {
-                uint64_t x = rs.at(rs.ctr + popc(frm & below_mask(lane)));
+                uint64_t x = rs.peek((uint32_t)popc(frm & below_mask(lane)));
                 uint64_t lo = x * dim;
                 if (lo < dim && lo < (0ull - dim) % dim)
                     reject = true;
@@ -663,7 +751,7 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, WaveStream &rs, con
             }
         }
         if (vs && !fr && (int64_t)(f >> 2) < w.n) {
-            const uint32_t vt = VT[f >> 2];
+            const uint32_t vt = rec.get_vt(f >> 2);
             const DevVD &vd = m.types[vt >> 10];
             val = vd.leg_states[4 * ((vt & 1023u) >> 1) + (f & 3)];
         }
@@ -677,7 +765,7 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, WaveStream &rs, con
                     val = vi;
             }
         } else {
-            rs.ctr += (uint64_t)D;
+            rs.advance((uint32_t)D);
         }
         if (vs)
             w.spins[s] = val;
@@ -689,24 +777,26 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, WaveStream &rs, con
         if ((mk >> lane) & 1ull) {
             const int r = (int)(BASE[base >> 6] + popc(mk & below_mask(lane)));
             const uint32_t op = w.ops[p];
-            const uint32_t nop = (op & ~(1023u << 1)) | ((VT[r] & 1023u) << 1);
+            const uint32_t nop = (op & ~(1023u << 1)) | ((rec.get_vt(r) & 1023u) << 1);
             if (nop != op)
                 w.ops[p] = nop;
         }
     }
 }
 
-// LDS budget of the worm kernel for a walker with n operators and cutoff L
-__host__ __device__ inline size_t k2_lds_need(int64_t n, int64_t L, int wtab_bytes)
+#define SSE_DTAB_LDS_MAX 32768
+
+// LDS bytes the worm kernel needs to hold a walker with n operators, cutoff L
+__host__ __device__ inline size_t k2_lds_need(int64_t n, int64_t L, int dtab_bytes)
 {
     size_t nch = (size_t)((L + 63) / 64);
-    size_t b = 2048 + (((size_t)wtab_bytes + 15) & ~(size_t)15) + nch * 8 + ((nch * 4 + 15) & ~(size_t)15)
-               + (((size_t)n * 2 + 15) & ~(size_t)15) + (size_t)n * 8;
-    return b;
+    size_t dt = dtab_bytes <= SSE_DTAB_LDS_MAX ? (size_t)dtab_bytes : 0;
+    return 2048 + dt + nch * 8 + ((nch * 4 + 15) & ~(size_t)15) + (size_t)n * 8 +
+           (((size_t)n * 2 + 15) & ~(size_t)15);
 }
 
-// K2: LDS layout  rbuf u64[256] | wtab copy | mask u64[nch] | base u32[nch] |
-//                 vt u16[n] | link u16[4n]
+// K2: LDS layout  rbuf u64[256] | direct table | mask u64[nch] | base u32[nch] |
+//                 links u64[n] | vt u16[n]
 // Walkers whose records do not fit (or n >= 16384, or > 64 bond types) chase
 // their records in global memory instead.
 __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevModel *models,
@@ -727,24 +817,28 @@ __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevMo
     WaveStream rs;
     rs.init(rbuf, w.key, w.ctr);
     const int64_t nch = (w.L + 63) / 64;
-    const bool fits = w.n < 16384 && m.n_types <= 64 &&
-                      k2_lds_need(w.n, w.L, m.wtab_bytes) <= (size_t)lds_bytes;
+    const bool fits = w.n < 16384 && m.n_types <= 64 && m.dtab_bytes <= SSE_DTAB_LDS_MAX &&
+                      k2_lds_need(w.n, w.L, m.dtab_bytes) <= (size_t)lds_bytes;
     double total = 1.0;
     int64_t scat = 0;
     int err;
     if (fits) {
         char *q = lds + 2048;
-        uint32_t *ltab = (uint32_t *)q;
-        q += (m.wtab_bytes + 15) & ~15;
+        {
+            uint4 *ldt = (uint4 *)q;
+            const uint4 *gdt = (const uint4 *)m.dtab;
+            for (int i = lane; i < m.dtab_bytes / 16; i += 64)
+                ldt[i] = gdt[i];
+        }
+        const DirectEntry *DT = (const DirectEntry *)q;
+        q += m.dtab_bytes;
         uint64_t *lmask = (uint64_t *)q;
         q += nch * 8;
         uint32_t *lbase = (uint32_t *)q;
         q += (nch * 4 + 15) & ~(int64_t)15;
+        uint64_t *llk = (uint64_t *)q;
+        q += w.n * 8;
         uint16_t *lvt = (uint16_t *)q;
-        q += (w.n * 2 + 15) & ~(int64_t)15;
-        uint16_t *llk = (uint16_t *)q;
-        for (int i = lane; i < m.wtab_bytes / 4; i += 64)
-            ltab[i] = m.wtab[i];
         const uint64_t *gmask = rec_mask(w);
         const uint32_t *gbase = rec_base(w);
         for (int64_t i = lane; i < nch; i += 64) {
@@ -752,25 +846,25 @@ __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevMo
             lbase[i] = gbase[i];
         }
         const uint32_t *gvt = rec_vt(w);
-        const uint32_t *glk = rec_link(w);
-        for (int64_t i = lane; i < w.n; i += 64)
+        const uint4 *glk = (const uint4 *)rec_link(w);
+        for (int64_t i = lane; i < w.n; i += 64) {
             lvt[i] = (uint16_t)gvt[i];
-        for (int64_t i = lane; i < 4 * w.n; i += 64)
-            llk[i] = (uint16_t)glk[i];
+            const uint4 l4 = glk[i];
+            llk[i] = (uint64_t)(l4.x & 0xFFFFu) | ((uint64_t)(l4.y & 0xFFFFu) << 16) |
+                     ((uint64_t)(l4.z & 0xFFFFu) << 32) | ((uint64_t)(l4.w & 0xFFFFu) << 48);
+        }
         __syncthreads();
-        const WormTab tab = worm_tab((const char *)ltab, m);
-        err = worm_body<uint16_t, uint16_t>(w, m, rs, lvt, llk, lmask, lbase, tab, total, scat);
+        RecLds rec{lvt, llk};
+        err = worm_body(w, m, rs, rec, lmask, lbase, DT, total, scat);
         __syncthreads();
         if (!err)
-            worm_finish<uint16_t>(w, m, rs, lvt, lmask, lbase);
+            worm_finish(w, m, rs, rec, lmask, lbase);
     } else {
-        const WormTab tab = worm_tab((const char *)m.wtab, m);
-        uint32_t *gvt = rec_vt(w);
-        err = worm_body<uint32_t, uint32_t>(w, m, rs, gvt, rec_link(w), rec_mask(w), rec_base(w),
-                                            tab, total, scat);
+        RecGlobal rec{rec_vt(w), (const uint4 *)rec_link(w)};
+        err = worm_body(w, m, rs, rec, rec_mask(w), rec_base(w), m.dtab, total, scat);
         __syncthreads();
         if (!err)
-            worm_finish<uint32_t>(w, m, rs, gvt, rec_mask(w), rec_base(w));
+            worm_finish(w, m, rs, rec, rec_mask(w), rec_base(w));
     }
     if (err) {
         if (lane == 0) {
@@ -783,7 +877,7 @@ __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevMo
     worm_controller(w, total, therm);
     if (lane == 0) {
         DevWalker &o = W[wi];
-        o.ctr = rs.ctr;
+        o.ctr = rs.ctr();
         o.avg_worm_length = w.avg_worm_length;
         o.num_worms = w.num_worms;
         o.last_scatter = w.last_scatter;
@@ -821,7 +915,7 @@ __global__ __launch_bounds__(64) void wave_init_kernel(DevWalker *W, const DevMo
         w.ops[p] = 0u;
     if (lane == 0) {
         W[wi].n = 0;
-        W[wi].ctr = rs.ctr;
+        W[wi].ctr = rs.ctr();
     }
 }
 
@@ -965,7 +1059,7 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
     if (c.lds_k1 > c.lds_limit)
         return SSE_ERR_INVALID;
     for (const DevModel &m : models)
-        c.max_wtab = std::max(c.max_wtab, (int)m.wtab_bytes);
+        c.max_wtab = std::max(c.max_wtab, (int)m.dtab_bytes);
     c.lds_k2 = 2048 + 4096;
     if (c.lds_limit > 65536) {
         (void)hipFuncSetAttribute((const void *)wave_worm_kernel,
