@@ -200,6 +200,27 @@ int sse_phase_times(const sse_batch *b, double *ms3);
 int sse_work_counters(sse_batch *b, int64_t *L_sum, int64_t *n_sum,
                       int64_t *W_sum, int64_t *walker_sweeps);
 
+/* ---- serving: slots ------------------------------------------------------
+ * Every walker of a batch is a slot.  A slot can be restarted with a new job
+ * (walker parameters, model and a budget of sweeps) while the others keep
+ * running; sse_run/sse_sweep initialise pending slots, sweep active ones and
+ * retire a slot after its last budgeted sweep (+ measure).  The job's results
+ * are the slot's accumulators.  These entry points have no single counterpart
+ * in the reference: they replace Carlo's per-task scheduling
+ * (JobTools/TaskMaker + one MC per task, examples/bani2v2o8.jl:1-42).       */
+
+/* Upload one more model into the batch (n_sites <= the batch's maximum). */
+int sse_add_model(sse_batch *b, const sse_model_desc *model, int32_t *index);
+/* Restart slot with walker parameters wd for n_sweeps sweeps (-1 = unlimited);
+ * the slot's accumulators are cleared.                                      */
+int sse_slot_start(sse_batch *b, int32_t slot, const sse_walker_desc *wd, int64_t n_sweeps);
+/* Stop a slot (it is neither initialised nor swept until restarted).       */
+int sse_slot_stop(sse_batch *b, int32_t slot);
+/* Slots that finished their budget since the last poll.                     */
+int sse_slot_poll(sse_batch *b, int32_t *done_slots, int32_t capacity, int32_t *n_done);
+/* Bytes of the slab pool in use (opstring + records of every walker).       */
+int sse_pool_usage(sse_batch *b, int64_t *used_bytes, int64_t *total_bytes);
+
 /* Encoding helpers (src/opercode.jl:18-71). */
 uint64_t sse_opcode_to_reference(uint32_t op);
 uint32_t sse_opcode_from_reference(uint64_t op);

@@ -188,6 +188,11 @@ def load_library(path=None):
     lib.sse_opcode_to_reference.restype = C.c_uint64
     lib.sse_opcode_from_reference.argtypes = [C.c_uint64]
     lib.sse_opcode_from_reference.restype = C.c_uint32
+    lib.sse_add_model.argtypes = [vp, C.POINTER(ModelDesc), C.POINTER(C.c_int32)]
+    lib.sse_slot_start.argtypes = [vp, C.c_int32, C.POINTER(WalkerDesc), C.c_int64]
+    lib.sse_slot_stop.argtypes = [vp, C.c_int32]
+    lib.sse_slot_poll.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]
+    lib.sse_pool_usage.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.sse_last_error.restype = C.c_char_p
     lib.sse_abi_version.restype = C.c_int32
     if path is None:
@@ -200,7 +205,8 @@ EXPORTED_SYMBOLS = (
     "sse_measure_width", "sse_run", "sse_read_accumulators", "sse_reset_accumulators",
     "sse_get_walker", "sse_set_walker", "sse_sweep_phases", "sse_get_vertex_list",
     "sse_phase_times", "sse_work_counters", "sse_opcode_to_reference",
-    "sse_opcode_from_reference", "sse_last_error", "sse_abi_version")
+    "sse_opcode_from_reference", "sse_last_error", "sse_abi_version", "sse_add_model",
+    "sse_slot_start", "sse_slot_stop", "sse_slot_poll", "sse_pool_usage")
 
 
 class SSEError(RuntimeError):
@@ -228,6 +234,7 @@ class Batch:
                                                   len(self.walkers), mode, C.byref(h)))
         self.handle = h
         self.mode = mode
+        self._slot_widths = [self.flats[w.model].measure_width for w in self.walkers]
 
     def close(self):
         if self.handle:
@@ -257,13 +264,13 @@ class Batch:
         return self.lib.sse_measure_width(self.handle, w)
 
     def measure(self):
-        widths = [self.flats[w.model].measure_width for w in self.walkers]
+        widths = self._slot_widths
         total = sum(widths)
         out = np.zeros(total, dtype=np.float64)
         check(self.lib, self.lib.sse_measure(self.handle, _ptr(out, C.c_double)))
         res, k = [], 0
-        for wd in widths:
-            res.append(out[k:k + wd])
+        for w, wd in enumerate(widths):
+            res.append(out[k:k + self.flats[self.walkers[w].model].measure_width])
             k += wd
         return res
 
@@ -271,7 +278,7 @@ class Batch:
         check(self.lib, self.lib.sse_run(self.handle, n_sweeps, int(bool(measure))))
 
     def accumulators(self):
-        widths = [self.flats[w.model].measure_width for w in self.walkers]
+        widths = self._slot_widths
         total = sum(widths)
         s = np.zeros(total)
         s2 = np.zeros(total)
@@ -281,7 +288,8 @@ class Batch:
                                                        _ptr(cnt, C.c_int64)))
         out, k = [], 0
         for w, wd in enumerate(widths):
-            out.append((s[k:k + wd], s2[k:k + wd], int(cnt[w])))
+            cw = self.flats[self.walkers[w].model].measure_width
+            out.append((s[k:k + cw], s2[k:k + cw], int(cnt[w])))
             k += wd
         return out
 
@@ -324,6 +332,38 @@ class Batch:
                                                      len(links), _ptr(first, C.c_int32),
                                                      _ptr(last, C.c_int32)))
         return links[:4 * L], first, last
+
+    # ---- serving slots
+    def add_model(self, flat):
+        descs = ModelDescs([flat])
+        self._extra_models = getattr(self, "_extra_models", [])
+        self._extra_models.append(descs)
+        idx = C.c_int32()
+        check(self.lib, self.lib.sse_add_model(self.handle, descs.array, C.byref(idx)))
+        self.flats.append(flat)
+        return idx.value
+
+    def slot_start(self, slot, walker, n_sweeps):
+        check(self.lib, self.lib.sse_slot_start(self.handle, slot, C.byref(walker), int(n_sweeps)))
+        self.walkers[slot] = walker
+
+    def slot_stop(self, slot):
+        check(self.lib, self.lib.sse_slot_stop(self.handle, slot))
+
+    def slot_poll(self):
+        buf = (C.c_int32 * self.n_walkers)()
+        n = C.c_int32()
+        check(self.lib, self.lib.sse_slot_poll(self.handle, buf, self.n_walkers, C.byref(n)))
+        return [buf[i] for i in range(n.value)]
+
+    def slot_result(self, slot):
+        """(sums, sums of squares, measurement count) of one slot."""
+        return self.accumulators()[slot]
+
+    def pool_usage(self):
+        u, t = C.c_int64(), C.c_int64()
+        check(self.lib, self.lib.sse_pool_usage(self.handle, C.byref(u), C.byref(t)))
+        return u.value, t.value
 
     def phase_times(self):
         t = np.zeros(3)

@@ -58,7 +58,10 @@ struct sse_batch {
     int32_t *d_first = nullptr, *d_last = nullptr;
     double *d_out = nullptr, *d_sum = nullptr, *d_sumsq = nullptr;
     int64_t *d_count = nullptr;
+    int32_t *d_done = nullptr;
     uint8_t *d_scratch = nullptr;
+    bool any_pending = true;
+    bool any_budget = false;
     int max_sites = 0, total_width = 0;
     int64_t max_records = 0;
     bool inited = false;
@@ -246,7 +249,7 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
     std::vector<DirectEntry> dt((size_t)m.n_types * dnv * dmw * 4);
     for (auto &x : dt) {
         for (int j = 0; j < 4; ++j) {
-            x.cum[j] = -1.0;
+            x.K[j] = 0;
             x.out[j] = 0;
         }
         x.count = 0;
@@ -268,7 +271,11 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
                         x.count = (uint32_t)cnt;
                         x.trans = (uint32_t)(ebase + off) | ((uint32_t)cnt << 20);
                         for (int j = 0; j < cnt && j < 4; ++j) {
-                            x.cum[j] = vd.exit_cumprob[off + j];
+                            double c = vd.exit_cumprob[off + j];
+                            // r < c  <=>  (word >> 11) < ceil(c * 2^53)
+                            x.K[j] = c <= 0.0 ? 0ull
+                                     : c >= 1.0 ? (1ull << 53)
+                                                : (uint64_t)std::ceil(std::ldexp(c, 53));
                             x.out[j] = (uint32_t)vd.exit_target[off + j] |
                                        ((uint32_t)vd.exit_leg[off + j] << 10) |
                                        ((uint32_t)vd.exit_worm[off + j] << 12);
@@ -518,6 +525,8 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
     if (!rc)
         rc = dev_alloc(b, &b->d_count, (size_t)n_walkers);
     if (!rc)
+        rc = dev_alloc(b, &b->d_done, (size_t)n_walkers);
+    if (!rc)
         rc = dev_alloc(b, &b->d_walkers, (size_t)n_walkers);
     if (!rc) {
         b->h_walkers.resize(n_walkers);
@@ -553,6 +562,9 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
             x.last = b->d_last + (size_t)w * b->max_sites;
             x.meas_offset = b->offsets[w];
             x.meas_width = b->widths[w];
+            x.active = 0;
+            x.pending = 1;
+            x.remaining = -1;
         }
     }
     if (!rc && hipMemcpy(b->d_walkers, b->h_walkers.data(), sizeof(DevWalker) * n_walkers,
@@ -564,6 +576,7 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
         (void)hipMemset(b->d_sum, 0, sizeof(double) * b->total_width);
         (void)hipMemset(b->d_sumsq, 0, sizeof(double) * b->total_width);
         (void)hipMemset(b->d_count, 0, sizeof(int64_t) * n_walkers);
+        (void)hipMemset(b->d_done, 0, sizeof(int32_t) * n_walkers);
         rc = wave_configure(b->wave, b->h_models, b->max_sites);
     }
     if (rc) {
@@ -649,14 +662,14 @@ static void collect_phase_times(sse_batch *b, int n_sweeps, bool with_measure)
     b->phase_launches = n_sweeps;
 }
 
-int sse_init(sse_batch *b)
+// Carlo.init! of every pending walker (all walkers after creation, or the
+// slots restarted by sse_slot_start)
+static int launch_init_pending(sse_batch *b)
 {
-    if (!b)
-        return fail(SSE_ERR_INVALID, "null batch");
     int n = b->n_walkers;
     if (b->mode == SSE_MODE_LANE) {
         hipLaunchKernelGGL(lane_init_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream,
-                           b->d_walkers, b->d_models, b->d_pool, 0, n);
+                           b->d_walkers, b->d_models, b->d_pool, 0, n, 1);
         HIPCHECK(hipGetLastError());
     } else {
         int max_warmup = 0;
@@ -667,7 +680,17 @@ int sse_init(sse_batch *b)
         if (rc)
             return rc;
     }
-    int rc = check_walker_errors(b);
+    b->any_pending = false;
+    return SSE_OK;
+}
+
+int sse_init(sse_batch *b)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    int rc = launch_init_pending(b);
+    if (!rc)
+        rc = check_walker_errors(b);
     if (rc)
         return rc;
     b->inited = true;
@@ -696,6 +719,11 @@ int sse_sweep(sse_batch *b, int32_t n_sweeps)
         if (rc)
             return rc;
     }
+    if (b->any_budget) {
+        hipLaunchKernelGGL(retire_kernel, dim3((b->n_walkers + 63) / 64), dim3(64), 0, b->stream,
+                           b->d_walkers, b->n_walkers, b->d_done);
+        HIPCHECK(hipGetLastError());
+    }
     int rc = check_walker_errors(b);
     if (timed)
         collect_phase_times(b, n_sweeps, false);
@@ -722,11 +750,11 @@ static int launch_measure(sse_batch *b, int mode)
     if (b->mode == SSE_MODE_LANE) {
         hipLaunchKernelGGL(lane_measure_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream,
                            b->d_walkers, b->d_models, b->d_out, b->d_sum, b->d_sumsq, b->d_count,
-                           b->d_scratch, b->max_sites, n, mode);
+                           b->d_scratch, b->max_sites, n, mode, b->d_done);
     } else {
         int rc = wave_launch_measure(b->wave, b->stream, b->d_walkers, b->d_models, b->d_out,
                                      b->d_sum, b->d_sumsq, b->d_count, b->d_scratch, b->max_sites,
-                                     n, mode);
+                                     n, mode, b->d_done);
         if (rc)
             return rc;
     }
@@ -763,6 +791,11 @@ int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure)
         return fail(SSE_ERR_INVALID, "null batch");
     if (!b->inited)
         return fail(SSE_ERR_STATE, "run before init");
+    if (b->any_pending) {
+        int rc = launch_init_pending(b);
+        if (rc)
+            return rc;
+    }
     const bool timed = n_sweeps <= 256;
     if (timed) {
         int rc = ensure_events(b, 4 * (size_t)n_sweeps);
@@ -887,6 +920,9 @@ int sse_set_walker(sse_batch *b, int32_t walker, const sse_walker_scalars *s, co
     w.ctr = s->rng_counter;
     w.sweeps = s->sweeps;
     w.error = 0;
+    w.active = 1;
+    w.pending = 0;
+    w.retire = 0;
     if (w.L > 0)
         HIPCHECK(hipMemcpy(w.ops, ops, sizeof(uint32_t) * w.L, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(w.spins, spins, w.n_sites, hipMemcpyHostToDevice));
@@ -952,6 +988,211 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links, int64_t li
         HIPCHECK(hipMemcpy(first, w.first, sizeof(int32_t) * w.n_sites, hipMemcpyDeviceToHost));
     if (last)
         HIPCHECK(hipMemcpy(last, w.last, sizeof(int32_t) * w.n_sites, hipMemcpyDeviceToHost));
+    return SSE_OK;
+}
+
+// ---------------------------------------------------------------- serving
+
+// Re-pack every live slab into a fresh pool (the bump allocator never frees).
+// Slots that finished their budget keep no slab (their results live in the
+// accumulators).  Only called while no kernel runs.
+static int compact_pool(sse_batch *b)
+{
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    std::vector<DevWalker> h(b->n_walkers);
+    HIPCHECK(hipMemcpy(h.data(), b->d_walkers, sizeof(DevWalker) * b->n_walkers,
+                       hipMemcpyDeviceToHost));
+    char *np = nullptr;
+    if (hipMalloc((void **)&np, b->pool_size) != hipSuccess)
+        return fail(SSE_ERR_OOM, "pool compaction: no memory for the new pool");
+    size_t next = 0;
+    for (auto &w : h) {
+        bool live = w.active || w.pending || w.remaining != 0;
+        int64_t cap = live ? w.cap : 64;
+        size_t bytes = slab_bytes(cap);
+        if (next + bytes > b->pool_size) {
+            (void)hipFree(np);
+            return fail(SSE_ERR_OOM, "pool compaction: live slabs exceed the pool");
+        }
+        char *dst = np + next;
+        next += bytes;
+        if (live && w.L > 0)
+            HIPCHECK(hipMemcpyAsync(dst, w.ops, sizeof(uint32_t) * w.L, hipMemcpyDeviceToDevice,
+                                    b->stream));
+        if (!live)
+            w.L = 0;
+        w.ops = (uint32_t *)dst;
+        w.links = (int32_t *)(dst + (size_t)cap * 4);
+        w.cap = cap;
+    }
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    for (auto it = b->allocs.begin(); it != b->allocs.end(); ++it)
+        if (*it == (void *)b->pool_base) {
+            b->allocs.erase(it);
+            break;
+        }
+    (void)hipFree(b->pool_base);
+    b->pool_base = np;
+    b->allocs.push_back(np);
+    b->pool_host_next = next;
+    HIPCHECK(hipMemcpy(b->d_walkers, h.data(), sizeof(DevWalker) * b->n_walkers,
+                       hipMemcpyHostToDevice));
+    return sync_pool_to_device(b);
+}
+
+int sse_add_model(sse_batch *b, const sse_model_desc *model, int32_t *index)
+{
+    if (!b || !model || !index)
+        return fail(SSE_ERR_INVALID, "sse_add_model: bad arguments");
+    int rc = validate_model(*model, b->n_models);
+    if (rc)
+        return rc;
+    if (model->n_sites > b->max_sites)
+        return fail(SSE_ERR_INVALID, "sse_add_model: more sites than the batch was created for");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    DevModel dm;
+    rc = upload_model(b, *model, dm);
+    if (rc)
+        return rc;
+    b->h_models.push_back(dm);
+    DevModel *nd = nullptr;
+    rc = dev_alloc(b, &nd, b->h_models.size());
+    if (rc)
+        return rc;
+    HIPCHECK(hipMemcpy(nd, b->h_models.data(), sizeof(DevModel) * b->h_models.size(),
+                       hipMemcpyHostToDevice));
+    b->d_models = nd;
+    *index = b->n_models++;
+    return SSE_OK;
+}
+
+int sse_slot_start(sse_batch *b, int32_t slot, const sse_walker_desc *wd, int64_t n_sweeps)
+{
+    if (!b || !wd || slot < 0 || slot >= b->n_walkers)
+        return fail(SSE_ERR_INVALID, "sse_slot_start: bad slot");
+    if (wd->model < 0 || wd->model >= b->n_models || !(wd->T > 0) || wd->init_num_worms < 1 ||
+        wd->diagonal_warmup_sweeps < 0 || n_sweeps == 0)
+        return fail(SSE_ERR_INVALID, "sse_slot_start: invalid walker parameters");
+    const DevModel &m = b->h_models[wd->model];
+    int width = 5 + 5 * m.n_est;
+    if (width > b->widths[slot])
+        return fail(SSE_ERR_INVALID, "sse_slot_start: model has more estimators than the slot");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    int rc = sync_pool_from_device(b);
+    if (rc)
+        return rc;
+    DevWalker x;
+    HIPCHECK(hipMemcpy(&x, b->d_walkers + slot, sizeof x, hipMemcpyDeviceToHost));
+    int64_t L0 = wd->init_opstring_cutoff >= 0 ? wd->init_opstring_cutoff
+                                               : (int64_t)std::nearbyint((double)m.n_sites * wd->T);
+    int64_t cap = slab_capacity(L0);
+    if (cap > x.cap || !x.ops) {
+        char *s = host_slab(b, cap);
+        if (!s) {
+            x.active = 0;
+            x.pending = 0;
+            x.remaining = 0;  // release the old slab during compaction
+            HIPCHECK(hipMemcpy(b->d_walkers + slot, &x, sizeof x, hipMemcpyHostToDevice));
+            rc = compact_pool(b);
+            if (rc)
+                return rc;
+            s = host_slab(b, cap);
+            if (!s)
+                return fail(SSE_ERR_OOM, "slab pool full; raise SSE_POOL_MB");
+        }
+        x.ops = (uint32_t *)s;
+        x.links = (int32_t *)(s + (size_t)cap * 4);
+        x.cap = cap;
+        rc = sync_pool_to_device(b);
+        if (rc)
+            return rc;
+    }
+    x.model = wd->model;
+    x.warmup = wd->diagonal_warmup_sweeps;
+    x.T = wd->T;
+    x.target_wlf = wd->target_worm_length_fraction;
+    x.atten = wd->num_worms_attenuation_factor;
+    x.key = wd->seed;
+    x.thermalization = wd->thermalization;
+    x.init_cutoff = L0;
+    x.avg_worm_length = 1.0;
+    x.num_worms = wd->init_num_worms;
+    x.L = L0;
+    x.n = 0;
+    x.ctr = 0;
+    x.sweeps = 0;
+    x.measured = 0;
+    x.last_scatter = 0;
+    x.last_total = 0;
+    x.last_wlf = NAN;
+    x.error = 0;
+    x.n_sites = m.n_sites;
+    x.acc_L = x.acc_n = x.acc_W = x.acc_sweeps = 0;
+    x.meas_width = width;
+    x.active = 0;
+    x.pending = 1;
+    x.retire = 0;
+    x.remaining = n_sweeps;
+    HIPCHECK(hipMemcpy(b->d_walkers + slot, &x, sizeof x, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(b->d_sum + x.meas_offset, 0, sizeof(double) * b->widths[slot]));
+    HIPCHECK(hipMemset(b->d_sumsq + x.meas_offset, 0, sizeof(double) * b->widths[slot]));
+    HIPCHECK(hipMemset(b->d_count + slot, 0, sizeof(int64_t)));
+    HIPCHECK(hipMemset(b->d_done + slot, 0, sizeof(int32_t)));
+    b->any_pending = true;
+    if (n_sweeps > 0)
+        b->any_budget = true;
+    b->inited = true;
+    return SSE_OK;
+}
+
+int sse_slot_stop(sse_batch *b, int32_t slot)
+{
+    if (!b || slot < 0 || slot >= b->n_walkers)
+        return fail(SSE_ERR_INVALID, "sse_slot_stop: bad slot");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    DevWalker x;
+    HIPCHECK(hipMemcpy(&x, b->d_walkers + slot, sizeof x, hipMemcpyDeviceToHost));
+    x.active = 0;
+    x.pending = 0;
+    x.retire = 0;
+    x.remaining = 0;
+    HIPCHECK(hipMemcpy(b->d_walkers + slot, &x, sizeof x, hipMemcpyHostToDevice));
+    b->inited = true;
+    return SSE_OK;
+}
+
+int sse_slot_poll(sse_batch *b, int32_t *done_slots, int32_t capacity, int32_t *n_done)
+{
+    if (!b || !n_done)
+        return fail(SSE_ERR_INVALID, "sse_slot_poll: bad arguments");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    std::vector<int32_t> d(b->n_walkers);
+    HIPCHECK(hipMemcpy(d.data(), b->d_done, sizeof(int32_t) * b->n_walkers, hipMemcpyDeviceToHost));
+    int32_t k = 0;
+    for (int i = 0; i < b->n_walkers; ++i)
+        if (d[i]) {
+            if (done_slots && k < capacity)
+                done_slots[k] = i;
+            ++k;
+        }
+    if (k)
+        HIPCHECK(hipMemset(b->d_done, 0, sizeof(int32_t) * b->n_walkers));
+    *n_done = k;
+    return SSE_OK;
+}
+
+int sse_pool_usage(sse_batch *b, int64_t *used_bytes, int64_t *total_bytes)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    int rc = sync_pool_from_device(b);
+    if (rc)
+        return rc;
+    if (used_bytes)
+        *used_bytes = (int64_t)b->pool_host_next;
+    if (total_bytes)
+        *total_bytes = (int64_t)b->pool_size;
     return SSE_OK;
 }
 

@@ -65,8 +65,11 @@ struct DevModel {
     int32_t dt_nv, dt_mw, dtab_bytes, pad_dt;
 };
 
+// Exit j is taken by the first j with r < cum[j], r = (word >> 11) * 2^-53.
+// With m = word >> 11 (an integer < 2^53) and K = ceil(cum * 2^53) this is
+// exactly m < K, so the worm kernel compares integers; unused slots K = 0.
 struct __align__(16) DirectEntry {
-    double cum[4];     // cumulative probabilities; unused slots -1 (never taken)
+    uint64_t K[4];     // integer thresholds ceil(cum[j] * 2^53)
     uint32_t out[4];   // target vcode | leg_out << 10 | worm_out << 12
     uint32_t count;    // number of exits
     uint32_t trans;    // wtab trans word (exit index | count << 20)
@@ -97,6 +100,11 @@ struct DevWalker {
     // work counters (sum over sweeps since reset)
     int64_t acc_L, acc_n, acc_W, acc_sweeps;
     int32_t meas_offset, meas_width;
+    // serving slots: active walkers are swept; a finite budget of sweeps
+    // (remaining >= 0) retires the walker after its last sweep + measure
+    int32_t active, pending;
+    int64_t remaining;
+    int32_t retire, slot_pad;
 };
 
 struct DevPool {

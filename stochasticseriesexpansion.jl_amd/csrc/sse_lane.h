@@ -211,7 +211,7 @@ __global__ void lane_sweep_kernel(DevWalker *W, const DevModel *models, DevPool 
     if (wi >= w_end)
         return;
     DevWalker w = W[wi];
-    if (w.error)
+    if (w.error || !w.active)
         return;
     const DevModel &m = models[w.model];
     bool therm = w.sweeps > w.thermalization;
@@ -238,18 +238,22 @@ __global__ void lane_sweep_kernel(DevWalker *W, const DevModel *models, DevPool 
         w.acc_n += w.n;
         w.acc_W += w.last_scatter;
         w.acc_sweeps += 1;
+        if (w.remaining > 0 && --w.remaining == 0)
+            w.retire = 1;
     }
     W[wi] = w;
 }
 
 // Carlo.init! (sse.jl:47-60): spins, identity opstring, diagonal warm-up.
 __global__ void lane_init_kernel(DevWalker *W, const DevModel *models, DevPool *pool, int w_begin,
-                                 int w_end)
+                                 int w_end, int pending_only)
 {
     int wi = w_begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (wi >= w_end)
         return;
     DevWalker w = W[wi];
+    if (pending_only && !w.pending)
+        return;
     const DevModel &m = models[w.model];
     LaneRng rng;
     rng.init(w.key, 0);
@@ -266,6 +270,8 @@ __global__ void lane_init_kernel(DevWalker *W, const DevModel *models, DevPool *
         lane_diagonal(w, m, rng);
     }
     w.ctr = rng.ctr;
+    w.pending = 0;
+    w.active = 1;
     W[wi] = w;
 }
 
@@ -347,22 +353,27 @@ __device__ __forceinline__ void accumulate(double *sum, double *sumsq, const dou
 //         sweeps > thermalization) and accumulate into sums
 __global__ void lane_measure_kernel(DevWalker *W, const DevModel *models, double *out, double *sum,
                                     double *sumsq, int64_t *count, uint8_t *scratch_spins,
-                                    int max_sites, int n_walkers, int mode)
+                                    int max_sites, int n_walkers, int mode, int32_t *done)
 {
     int wi = blockIdx.x * blockDim.x + threadIdx.x;
     if (wi >= n_walkers)
         return;
     const DevWalker &w = W[wi];
-    if (w.error)
-        return;
-    if (mode == 1 && !(w.sweeps > w.thermalization))
+    if (w.error || (mode == 1 && !w.active))
         return;
     const DevModel &m = models[w.model];
     double *o = out + w.meas_offset;
-    measure_walker_serial(w, m, o, scratch_spins + (size_t)wi * max_sites);
-    if (mode == 1) {
-        accumulate(sum + w.meas_offset, sumsq + w.meas_offset, o, w.meas_width);
-        count[wi] += 1;
+    if (mode == 0 || w.sweeps > w.thermalization) {
+        measure_walker_serial(w, m, o, scratch_spins + (size_t)wi * max_sites);
+        if (mode == 1) {
+            accumulate(sum + w.meas_offset, sumsq + w.meas_offset, o, w.meas_width);
+            count[wi] += 1;
+        }
+    }
+    if (mode == 1 && w.retire) {
+        W[wi].retire = 0;
+        W[wi].active = 0;
+        done[wi] = 1;
     }
 }
 
@@ -376,4 +387,17 @@ __global__ void reset_work_counters_kernel(DevWalker *W, int n_walkers)
     W[wi].acc_n = 0;
     W[wi].acc_W = 0;
     W[wi].acc_sweeps = 0;
+}
+
+// retire walkers whose sweep budget ran out (sweeps without measurement)
+__global__ void retire_kernel(DevWalker *W, int n_walkers, int32_t *done)
+{
+    int wi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (wi >= n_walkers)
+        return;
+    if (W[wi].active && W[wi].retire) {
+        W[wi].retire = 0;
+        W[wi].active = 0;
+        done[wi] = 1;
+    }
 }

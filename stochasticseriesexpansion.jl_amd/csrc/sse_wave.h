@@ -44,12 +44,13 @@ __device__ __forceinline__ int popc(uint64_t x) { return __popcll(x); }
 // Philox words of one walker staged in LDS: buf holds words
 // [bbase, bbase+256) and the next word to draw is bbase+off.  All members are
 // wave-uniform; every call must be made by the whole wave.
+template <int NW>
 __device__ __noinline__ void wave_refill(uint64_t *buf, uint64_t key, uint64_t base)
 {
     const int l = threadIdx.x;
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < NW / 128; ++r) {
         int j = l + 64 * r;
         uint32_t o0, o1, o2, o3;
         philox_block((base >> 1) + (uint64_t)j, key, o0, o1, o2, o3);
@@ -59,7 +60,8 @@ __device__ __noinline__ void wave_refill(uint64_t *buf, uint64_t key, uint64_t b
     __syncthreads();
 }
 
-struct WaveStream {
+template <int NW>
+struct WaveStreamT {
     uint64_t *buf;
     uint64_t key, bbase;
     uint32_t off;  // next word is bbase + off; off > 256 marks an empty buffer
@@ -78,12 +80,12 @@ struct WaveStream {
             const uint64_t c = bbase;
             bbase = c & ~1ull;
             off = (uint32_t)(c - bbase);
-            wave_refill(buf, key, bbase);
-        } else if (off + need > 256) {
+            wave_refill<NW>(buf, key, bbase);
+        } else if (off + need > NW) {
             const uint64_t c = bbase + off;
             bbase = c & ~1ull;
             off = (uint32_t)(c - bbase);
-            wave_refill(buf, key, bbase);
+            wave_refill<NW>(buf, key, bbase);
         }
     }
     // word k positions after the next one (after ensure)
@@ -112,6 +114,9 @@ struct WaveStream {
         return hi;
     }
 };
+
+typedef WaveStreamT<256> WaveStream;
+typedef WaveStreamT<128> WaveStream128;
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ uint64_t uni64(uint64_t x)
@@ -236,7 +241,7 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
     DevWalker w = W[wi];
     if (w.error)
         return;
-    if (warm >= 0 && warm >= w.warmup)
+    if (warm >= 0 ? (warm >= w.warmup || !w.pending) : !w.active)
         return;
     const DevModel &m = models[w.model];
     const int N = m.n_sites;
@@ -634,14 +639,17 @@ struct LinkType<RecGlobal> {
     typedef uint4 T;
 };
 
+typedef const __attribute__((address_space(4))) DirectEntry ConstEntry;
+
 // The worm update of one walker (sse.jl:193-303), executed uniformly by the
-// whole wave: per chase step one record load (vcode + links) and one direct
-// table entry (up to 4 exits inline).  The chase state is made scalar with
-// readfirstlane (every lane holds the same values), so the integer work and
-// the branches run on the scalar unit.  Returns an error code (0 = ok).
-template <class R>
-__device__ int worm_body(DevWalker &w, const DevModel &m, WaveStream &rs, const R &rec,
-                         const uint64_t *MASK, const uint32_t *BASE, const DirectEntry *DT,
+// whole wave.  The chase is scalar work: every lane holds the same state, the
+// records come from LDS (made scalar with readfirstlane), the transition
+// entry arrives through a scalar load (constant address space, uniform index)
+// and the exit is chosen by integer compares of the Philox word against the
+// entry's thresholds.  Returns an error code (0 = ok).
+template <class R, class RS>
+__device__ int worm_body(DevWalker &w, const DevModel &m, RS &rs, const R &rec,
+                         const uint64_t *MASK, const uint32_t *BASE, ConstEntry *DT,
                          double &total, int64_t &scat)
 {
     const int lane = threadIdx.x;
@@ -681,17 +689,23 @@ __device__ int worm_body(DevWalker &w, const DevModel &m, WaveStream &rs, const 
             const uint32_t vidx = (vt & 1023u) >> 1, t = vt >> 10;
             if (vidx >= dnv || (uint32_t)(wf - 1) >= dmw)
                 return SSE_ERR_STATE;
-            const DirectEntry &de = DT[((t * dnv + vidx) * dmw + (uint32_t)(wf - 1)) * 4u + leg_in];
-            const double c0 = de.cum[0], c1 = de.cum[1], c2 = de.cum[2], c3 = de.cum[3];
+            const uint32_t ei = uni(((t * dnv + vidx) * dmw + (uint32_t)(wf - 1)) * 4u + (uint32_t)leg_in);
+            ConstEntry &de = DT[ei];
+            // whole entry in scalar registers before any branch
+            const uint64_t K0 = de.K[0], K1 = de.K[1], K2 = de.K[2], K3 = de.K[3];
             const uint32_t o0 = de.out[0], o1 = de.out[1], o2 = de.out[2], o3 = de.out[3];
-            const uint32_t cnt = uni(de.count);
-            const double x = word_to_float(uni64(rs.next()));
+            const uint32_t cnt = de.count, trw = de.trans;
+            const uint64_t word = uni64(rs.next());
+            const uint64_t mm = word >> 11;  // rand() = mm * 2^-53
             sc += 1;
-            uint32_t eo = (x < c0) ? o0 : (x < c1) ? o1 : (x < c2) ? o2 : (x < c3) ? o3 : 0xFFFFFFFFu;
-            eo = uni(eo);
-            if (cnt > 4u) {
+            uint32_t eo = mm < K0 ? o0 : mm < K1 ? o1 : mm < K2 ? o2 : mm < K3 ? o3 : 0xFFFFFFFFu;
+            if (cnt <= 4u) {
+                if (eo == 0xFFFFFFFFu)
+                    return SSE_ERR_SCATTER;  // fall-through (vertex_data.jl:124) or invalid
+            } else {
                 const WormTab tab = worm_tab((const char *)m.wtab, m);
-                const uint32_t tr = uni(de.trans);
+                const double x = word_to_float(word);
+                const uint32_t tr = trw;
                 uint32_t e = tr & 0xFFFFFu;
                 const uint32_t eend = e + (tr >> 20);
                 while (e < eend && !(x < tab.cum[e]))
@@ -699,8 +713,6 @@ __device__ int worm_body(DevWalker &w, const DevModel &m, WaveStream &rs, const 
                 if (e >= eend)
                     return SSE_ERR_SCATTER;
                 eo = uni(tab.eout[e]);
-            } else if (eo == 0xFFFFFFFFu) {
-                return SSE_ERR_SCATTER;  // fall-through (vertex_data.jl:124) or invalid
             }
             const int leg_out = (int)((eo >> 10) & 3u);
             const int wf_out = (int)(eo >> 12);
@@ -724,8 +736,8 @@ __device__ int worm_body(DevWalker &w, const DevModel &m, WaveStream &rs, const 
 }
 
 // state refresh (sse.jl:219-228) and operator write-back after the worms
-template <class R>
-__device__ void worm_finish(DevWalker &w, const DevModel &m, WaveStream &rs, const R &rec,
+template <class R, class RS>
+__device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &rec,
                             const uint64_t *MASK, const uint32_t *BASE)
 {
     const int lane = threadIdx.x;
@@ -784,18 +796,15 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, WaveStream &rs, con
     }
 }
 
-#define SSE_DTAB_LDS_MAX 32768
-
 // LDS bytes the worm kernel needs to hold a walker with n operators, cutoff L
-__host__ __device__ inline size_t k2_lds_need(int64_t n, int64_t L, int dtab_bytes)
+__host__ __device__ inline size_t k2_lds_need(int64_t n, int64_t L)
 {
     size_t nch = (size_t)((L + 63) / 64);
-    size_t dt = dtab_bytes <= SSE_DTAB_LDS_MAX ? (size_t)dtab_bytes : 0;
-    return 2048 + dt + nch * 8 + ((nch * 4 + 15) & ~(size_t)15) + (size_t)n * 8 +
+    return 1024 + nch * 8 + ((nch * 4 + 15) & ~(size_t)15) + (size_t)n * 8 +
            (((size_t)n * 2 + 15) & ~(size_t)15);
 }
 
-// K2: LDS layout  rbuf u64[256] | direct table | mask u64[nch] | base u32[nch] |
+// K2: LDS layout  rbuf u64[128] | mask u64[nch] | base u32[nch] |
 //                 links u64[n] | vt u16[n]
 // Walkers whose records do not fit (or n >= 16384, or > 64 bond types) chase
 // their records in global memory instead.
@@ -809,29 +818,21 @@ __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevMo
         return;
     const int lane = threadIdx.x;
     DevWalker w = W[wi];
-    if (w.error)
+    if (w.error || !w.active)
         return;
     const DevModel &m = models[w.model];
     const bool therm = w.sweeps > w.thermalization;
     uint64_t *rbuf = (uint64_t *)lds;
-    WaveStream rs;
+    WaveStream128 rs;
     rs.init(rbuf, w.key, w.ctr);
     const int64_t nch = (w.L + 63) / 64;
-    const bool fits = w.n < 16384 && m.n_types <= 64 && m.dtab_bytes <= SSE_DTAB_LDS_MAX &&
-                      k2_lds_need(w.n, w.L, m.dtab_bytes) <= (size_t)lds_bytes;
+    const bool fits = w.n < 16384 && m.n_types <= 64 && k2_lds_need(w.n, w.L) <= (size_t)lds_bytes;
+    ConstEntry *DT = (ConstEntry *)(const void *)m.dtab;
     double total = 1.0;
     int64_t scat = 0;
     int err;
     if (fits) {
-        char *q = lds + 2048;
-        {
-            uint4 *ldt = (uint4 *)q;
-            const uint4 *gdt = (const uint4 *)m.dtab;
-            for (int i = lane; i < m.dtab_bytes / 16; i += 64)
-                ldt[i] = gdt[i];
-        }
-        const DirectEntry *DT = (const DirectEntry *)q;
-        q += m.dtab_bytes;
+        char *q = lds + 1024;
         uint64_t *lmask = (uint64_t *)q;
         q += nch * 8;
         uint32_t *lbase = (uint32_t *)q;
@@ -861,7 +862,7 @@ __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevMo
             worm_finish(w, m, rs, rec, lmask, lbase);
     } else {
         RecGlobal rec{rec_vt(w), (const uint4 *)rec_link(w)};
-        err = worm_body(w, m, rs, rec, rec_mask(w), rec_base(w), m.dtab, total, scat);
+        err = worm_body(w, m, rs, rec, rec_mask(w), rec_base(w), DT, total, scat);
         __syncthreads();
         if (!err)
             worm_finish(w, m, rs, rec, rec_mask(w), rec_base(w));
@@ -889,6 +890,11 @@ __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevMo
             o.acc_n = w.acc_n + w.n;
             o.acc_W = w.acc_W + scat;
             o.acc_sweeps = w.acc_sweeps + 1;
+            if (w.remaining > 0) {
+                o.remaining = w.remaining - 1;
+                if (w.remaining == 1)
+                    o.retire = 1;
+            }
         }
     }
 }
@@ -902,6 +908,8 @@ __global__ __launch_bounds__(64) void wave_init_kernel(DevWalker *W, const DevMo
         return;
     const int lane = threadIdx.x;
     DevWalker w = W[wi];
+    if (!w.pending)
+        return;
     const DevModel &m = models[w.model];
     WaveStream rs;
     rs.init(rbuf, w.key, 0);
@@ -944,17 +952,24 @@ __device__ __forceinline__ double wave_incl_scan(double x)
 // accumulated into the per-walker sums (fused sweep+measure loop).
 __global__ __launch_bounds__(64) void wave_measure_kernel(DevWalker *W, const DevModel *models,
                                                           double *out, double *sum, double *sumsq,
-                                                          int64_t *count, int n_walkers, int mode)
+                                                          int64_t *count, int n_walkers, int mode,
+                                                          int32_t *done)
 {
     const int wi = blockIdx.x;
     if (wi >= n_walkers)
         return;
     const int lane = threadIdx.x;
     const DevWalker w = W[wi];
-    if (w.error)
+    if (w.error || (mode == 1 && !w.active))
         return;
-    if (mode == 1 && !(w.sweeps > w.thermalization))
+    if (mode == 1 && !(w.sweeps > w.thermalization)) {
+        if (w.retire && lane == 0) {
+            W[wi].retire = 0;
+            W[wi].active = 0;
+            done[wi] = 1;
+        }
         return;
+    }
     const DevModel &m = models[w.model];
     const int64_t L = w.L;
     // sign (sse.jl:305-314)
@@ -1041,8 +1056,26 @@ __global__ __launch_bounds__(64) void wave_measure_kernel(DevWalker *W, const De
             sum[w.meas_offset + j] += x;
             sumsq[w.meas_offset + j] += x * x;
         }
-        if (lane == 0)
+        if (lane == 0) {
             count[wi] += 1;
+            if (w.retire) {
+                W[wi].retire = 0;
+                W[wi].active = 0;
+                done[wi] = 1;
+            }
+        }
+    }
+}
+
+// pending walkers become active once initialised
+__global__ void activate_pending_kernel(DevWalker *W, int n_walkers)
+{
+    int wi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (wi >= n_walkers)
+        return;
+    if (W[wi].pending) {
+        W[wi].pending = 0;
+        W[wi].active = 1;
     }
 }
 
@@ -1060,7 +1093,7 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
         return SSE_ERR_INVALID;
     for (const DevModel &m : models)
         c.max_wtab = std::max(c.max_wtab, (int)m.dtab_bytes);
-    c.lds_k2 = 2048 + 4096;
+    c.lds_k2 = 1024 + 4096;
     if (c.lds_limit > 65536) {
         (void)hipFuncSetAttribute((const void *)wave_worm_kernel,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_limit);
@@ -1074,8 +1107,8 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
 // headroom; walkers that still do not fit use the global-memory chase.
 static inline void wave_resize_lds(WaveConfig &c, unsigned long long max_n, unsigned long long max_L)
 {
-    size_t need = k2_lds_need((int64_t)(max_n + max_n / 8 + 64), (int64_t)max_L, c.max_wtab);
-    need = (need + 1023) & ~(size_t)1023;
+    size_t need = k2_lds_need((int64_t)(max_n + max_n / 16 + 32), (int64_t)max_L);
+    need = (need + 511) & ~(size_t)511;
     if (need > c.lds_limit)
         need = c.lds_limit;
     if (need > c.lds_k2)
@@ -1088,6 +1121,7 @@ static inline int wave_launch_init(WaveConfig &c, hipStream_t s, DevWalker *W, c
     hipLaunchKernelGGL(wave_init_kernel, dim3(n), dim3(64), 0, s, W, M, n);
     for (int r = 0; r < max_warmup; ++r)
         hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, n, 1, r);
+    hipLaunchKernelGGL(activate_pending_kernel, dim3((n + 63) / 64), dim3(64), 0, s, W, n);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
 
@@ -1110,10 +1144,10 @@ static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, 
 
 static inline int wave_launch_measure(WaveConfig &, hipStream_t s, DevWalker *W, const DevModel *M,
                                       double *out, double *sum, double *sumsq, int64_t *count,
-                                      uint8_t *, int, int n, int mode)
+                                      uint8_t *, int, int n, int mode, int32_t *done)
 {
     hipLaunchKernelGGL(wave_measure_kernel, dim3(n), dim3(64), 0, s, W, M, out, sum, sumsq, count, n,
-                       mode);
+                       mode, done);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
 

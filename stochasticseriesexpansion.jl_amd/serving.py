@@ -1,0 +1,174 @@
+"""Serving: continuous batching of simulation jobs on one GPU.
+
+Jobs (a model, temperature, seed and a number of sweeps) arrive as an
+open-loop stream.  Each job is one walker in a slot of a resident batch: it is
+admitted when a slot is free and its estimated opstring+record footprint fits
+the memory budget, initialised on the device while the other slots keep
+sweeping, and retired after its last sweep; its binned results (device
+accumulators) stream back as soon as it finishes.  The loop reports p50/p99
+time-to-result beside the aggregate vertex-update throughput.
+
+This replaces Carlo's task farming (one MC per task per rank) for the
+serving use; each job's result is independent of what else runs, because a
+walker's chain depends only on its own seed and parameters.
+"""
+from __future__ import annotations
+
+import heapq
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+from .lattice import UnitCells
+from .models import MagnetModel
+
+
+@dataclass
+class Job:
+    job_id: int
+    arrival: float          # seconds after start
+    model_key: tuple        # (L, kind)
+    T: float
+    sweeps: int
+    thermalization: int
+    seed: int
+    # filled by the loop
+    slot: int = -1
+    admitted: float = -1.0
+    finished: float = -1.0
+    result: tuple = None
+
+
+def magnet_square(L, kind):
+    """Heisenberg (kind 'heis') or XXZ with Jz = 0.5 (kind 'xxz') on an LxL square lattice."""
+    p = {"lattice": {"unitcell": UnitCells.square, "size": (L, L)}, "J": 1.0,
+         "measure": ["magnetization", "staggered_magnetization"]}
+    if kind == "xxz":
+        p["d"] = -0.5
+    return MagnetModel(p)
+
+
+def poisson_jobs(n_jobs, rate=200.0, seed=7, L_range=(8, 48), beta_range=(1.0, 32.0),
+                 sweeps_range=(1e3, 1e5), sweep_scale=1.0, therm_fraction=0.1):
+    """Open-loop Poisson arrival stream: exponential gaps of mean 1/rate,
+    L uniform, beta and sweeps log-uniform, Heisenberg/XXZ 50/50."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for j in range(n_jobs):
+        t += rng.exponential(1.0 / rate)
+        L = int(rng.integers(L_range[0], L_range[1] + 1))
+        kind = "heis" if rng.random() < 0.5 else "xxz"
+        beta = float(np.exp(rng.uniform(np.log(beta_range[0]), np.log(beta_range[1]))))
+        sw = float(np.exp(rng.uniform(np.log(sweeps_range[0]), np.log(sweeps_range[1]))))
+        sw = max(2, int(round(sw * sweep_scale)))
+        th = max(1, int(round(sw * therm_fraction)))
+        jobs.append(Job(j, t, (L, kind), 1.0 / beta, sw, th, seed * 1000003 + j))
+    return jobs
+
+
+def footprint_bytes(n_sites, n_bonds, T):
+    """Slab bytes a job needs at its steady-state cutoff: n ~ beta * (|E| +
+    sum of bond offsets) bounded by beta * 0.375 * Nb * 2.5, L ~ 2.2 n."""
+    beta = 1.0 / T
+    n_est = beta * n_bonds * 0.375 * 2.5 + 64
+    L = int(2.2 * n_est) + 128
+    cap = L + L // 2 + 100
+    return 24 * cap + 16 * (cap // 64 + 1)
+
+
+class ServingLoop:
+    def __init__(self, n_slots=512, max_L=48, memory_budget=8 << 30, sweeps_per_step=10,
+                 lib=None):
+        self.model_cache = {}
+        self.flats = []
+        proto = magnet_square(max_L, "heis").flat()
+        # slot 0..n-1 start stopped; the prototype model sizes the per-walker arrays
+        walkers = [capi.walker_desc(model=0, T=1.0, seed=0) for _ in range(n_slots)]
+        self.batch = capi.Batch([proto], walkers, lib=lib)
+        for s in range(n_slots):
+            self.batch.slot_stop(s)
+        self.model_cache[(max_L, "heis")] = 0
+        self.n_slots = n_slots
+        self.memory_budget = memory_budget
+        self.sweeps_per_step = sweeps_per_step
+
+    def model_index(self, key):
+        if key not in self.model_cache:
+            flat = magnet_square(*key).flat()
+            self.model_cache[key] = self.batch.add_model(flat)
+        return self.model_cache[key]
+
+    def run(self, jobs, time_limit=None, log=None):
+        """Serve `jobs` (sorted by arrival); returns the finished jobs and stats."""
+        b = self.batch
+        pending = list(jobs)
+        queue = []           # arrived, not admitted (FIFO)
+        free = list(range(self.n_slots))
+        running = {}         # slot -> (job, footprint)
+        used = 0
+        done = []
+        t0 = time.perf_counter()
+        b.reset_accumulators()
+        L0, n0, W0, s0 = b.work_counters()
+        steps = 0
+        while pending or queue or running:
+            now = time.perf_counter() - t0
+            if time_limit is not None and now > time_limit:
+                break
+            while pending and pending[0].arrival <= now:
+                queue.append(pending.pop(0))
+            # memory-based admission, FIFO
+            while queue and free:
+                job = queue[0]
+                mi = self.model_index(job.model_key)
+                flat = b.flats[mi]
+                fp = footprint_bytes(flat.n_sites, flat.n_bonds, job.T)
+                if used + fp > self.memory_budget and running:
+                    break
+                queue.pop(0)
+                slot = free.pop()
+                wd = capi.walker_desc(model=mi, T=job.T, seed=job.seed,
+                                      thermalization=job.thermalization)
+                b.slot_start(slot, wd, job.thermalization + job.sweeps)
+                job.slot = slot
+                job.admitted = time.perf_counter() - t0
+                running[slot] = (job, fp)
+                used += fp
+            if not running:
+                if pending:
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
+                continue
+            b.run(self.sweeps_per_step, measure=True)
+            steps += 1
+            finished = b.slot_poll()
+            if finished:
+                acc = b.accumulators()
+                tnow = time.perf_counter() - t0
+                for slot in finished:
+                    job, fp = running.pop(slot)
+                    job.finished = tnow
+                    sums, sq, cnt = acc[slot]
+                    job.result = (sums.copy(), sq.copy(), cnt)
+                    used -= fp
+                    free.append(slot)
+                    done.append(job)
+                if log:
+                    log(f"t={tnow:.2f}s done={len(done)} running={len(running)} "
+                        f"queued={len(queue)}")
+        wall = time.perf_counter() - t0
+        L1, n1, W1, s1 = b.work_counters()
+        lat = np.array([j.finished - j.arrival for j in done]) if done else np.zeros(1)
+        stats = {"jobs_done": len(done), "wall_s": wall,
+                 "vertex_updates_per_s": ((L1 - L0) + (W1 - W0)) / wall,
+                 "walker_sweeps": s1 - s0,
+                 "p50_time_to_result_s": float(np.percentile(lat, 50)),
+                 "p99_time_to_result_s": float(np.percentile(lat, 99)),
+                 "steps": steps}
+        return done, stats
+
+    def close(self):
+        self.batch.close()

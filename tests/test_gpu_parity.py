@@ -229,3 +229,36 @@ def test_parity_many_walkers_c2_shape(sse_pkg, oracle_mod, mode):
         for key in KEYS:
             assert gs[key] == os_[key]
     batch.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_parity_heterogeneous_batch(sse_pkg, oracle_mod, mode):
+    """One batch mixing lattices, temperatures and XXZ anisotropies (one model
+    per walker); every walker bit-exact against its own oracle."""
+    rng = np.random.default_rng(11)
+    flats, walkers = [], []
+    for k in range(8):
+        L = int(rng.choice([4, 6, 8]))
+        jz = float(rng.uniform(0.5, 1.5))
+        model = square_heisenberg(sse_pkg, L, d=jz - 1.0, hz=0.1)
+        flats.append(model.flat())
+        beta = float(np.exp(rng.uniform(np.log(0.5), np.log(16.0))))
+        walkers.append(sse_pkg.walker_desc(model=k, T=1.0 / beta, seed=300 + k, thermalization=6))
+    _say(f"heterogeneous: mode={mode}")
+    batch = sse_pkg.Batch(flats, walkers, mode=mode)
+    batch.init()
+    oracles = []
+    for k, w in enumerate(walkers):
+        o = oracle_mod.OracleWalker(sse_pkg.capi, flats[k], sse_pkg.walker_desc(
+            T=w.T, seed=w.seed, thermalization=6))
+        o.init()
+        oracles.append(o)
+    for s in range(15):
+        batch.sweep(1)
+        for o in oracles:
+            o.sweep(1)
+        _compare(batch, oracles, f"hetero sweep {s}")
+    gm = batch.measure()
+    for k, o in enumerate(oracles):
+        assert np.allclose(gm[k], o.measure(), rtol=1e-12, atol=1e-13)
+    batch.close()
