@@ -118,6 +118,118 @@ def cpu_baseline(sse, flat, wds, states, seconds, threads):
             "ns_per_operator_sweep": wall * threads / nops * 1e9 if nops else None}
 
 
+# ---------------------------------------------------------------- extras
+def _xxz_types(args):
+    """VertexData of the two square-lattice bond types for one walker's Jz
+    (J=1, d=Jz-1, h split over the 4 bonds of a site)."""
+    jz, h = args
+    import __graft_entry__
+    sse = __graft_entry__.load_package()
+    from sse_amd.models import MagnetBondParams, generate_bond_hamiltonian
+    bp = MagnetBondParams(1.0, jz - 1.0, (0.0, 0.0), (0.0, 0.0), (h / 4, h / 4))
+    dims, H, f = generate_bond_hamiltonian(None, bp, (2, 2))
+    vd = sse.VertexData(dims, H, energy_offset_factor=f)
+    return [vd, vd]
+
+
+def _ladder_flat(jl):
+    import __graft_entry__
+    sse = __graft_entry__.load_package()
+    m = sse.ClusterModel({
+        "inner_model": sse.MagnetModel, "cluster_bases": (sse.ClusterBases.dimer,),
+        "lattice": {"unitcell": sse.UnitCells.ladder, "size": (64, 1)},
+        "measure_quantum_numbers": [{"name": "", "quantum_number": 2}],
+        "parameter_map": {"J": ["Jr", "Jl", "Jl"]}, "Jr": 1.0, "Jl": jl})
+    return m.flat()
+
+
+def run_batch(sse, flats, wds, thermalization, steps, what):
+    batch = sse.Batch(flats, wds)
+    t0 = time.perf_counter()
+    batch.init()
+    done = 0
+    while done < thermalization:
+        k = min(100, thermalization - done)
+        batch.sweep(k)
+        done += k
+    log(f"{what}: thermalized {thermalization} sweeps in {time.perf_counter() - t0:.1f} s")
+    batch.run(1, measure=True)
+    batch.reset_accumulators()
+    t1 = time.perf_counter()
+    batch.set_option(sse.capi.SSE_OPT_TIMING, 0)
+    batch.run(steps, measure=True)
+    dt = time.perf_counter() - t1
+    Ls, ns, Ws, wsw = batch.work_counters()
+    batch.close()
+    return {"vertex_updates_per_s": (Ls + Ws) / dt, "sweeps_per_s": wsw / dt,
+            "walkers": len(wds), "timed_sweeps": steps, "thermalization_sweeps": thermalization,
+            "mean_L": Ls / wsw, "mean_n": ns / wsw, "mean_W": Ws / wsw}
+
+
+def extra_heterogeneous(sse, walkers=2048, thermalization=500, steps=20, seed=3):
+    """2048 walkers, square L in {8,12,16,24,32}, beta log-uniform [0.5,16],
+    XXZ with Jz ~ U[0.5,1.5], Jxy=1, h=0.1; one model per walker."""
+    from concurrent.futures import ProcessPoolExecutor
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([8, 12, 16, 24, 32], size=walkers)
+    betas = np.exp(rng.uniform(np.log(0.5), np.log(16.0), size=walkers))
+    jzs = rng.uniform(0.5, 1.5, size=walkers)
+    base = {L: sse.MagnetModel({"lattice": {"unitcell": sse.UnitCells.square, "size": (L, L)},
+                                "J": 1.0, "d": 0.0, "hz": 0.1,
+                                "measure": ["magnetization", "staggered_magnetization"]})
+            for L in (8, 12, 16, 24, 32)}
+    base_flat = {L: m.flat() for L, m in base.items()}
+    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        types = list(ex.map(_xxz_types, [(float(j), 0.1) for j in jzs], chunksize=32))
+    from sse_amd.sse_data import FlatModel, SSEData
+    bonds = {L: [sse.SSEBond(b.type, (b.i, b.j)) for b in m.lattice.bonds] for L, m in base.items()}
+    flats, wds = [], []
+    for k in range(walkers):
+        L = int(sizes[k])
+        m = base[L]
+        flats.append(FlatModel(SSEData(types[k], bonds[L]), m.opstring_estimators,
+                               m.normalization_site_count(),
+                               (base_flat[L].est_coef, base_flat[L].est_mval)))
+        wds.append(sse.walker_desc(model=k, T=1.0 / float(betas[k]), seed=100000 + k,
+                                   thermalization=thermalization))
+    r = run_batch(sse, flats, wds, thermalization, steps, "heterogeneous")
+    r["workload"] = ("heterogeneous XXZ batch: 2048 walkers, L in {8,12,16,24,32}, "
+                     "beta log-uniform [0.5,16], Jz ~ U[0.5,1.5], h=0.1")
+    return r
+
+
+def extra_ladder(sse, walkers=1024, thermalization=500, steps=20, seed=4):
+    """1024 walkers of a 64-rung ladder in the dimer basis (local dim 4),
+    J_rung=1, J_leg ~ U[0.2,1.0], beta=10."""
+    from concurrent.futures import ProcessPoolExecutor
+    rng = np.random.default_rng(seed)
+    jls = rng.uniform(0.2, 1.0, size=walkers)
+    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        flats = list(ex.map(_ladder_flat, [float(j) for j in jls], chunksize=16))
+    wds = [sse.walker_desc(model=k, T=0.1, seed=200000 + k, thermalization=thermalization)
+           for k in range(walkers)]
+    r = run_batch(sse, flats, wds, thermalization, steps, "ladder")
+    r["workload"] = "two-leg ladder, 64 rungs, dimer basis, 1024 walkers, J_leg ~ U[0.2,1], beta=10"
+    return r
+
+
+def extra_serving(sse, seconds=30.0, sweep_scale=1e-3, slots=1024):
+    """Open-loop Poisson stream (200 jobs/s, seed 7) of Heisenberg/XXZ jobs on
+    L in 8..48, beta log-uniform [1,32]; requested sweeps log-uniform
+    [1e3,1e5] scaled by sweep_scale to bound the run."""
+    from importlib import import_module
+    sv = import_module("sse_amd.serving")
+    jobs = sv.poisson_jobs(int(200 * seconds), rate=200.0, seed=7, sweep_scale=sweep_scale)
+    loop = sv.ServingLoop(n_slots=slots, max_L=48, sweeps_per_step=5)
+    loop.batch.set_option(sse.capi.SSE_OPT_TIMING, 0)
+    done, stats = loop.run(jobs, time_limit=seconds * 3)
+    loop.close()
+    stats["workload"] = (f"serving: Poisson 200 jobs/s seed 7, L 8..48, beta 1..32, sweeps "
+                         f"log-uniform [1e3,1e5] x {sweep_scale:g}, {slots} slots")
+    stats["jobs_submitted"] = len(jobs)
+    return stats
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +244,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--extra", action="store_true",
+                    help="also run the heterogeneous, ladder and serving workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); walkers are independent "
@@ -230,6 +344,13 @@ def main():
         "cpu_baseline": cpu,
     }
     out.update(baselines)
+    if args.extra:
+        extras = {}
+        for name, fn in (("heterogeneous", extra_heterogeneous), ("ladder", extra_ladder),
+                         ("serving", extra_serving)):
+            log(f"extra workload: {name}")
+            extras[name] = fn(sse)
+        out["extra_workloads"] = extras
     print(json.dumps(out), flush=True)
 
 

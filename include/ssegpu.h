@@ -221,6 +221,14 @@ int sse_slot_poll(sse_batch *b, int32_t *done_slots, int32_t capacity, int32_t *
 /* Bytes of the slab pool in use (opstring + records of every walker).       */
 int sse_pool_usage(sse_batch *b, int64_t *used_bytes, int64_t *total_bytes);
 
+/* ---- options -------------------------------------------------------------
+ * SSE_OPT_GRAPH  (default 1): sse_run/sse_sweep replay a captured HIP graph of
+ *                up to 32 sweeps (+ measures) instead of launching each kernel.
+ * SSE_OPT_TIMING (default 1): calls of <= 256 sweeps record HIP events around
+ *                each kernel (read with sse_phase_times) and launch eagerly.  */
+enum sse_option { SSE_OPT_GRAPH = 1, SSE_OPT_TIMING = 2 };
+int sse_set_option(sse_batch *b, int32_t option, int64_t value);
+
 /* Encoding helpers (src/opercode.jl:18-71). */
 uint64_t sse_opcode_to_reference(uint32_t op);
 uint32_t sse_opcode_from_reference(uint64_t op);

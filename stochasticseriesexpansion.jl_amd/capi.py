@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(PKG_DIR, "libssegpu.so")
 
 SSE_MODE_WAVE = 0
 SSE_MODE_LANE = 1
+SSE_OPT_GRAPH = 1
+SSE_OPT_TIMING = 2
 
 STATUS = {0: "SSE_OK", 1: "SSE_ERR_INVALID", 2: "SSE_ERR_HIP", 3: "SSE_ERR_OOM",
           4: "SSE_ERR_SCATTER", 5: "SSE_ERR_NO_DEVICE", 6: "SSE_ERR_STATE"}
@@ -191,6 +193,7 @@ def load_library(path=None):
     lib.sse_add_model.argtypes = [vp, C.POINTER(ModelDesc), C.POINTER(C.c_int32)]
     lib.sse_slot_start.argtypes = [vp, C.c_int32, C.POINTER(WalkerDesc), C.c_int64]
     lib.sse_slot_stop.argtypes = [vp, C.c_int32]
+    lib.sse_set_option.argtypes = [vp, C.c_int32, C.c_int64]
     lib.sse_slot_poll.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]
     lib.sse_pool_usage.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.sse_last_error.restype = C.c_char_p
@@ -206,7 +209,7 @@ EXPORTED_SYMBOLS = (
     "sse_get_walker", "sse_set_walker", "sse_sweep_phases", "sse_get_vertex_list",
     "sse_phase_times", "sse_work_counters", "sse_opcode_to_reference",
     "sse_opcode_from_reference", "sse_last_error", "sse_abi_version", "sse_add_model",
-    "sse_slot_start", "sse_slot_stop", "sse_slot_poll", "sse_pool_usage")
+    "sse_slot_start", "sse_slot_stop", "sse_slot_poll", "sse_pool_usage", "sse_set_option")
 
 
 class SSEError(RuntimeError):
@@ -346,6 +349,9 @@ class Batch:
     def slot_start(self, slot, walker, n_sweeps):
         check(self.lib, self.lib.sse_slot_start(self.handle, slot, C.byref(walker), int(n_sweeps)))
         self.walkers[slot] = walker
+
+    def set_option(self, option, value):
+        check(self.lib, self.lib.sse_set_option(self.handle, option, int(value)))
 
     def slot_stop(self, slot):
         check(self.lib, self.lib.sse_slot_stop(self.handle, slot))

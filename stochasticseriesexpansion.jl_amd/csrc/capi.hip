@@ -67,6 +67,12 @@ struct sse_batch {
     bool inited = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> evpool;  // 4 per sweep of the last timed call
+    // captured sweep(+measure) sequence, replayed by sse_run/sse_sweep
+    bool use_graph = true, use_timing = true;
+    hipGraphExec_t gexec = nullptr;
+    int g_sweeps = 0, g_measure = -1;
+    size_t g_lds = 0;
+    const DevModel *g_models = nullptr;
     double phase_ms[3] = {0, 0, 0};
     int64_t phase_launches = 0;
     WaveConfig wave;
@@ -602,6 +608,8 @@ int sse_batch_destroy(sse_batch *b)
             (void)hipEventDestroy(b->ev[i]);
     for (hipEvent_t e : b->evpool)
         (void)hipEventDestroy(e);
+    if (b->gexec)
+        (void)hipGraphExecDestroy(b->gexec);
     if (b->stream)
         (void)hipStreamDestroy(b->stream);
     delete b;
@@ -662,6 +670,68 @@ static void collect_phase_times(sse_batch *b, int n_sweeps, bool with_measure)
     b->phase_launches = n_sweeps;
 }
 
+static int launch_measure(sse_batch *b, int mode);
+
+// Capture `sweeps` x (sweep [+ fused measure]) into a HIP graph (no memset
+// nodes), recaptured whenever the kernel parameters it bakes in change.
+static int graph_run(sse_batch *b, int sweeps, int measure)
+{
+    const size_t lds = b->wave.lds_k2;
+    if (!b->gexec || b->g_sweeps != sweeps || b->g_measure != measure || b->g_lds != lds ||
+        b->g_models != b->d_models) {
+        if (b->gexec) {
+            (void)hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = SSE_OK;
+        for (int k = 0; k < sweeps && !rc; ++k) {
+            rc = launch_sweep(b, 7);
+            if (!rc && measure)
+                rc = launch_measure(b, 1);
+        }
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc)
+            return rc;
+        if (e != hipSuccess)
+            return fail(SSE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            b->gexec = nullptr;
+            return fail(SSE_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        b->g_sweeps = sweeps;
+        b->g_measure = measure;
+        b->g_lds = lds;
+        b->g_models = b->d_models;
+    }
+    HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
+    return SSE_OK;
+}
+
+// n_sweeps through replays of a captured chunk of up to 32 sweeps
+static int graph_run_many(sse_batch *b, int n_sweeps, int measure)
+{
+    const int chunk = n_sweeps < 32 ? n_sweeps : 32;
+    int done = 0;
+    while (done + chunk <= n_sweeps) {
+        int rc = graph_run(b, chunk, measure);
+        if (rc)
+            return rc;
+        done += chunk;
+    }
+    for (; done < n_sweeps; ++done) {
+        int rc = launch_sweep(b, 7);
+        if (!rc && measure)
+            rc = launch_measure(b, 1);
+        if (rc)
+            return rc;
+    }
+    return SSE_OK;
+}
+
 // Carlo.init! of every pending walker (all walkers after creation, or the
 // slots restarted by sse_slot_start)
 static int launch_init_pending(sse_batch *b)
@@ -705,19 +775,33 @@ int sse_sweep(sse_batch *b, int32_t n_sweeps)
         return fail(SSE_ERR_STATE, "sse_sweep before sse_init");
     if (n_sweeps < 0)
         return fail(SSE_ERR_INVALID, "negative sweep count");
-    const bool timed = n_sweeps <= 256;
+    if (b->any_pending) {
+        int rc = launch_init_pending(b);
+        if (rc)
+            return rc;
+    }
+    const bool timed = b->use_timing && n_sweeps <= 256;
     if (timed) {
         int rc = ensure_events(b, 4 * (size_t)n_sweeps);
         if (rc)
             return rc;
-    }
-    for (int k = 0; k < n_sweeps; ++k) {
-        hipEvent_t *e = timed ? &b->evpool[4 * (size_t)k] : nullptr;
-        if (e)
+        for (int k = 0; k < n_sweeps; ++k) {
+            hipEvent_t *e = &b->evpool[4 * (size_t)k];
             HIPCHECK(hipEventRecord(e[0], b->stream));
-        int rc = launch_sweep(b, 7, e);
+            int rc2 = launch_sweep(b, 7, e);
+            if (rc2)
+                return rc2;
+        }
+    } else if (b->use_graph) {
+        int rc = graph_run_many(b, n_sweeps, 0);
         if (rc)
             return rc;
+    } else {
+        for (int k = 0; k < n_sweeps; ++k) {
+            int rc = launch_sweep(b, 7);
+            if (rc)
+                return rc;
+        }
     }
     if (b->any_budget) {
         hipLaunchKernelGGL(retire_kernel, dim3((b->n_walkers + 63) / 64), dim3(64), 0, b->stream,
@@ -796,23 +880,38 @@ int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure)
         if (rc)
             return rc;
     }
-    const bool timed = n_sweeps <= 256;
+    const bool timed = b->use_timing && n_sweeps <= 256;
     if (timed) {
         int rc = ensure_events(b, 4 * (size_t)n_sweeps);
         if (rc)
             return rc;
-    }
-    for (int k = 0; k < n_sweeps; ++k) {
-        hipEvent_t *e = timed ? &b->evpool[4 * (size_t)k] : nullptr;
-        if (e)
+        for (int k = 0; k < n_sweeps; ++k) {
+            hipEvent_t *e = &b->evpool[4 * (size_t)k];
             HIPCHECK(hipEventRecord(e[0], b->stream));
-        int rc = launch_sweep(b, 7, e);
-        if (!rc && measure)
-            rc = launch_measure(b, 1);
+            int rc2 = launch_sweep(b, 7, e);
+            if (!rc2 && measure)
+                rc2 = launch_measure(b, 1);
+            if (rc2)
+                return rc2;
+            HIPCHECK(hipEventRecord(e[3], b->stream));
+        }
+    } else if (b->use_graph) {
+        int rc = graph_run_many(b, n_sweeps, measure ? 1 : 0);
         if (rc)
             return rc;
-        if (e)
-            HIPCHECK(hipEventRecord(e[3], b->stream));
+    } else {
+        for (int k = 0; k < n_sweeps; ++k) {
+            int rc = launch_sweep(b, 7);
+            if (!rc && measure)
+                rc = launch_measure(b, 1);
+            if (rc)
+                return rc;
+        }
+    }
+    if (!measure && b->any_budget) {
+        hipLaunchKernelGGL(retire_kernel, dim3((b->n_walkers + 63) / 64), dim3(64), 0, b->stream,
+                           b->d_walkers, b->n_walkers, b->d_done);
+        HIPCHECK(hipGetLastError());
     }
     int rc = check_walker_errors(b);
     if (timed)
@@ -1194,6 +1293,22 @@ int sse_pool_usage(sse_batch *b, int64_t *used_bytes, int64_t *total_bytes)
     if (total_bytes)
         *total_bytes = (int64_t)b->pool_size;
     return SSE_OK;
+}
+
+int sse_set_option(sse_batch *b, int32_t option, int64_t value)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    switch (option) {
+    case SSE_OPT_GRAPH:
+        b->use_graph = value != 0;
+        return SSE_OK;
+    case SSE_OPT_TIMING:
+        b->use_timing = value != 0;
+        return SSE_OK;
+    default:
+        return fail(SSE_ERR_INVALID, "unknown option");
+    }
 }
 
 int sse_phase_times(const sse_batch *b, double *ms3)

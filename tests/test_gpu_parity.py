@@ -262,3 +262,41 @@ def test_parity_heterogeneous_batch(sse_pkg, oracle_mod, mode):
     for k, o in enumerate(oracles):
         assert np.allclose(gm[k], o.measure(), rtol=1e-12, atol=1e-13)
     batch.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_graph_replay_equals_eager(sse_pkg, oracle_mod, mode):
+    """sse_run replaying a captured HIP graph (several replays with eager work
+    and host synchronization in between) gives the eager result bit for bit."""
+    model = square_heisenberg(sse_pkg, 6)
+    flat = model.flat()
+    wd = [sse_pkg.walker_desc(T=0.3, seed=120 + s, thermalization=10) for s in range(6)]
+    _say(f"graph replay: mode={mode}")
+    a = sse_pkg.Batch([flat], wd, mode=mode)
+    b = sse_pkg.Batch([flat], wd, mode=mode)
+    a.set_option(sse_pkg.capi.SSE_OPT_TIMING, 0)      # graph path
+    b.set_option(sse_pkg.capi.SSE_OPT_GRAPH, 0)       # eager path
+    b.set_option(sse_pkg.capi.SSE_OPT_TIMING, 0)
+    a.init()
+    b.init()
+    for rep in range(4):
+        a.run(70, measure=True)    # 2 replays of a 32-sweep graph + 6 eager
+        b.run(70, measure=True)
+        a.measure()                # eager work between replays
+        for k in range(6):
+            sa, oa, pa = a.get_walker(k)
+            sb, ob, pb = b.get_walker(k)
+            assert np.array_equal(oa, ob) and np.array_equal(pa, pb), f"replay {rep} walker {k}"
+            for key in KEYS:
+                assert sa[key] == sb[key]
+    acc_a, acc_b = a.accumulators(), b.accumulators()
+    for (s1, q1, c1), (s2, q2, c2) in zip(acc_a, acc_b):
+        assert c1 == c2 and np.array_equal(s1, s2) and np.array_equal(q1, q2)
+    # and against the oracle for one walker
+    o = oracle_mod.OracleWalker(sse_pkg.capi, flat, wd[0])
+    o.init()
+    o.sweep(280)
+    sa, oa, pa = a.get_walker(0)
+    assert np.array_equal(oa, o.ops()) and sa["rng_counter"] == o.scalars()["rng_counter"]
+    a.close()
+    b.close()
