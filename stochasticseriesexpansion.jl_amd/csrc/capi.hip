@@ -255,7 +255,7 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
     std::vector<DirectEntry> dt((size_t)m.n_types * dnv * dmw * 4);
     for (auto &x : dt) {
         for (int j = 0; j < 4; ++j) {
-            x.K[j] = 0;
+            x.K[j] = ~0ull;
             x.out[j] = 0;
         }
         x.count = 0;

@@ -67,7 +67,9 @@ struct DevModel {
 
 // Exit j is taken by the first j with r < cum[j], r = (word >> 11) * 2^-53.
 // With m = word >> 11 (an integer < 2^53) and K = ceil(cum * 2^53) this is
-// exactly m < K, so the worm kernel compares integers; unused slots K = 0.
+// exactly m < K, so the worm kernel compares integers.  The thresholds are
+// cumulative (non-decreasing) and unused slots hold UINT64_MAX, so the exit
+// index is the number of thresholds <= m; index == count is the fall-through.
 struct __align__(16) DirectEntry {
     uint64_t K[4];     // integer thresholds ceil(cum[j] * 2^53)
     uint32_t out[4];   // target vcode | leg_out << 10 | worm_out << 12

@@ -642,11 +642,12 @@ struct LinkType<RecGlobal> {
 typedef const __attribute__((address_space(4))) DirectEntry ConstEntry;
 
 // The worm update of one walker (sse.jl:193-303), executed uniformly by the
-// whole wave.  The chase is scalar work: every lane holds the same state, the
-// records come from LDS (made scalar with readfirstlane), the transition
-// entry arrives through a scalar load (constant address space, uniform index)
-// and the exit is chosen by integer compares of the Philox word against the
-// entry's thresholds.  Returns an error code (0 = ok).
+// whole wave.  The chase is scalar work: every value is made wave-uniform with
+// readfirstlane right after it leaves LDS, the next Philox word is fetched at
+// the top of the step together with the record (vcode + links), the
+// transition entry arrives through a scalar load (constant address space,
+// uniform index), and the exit is chosen by integer compares of the word
+// against the entry's thresholds.  Returns an error code (0 = ok).
 template <class R, class RS>
 __device__ int worm_body(DevWalker &w, const DevModel &m, RS &rs, const R &rec,
                          const uint64_t *MASK, const uint32_t *BASE, ConstEntry *DT,
@@ -656,7 +657,7 @@ __device__ int worm_body(DevWalker &w, const DevModel &m, RS &rs, const R &rec,
     const int64_t L = w.L;
     const int n = (int)w.n;
     const int64_t nworms = (int64_t)ceil(w.num_worms);
-    const uint32_t dnv = (uint32_t)m.dt_nv, dmw = (uint32_t)m.dt_mw;
+    const uint32_t dnv = uni((uint32_t)m.dt_nv), dmw = uni((uint32_t)m.dt_mw);
     total = 1.0;
     int64_t sc = 0;
     for (int64_t k = 0; k < nworms; ++k) {
@@ -665,8 +666,8 @@ __device__ int worm_body(DevWalker &w, const DevModel &m, RS &rs, const R &rec,
         int64_t p0;
         int l0, r0;
         for (;;) {  // sse.jl:241-247: an operator's four legs are all linked
-            p0 = (int64_t)rs.range((uint64_t)L);
-            l0 = (int)rs.range(4);
+            p0 = (int64_t)uni64(rs.range((uint64_t)L));
+            l0 = (int)uni((uint32_t)rs.range(4));
             const uint64_t mk = uni64(MASK[p0 >> 6]);
             const int bit = (int)(p0 & 63);
             if ((mk >> bit) & 1ull) {
@@ -675,44 +676,58 @@ __device__ int worm_body(DevWalker &w, const DevModel &m, RS &rs, const R &rec,
             }
         }
         const int2 b0 = m.bond_rec[op_bond(w.ops[p0])];
-        const int site0 = (l0 & 1) ? b0.y : (b0.x & 0xFFFFF);
+        const int site0 = (int)uni((uint32_t)((l0 & 1) ? b0.y : (b0.x & 0xFFFFF)));
         const int dim0 = (int)uni(m.site_dims[site0]);
-        const int wf0 = 1 + (int)rs.range((uint64_t)(dim0 - 1));
+        const int wf0 = 1 + (int)uni((uint32_t)rs.range((uint64_t)(dim0 - 1)));
         const int winv0 = dim0 - wf0;
         int leg_in = l0, wf = wf0, r = r0;
         int64_t len = 1;
         for (;;) {  // sse.jl:274-300
-            uint32_t vt;
+            // loop-carried chase state: keep it in scalar registers
+            r = (int)uni((uint32_t)r);
+            wf = (int)uni((uint32_t)wf);
+            leg_in = (int)uni((uint32_t)leg_in);
+            rs.off = uni(rs.off);
+            if (rs.off >= 128u)
+                rs.ensure(1);
+            const uint64_t word_v = rs.buf[rs.off];
+            uint32_t vt_v;
             typename LinkType<R>::T lk;
-            rec.load(r, vt, lk);
-            vt = uni(vt);
+            rec.load(r, vt_v, lk);
+            const uint32_t vt = uni(vt_v);
             const uint32_t vidx = (vt & 1023u) >> 1, t = vt >> 10;
             if (vidx >= dnv || (uint32_t)(wf - 1) >= dmw)
                 return SSE_ERR_STATE;
-            const uint32_t ei = uni(((t * dnv + vidx) * dmw + (uint32_t)(wf - 1)) * 4u + (uint32_t)leg_in);
+            const uint32_t ei = ((t * dnv + vidx) * dmw + (uint32_t)(wf - 1)) * 4u + (uint32_t)leg_in;
             ConstEntry &de = DT[ei];
-            // whole entry in scalar registers before any branch
             const uint64_t K0 = de.K[0], K1 = de.K[1], K2 = de.K[2], K3 = de.K[3];
             const uint32_t o0 = de.out[0], o1 = de.out[1], o2 = de.out[2], o3 = de.out[3];
             const uint32_t cnt = de.count, trw = de.trans;
-            const uint64_t word = uni64(rs.next());
+            const uint64_t word = uni64(word_v);
+            rs.off = uni(rs.off + 1u);
             const uint64_t mm = word >> 11;  // rand() = mm * 2^-53
             sc += 1;
-            uint32_t eo = mm < K0 ? o0 : mm < K1 ? o1 : mm < K2 ? o2 : mm < K3 ? o3 : 0xFFFFFFFFu;
-            if (cnt <= 4u) {
-                if (eo == 0xFFFFFFFFu)
-                    return SSE_ERR_SCATTER;  // fall-through (vertex_data.jl:124) or invalid
-            } else {
+            // first j with mm < K[j] = number of thresholds <= mm (K non-decreasing)
+            const uint32_t j = (uint32_t)(mm >= K0) + (uint32_t)(mm >= K1) + (uint32_t)(mm >= K2) +
+                               (uint32_t)(mm >= K3);
+            // mask select: keeps all four exit words live so they arrive with the
+            // thresholds in one scalar load instead of a dependent one after the compare
+            uint32_t eo = (o0 & (0u - (uint32_t)(j == 0u))) | (o1 & (0u - (uint32_t)(j == 1u))) |
+                          (o2 & (0u - (uint32_t)(j == 2u))) | (o3 & (0u - (uint32_t)(j >= 3u)));
+            if (cnt <= 4u && j >= cnt)
+                eo = 0xFFFFFFFFu;  // fall-through (or invalid transition, count 0)
+            if (cnt > 4u) {
                 const WormTab tab = worm_tab((const char *)m.wtab, m);
                 const double x = word_to_float(word);
-                const uint32_t tr = trw;
-                uint32_t e = tr & 0xFFFFFu;
-                const uint32_t eend = e + (tr >> 20);
+                uint32_t e = trw & 0xFFFFFu;
+                const uint32_t eend = e + (trw >> 20);
                 while (e < eend && !(x < tab.cum[e]))
                     ++e;
                 if (e >= eend)
                     return SSE_ERR_SCATTER;
                 eo = uni(tab.eout[e]);
+            } else if (eo == 0xFFFFFFFFu) {
+                return SSE_ERR_SCATTER;  // fall-through (vertex_data.jl:124) or invalid
             }
             const int leg_out = (int)((eo >> 10) & 3u);
             const int wf_out = (int)(eo >> 12);
