@@ -225,8 +225,13 @@ int sse_pool_usage(sse_batch *b, int64_t *used_bytes, int64_t *total_bytes);
  * SSE_OPT_GRAPH  (default 1): sse_run/sse_sweep replay a captured HIP graph of
  *                up to 32 sweeps (+ measures) instead of launching each kernel.
  * SSE_OPT_TIMING (default 1): calls of <= 256 sweeps record HIP events around
- *                each kernel (read with sse_phase_times) and launch eagerly.  */
-enum sse_option { SSE_OPT_GRAPH = 1, SSE_OPT_TIMING = 2 };
+ *                each kernel (read with sse_phase_times) and launch eagerly.
+ * SSE_OPT_WORM_RECORDS (default 0): record layout of the worm kernel.  0 picks
+ *                the smallest that fits (8-byte LDS records for n <= 4096 and
+ *                <= 256 chase codes, 10-byte LDS records for n <= 16384, else
+ *                global memory); 1 skips the 8-byte layout; 2 forces global
+ *                memory.  Results are identical; this is a tuning/testing knob. */
+enum sse_option { SSE_OPT_GRAPH = 1, SSE_OPT_TIMING = 2, SSE_OPT_WORM_RECORDS = 3 };
 int sse_set_option(sse_batch *b, int32_t option, int64_t value);
 
 /* Encoding helpers (src/opercode.jl:18-71). */

@@ -20,6 +20,7 @@ SSE_MODE_WAVE = 0
 SSE_MODE_LANE = 1
 SSE_OPT_GRAPH = 1
 SSE_OPT_TIMING = 2
+SSE_OPT_WORM_RECORDS = 3
 
 STATUS = {0: "SSE_OK", 1: "SSE_ERR_INVALID", 2: "SSE_ERR_HIP", 3: "SSE_ERR_OOM",
           4: "SSE_ERR_SCATTER", 5: "SSE_ERR_NO_DEVICE", 6: "SSE_ERR_STATE"}

@@ -72,6 +72,7 @@ struct sse_batch {
     hipGraphExec_t gexec = nullptr;
     int g_sweeps = 0, g_measure = -1;
     size_t g_lds = 0;
+    int g_rec_mode = 0;
     const DevModel *g_models = nullptr;
     double phase_ms[3] = {0, 0, 0};
     int64_t phase_launches = 0;
@@ -246,17 +247,23 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
     if (!cum.empty())
         memcpy(wt.data() + w_cum, cum.data(), cum.size() * 8);
     size_t o_wtab = put(wt.data(), wt.size());
-    // direct table
+    // chase table: one entry per (ecode, worm-1, leg_in), ecode = type << lgp | vcode
+    // with P = 2^lgp >= 2 * max vertex count, so both diagonal-flag variants of a
+    // vertex index address the same transitions (the flag rides in the vcode)
     int dnv = 1, dmw = 1;
     for (int t = 0; t < m.n_types; ++t) {
         dnv = std::max(dnv, (int)m.types[t].n_vertices);
         dmw = std::max(dmw, (int)m.types[t].max_worm_count);
     }
-    std::vector<DirectEntry> dt((size_t)m.n_types * dnv * dmw * 4);
-    for (auto &x : dt) {
+    int lgp = 1;
+    while ((1 << lgp) < 2 * dnv)
+        ++lgp;
+    const size_t P = (size_t)1 << lgp;
+    std::vector<DirectEntry> dt((size_t)m.n_types * P * dmw * 4);
+    for (auto &x : dt) {  // invalid transition: every word falls through
         for (int j = 0; j < 4; ++j) {
-            x.K[j] = ~0ull;
-            x.out[j] = 0;
+            x.K[j] = 0;
+            x.out[j] = 0xFFFFFFFFu;
         }
         x.count = 0;
         x.trans = 0xFFFFFFFFu;
@@ -270,22 +277,34 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
                 for (int wv = 0; wv < vd.max_worm_count; ++wv)
                     for (int l = 0; l < 4; ++l) {
                         int src = (v * vd.max_worm_count + wv) * 4 + l;
-                        DirectEntry &x = dt[(((size_t)t * dnv + v) * dmw + wv) * 4 + l];
                         int off = vd.transition_offset[src], cnt = vd.transition_count[src];
                         if (off < 0 || cnt <= 0)
                             continue;
+                        DirectEntry x;
                         x.count = (uint32_t)cnt;
                         x.trans = (uint32_t)(ebase + off) | ((uint32_t)cnt << 20);
-                        for (int j = 0; j < cnt && j < 4; ++j) {
-                            double c = vd.exit_cumprob[off + j];
-                            // r < c  <=>  (word >> 11) < ceil(c * 2^53)
-                            x.K[j] = c <= 0.0 ? 0ull
-                                     : c >= 1.0 ? (1ull << 53)
-                                                : (uint64_t)std::ceil(std::ldexp(c, 53));
-                            x.out[j] = (uint32_t)vd.exit_target[off + j] |
-                                       ((uint32_t)vd.exit_leg[off + j] << 10) |
-                                       ((uint32_t)vd.exit_worm[off + j] << 12);
+                        x.pad[0] = x.pad[1] = 0;
+                        if (cnt > 4) {  // exits looked up in wtab: fall through to it
+                            for (int j = 0; j < 4; ++j) {
+                                x.K[j] = 0;
+                                x.out[j] = 0xFFFFFFFFu;
+                            }
+                        } else {
+                            for (int j = 0; j < 4; ++j) {
+                                const int jj = std::min(j, cnt - 1);
+                                const double c = vd.exit_cumprob[off + jj];
+                                // r < c  <=>  (word >> 11) < ceil(c * 2^53)
+                                x.K[j] = c <= 0.0 ? 0ull
+                                         : c >= 1.0 ? (1ull << 53)
+                                                    : (uint64_t)std::ceil(std::ldexp(c, 53));
+                                x.out[j] = j < cnt ? chase_out((uint32_t)t << lgp |
+                                                                   (uint32_t)vd.exit_target[off + j],
+                                                               vd.exit_leg[off + j], vd.exit_worm[off + j])
+                                                   : 0xFFFFFFFFu;
+                            }
                         }
+                        for (int dg = 0; dg < 2; ++dg)
+                            dt[((((size_t)t << lgp) + (2 * v + dg)) * dmw + wv) * 4 + l] = x;
                     }
             ebase += vd.n_exits;
         }
@@ -343,6 +362,7 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
     dm.dtab = (const DirectEntry *)(d + o_dtab);
     dm.dt_nv = dnv;
     dm.dt_mw = dmw;
+    dm.dt_lgp = lgp;
     dm.dtab_bytes = (int32_t)(sizeof(DirectEntry) * dt.size());
     return SSE_OK;
 }
@@ -678,6 +698,7 @@ static int graph_run(sse_batch *b, int sweeps, int measure)
 {
     const size_t lds = b->wave.lds_k2;
     if (!b->gexec || b->g_sweeps != sweeps || b->g_measure != measure || b->g_lds != lds ||
+        b->g_rec_mode != b->wave.rec_mode ||
         b->g_models != b->d_models) {
         if (b->gexec) {
             (void)hipGraphExecDestroy(b->gexec);
@@ -705,6 +726,7 @@ static int graph_run(sse_batch *b, int sweeps, int measure)
         b->g_sweeps = sweeps;
         b->g_measure = measure;
         b->g_lds = lds;
+        b->g_rec_mode = b->wave.rec_mode;
         b->g_models = b->d_models;
     }
     HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
@@ -1154,6 +1176,12 @@ int sse_add_model(sse_batch *b, const sse_model_desc *model, int32_t *index)
     if (rc)
         return rc;
     b->h_models.push_back(dm);
+    if (((int64_t)dm.n_types << dm.dt_lgp) > 256 && b->wave.small_codes) {
+        b->wave.small_codes = false;  // 10-byte worm records from now on: size LDS for them
+        DevPool p;
+        HIPCHECK(hipMemcpy(&p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));
+        wave_resize_lds(b->wave, p.max_n, p.max_L);
+    }
     DevModel *nd = nullptr;
     rc = dev_alloc(b, &nd, b->h_models.size());
     if (rc)
@@ -1305,6 +1333,17 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         return SSE_OK;
     case SSE_OPT_TIMING:
         b->use_timing = value != 0;
+        return SSE_OK;
+    case SSE_OPT_WORM_RECORDS:
+        if (value < 0 || value > 2)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_WORM_RECORDS takes 0, 1 or 2");
+        b->wave.rec_mode = (int)value;
+        if (value == 1) {  // Rec10 needs more LDS than the Rec8 sizing gave
+            DevPool p;
+            HIPCHECK(hipStreamSynchronize(b->stream));
+            HIPCHECK(hipMemcpy(&p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));
+            wave_resize_lds(b->wave, p.max_n, p.max_L);
+        }
         return SSE_OK;
     default:
         return fail(SSE_ERR_INVALID, "unknown option");

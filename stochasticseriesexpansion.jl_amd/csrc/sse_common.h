@@ -57,26 +57,43 @@ struct DevModel {
     //   f64 cum[n_exits]     at byte offset wtab_cum_off
     const uint32_t *wtab;
     int32_t wtab_bytes, wtab_cum_off, wtab_eout_off, wtab_trans_off;
-    // direct worm table: one 64-byte entry per (type, vertex, worm, leg_in)
-    // at index ((type*dt_nv + vidx)*dt_mw + worm-1)*4 + leg_in; up to 4 exits
-    // inline (DirectEntry), count 0 = invalid transition, count > 4 = look
-    // the exits up in wtab
+    // chase table: one 64-byte entry (DirectEntry) per (ecode, worm-1, leg_in)
+    // at index (ecode*dt_mw + worm-1)*4 + leg_in, ecode = type << dt_lgp | vcode
+    // (2^dt_lgp >= 2*dt_nv); up to 4 exits inline, more are looked up in wtab
     const struct DirectEntry *dtab;
-    int32_t dt_nv, dt_mw, dtab_bytes, pad_dt;
+    int32_t dt_nv, dt_mw, dtab_bytes, dt_lgp;
 };
 
 // Exit j is taken by the first j with r < cum[j], r = (word >> 11) * 2^-53.
 // With m = word >> 11 (an integer < 2^53) and K = ceil(cum * 2^53) this is
-// exactly m < K, so the worm kernel compares integers.  The thresholds are
-// cumulative (non-decreasing) and unused slots hold UINT64_MAX, so the exit
-// index is the number of thresholds <= m; index == count is the fall-through.
+// exactly m < K, so the worm kernel compares integers.  Layout for the chase:
+// out[0] is taken for m < K[0], out[j] for K[j-1] <= m < K[j]; m >= K[3] falls
+// through.  With count < 4 exits the last threshold is repeated and the unused
+// outs are 0xFFFFFFFF, so m >= K[count-1] reaches the fall-through test.  An
+// invalid transition (count 0) or one with more than 4 exits has all K = 0:
+// every word falls through and the count decides.
 struct __align__(16) DirectEntry {
     uint64_t K[4];     // integer thresholds ceil(cum[j] * 2^53)
-    uint32_t out[4];   // target vcode | leg_out << 10 | worm_out << 12
+    uint32_t out[4];   // chase_out(target ecode, exit leg, exit worm)
     uint32_t count;    // number of exits
     uint32_t trans;    // wtab trans word (exit index | count << 20)
     uint32_t pad[2];
 };
+
+// exit word of the chase: worm-1 in bits 0..13, exit leg in 14..15, target
+// ecode (type << dt_lgp | vcode) in 16..31
+__host__ __device__ inline uint32_t chase_out(uint32_t ecode, uint32_t leg, uint32_t worm)
+{
+    return (worm - 1u) | (leg << 14) | (ecode << 16);
+}
+__host__ __device__ inline uint32_t ecode_of_vt(uint32_t vt, uint32_t lgp)
+{
+    return ((vt >> 10) << lgp) | (vt & 1023u);
+}
+__host__ __device__ inline uint32_t vt_of_ecode(uint32_t e, uint32_t lgp)
+{
+    return (e & ((1u << lgp) - 1u)) | ((e >> lgp) << 10);
+}
 
 struct DevWalker {
     // parameters

@@ -34,6 +34,8 @@ struct WaveConfig {
     int max_wtab = 0;
     size_t lds_k1 = 0, lds_k2 = 0, lds_k3 = 0;
     size_t lds_limit = 65536;  // device limit on dynamic LDS per workgroup
+    bool small_codes = true;   // every model has <= 256 chase codes (Rec8 records)
+    int rec_mode = 0;          // worm-kernel record layout: 0 auto, 1 no Rec8, 2 global
 };
 
 __device__ __forceinline__ uint64_t below_mask(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
@@ -585,166 +587,230 @@ __device__ __forceinline__ WormTab worm_tab(const char *base, const DevModel &m)
     return t;
 }
 
-// Compact worm records of one walker: vt[r] = vcode | type << 10 and the four
-// links of rank r (leg ids 4r'+leg').  LDS form: u16 vt, four u16 links in
-// one u64; global form: u32 vt, four u32 links in one uint4.
-struct RecLds {
-    uint16_t *vt;
-    const uint64_t *lk;
-    __device__ __forceinline__ void load(int r, uint32_t &v, uint64_t &l) const
-    {
-        v = vt[r];
-        l = lk[r];
-    }
-    __device__ __forceinline__ static uint32_t link(uint64_t l, int leg)
-    {
-        return uni((uint32_t)(l >> (16 * leg)) & 0xFFFFu);
-    }
-    __device__ __forceinline__ void store(int r, uint32_t v, int lane) const
-    {
-        if (lane == 0)
-            vt[r] = (uint16_t)v;
-    }
-    __device__ __forceinline__ uint32_t get_vt(int r) const { return vt[r]; }
+// Worm records of one walker during K2, indexed by operator rank r: the chase
+// code ecode = type << dt_lgp | vcode and the four links (leg ids 4r'+leg').
+// K1 leaves them in global memory (u32 vt = vcode | type << 10, uint4 links);
+// K2 stages them in LDS when they fit:
+//   Rec8   one u64 per rank: links in 14-bit fields, ecode in bits 56..63
+//          (n <= 4096, at most 256 ecodes)
+//   Rec10  u64 links in 16-bit fields + a u16 ecode array (n <= 16384)
+//   RecG   global memory: ecode rewritten in place over vt, links read through
+//          the scalar cache (K2 never writes them)
+// load() returns wave-uniform words; put_ecode() is called by the whole wave.
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
+struct U4 {
+    uint32_t x, y, z, w;
 };
-
-struct RecGlobal {
-    uint32_t *vt;
-    const uint4 *lk;
-    struct L4 {
-        uint32_t x[4];
-    };
-    __device__ __forceinline__ void load(int r, uint32_t &v, uint4 &l) const
-    {
-        v = vt[r];
-        l = lk[r];
-    }
-    __device__ __forceinline__ static uint32_t link(const uint4 &l, int leg)
-    {
-        return uni(leg == 0 ? l.x : leg == 1 ? l.y : leg == 2 ? l.z : l.w);
-    }
-    // every lane stores, so each lane's own later load is ordered after it
-    __device__ __forceinline__ void store(int r, uint32_t v, int) const { vt[r] = v; }
-    __device__ __forceinline__ uint32_t get_vt(int r) const { return vt[r]; }
-};
-
-template <class R>
-struct LinkType;
-template <>
-struct LinkType<RecLds> {
-    typedef uint64_t T;
-};
-template <>
-struct LinkType<RecGlobal> {
-    typedef uint4 T;
-};
-
+typedef const __attribute__((address_space(4))) U4 const_u4;
+typedef const __attribute__((address_space(4))) char const_char;
 typedef const __attribute__((address_space(4))) DirectEntry ConstEntry;
 
-// The worm update of one walker (sse.jl:193-303), executed uniformly by the
-// whole wave.  The chase is scalar work: every value is made wave-uniform with
-// readfirstlane right after it leaves LDS, the next Philox word is fetched at
-// the top of the step together with the record (vcode + links), the
-// transition entry arrives through a scalar load (constant address space,
-// uniform index), and the exit is chosen by integer compares of the word
-// against the entry's thresholds.  Returns an error code (0 = ok).
-template <class R, class RS>
-__device__ int worm_body(DevWalker &w, const DevModel &m, RS &rs, const R &rec,
-                         const uint64_t *MASK, const uint32_t *BASE, ConstEntry *DT,
-                         double &total, int64_t &scat)
+struct Rec8 {
+    lds_u64 *rec;
+    static constexpr bool kChecked = false;
+    struct Word {
+        uint32_t lo, hi;
+    };
+    __device__ __forceinline__ Word load(uint32_t r) const
+    {
+        const uint64_t x = rec[r];
+        return Word{uni((uint32_t)x), uni((uint32_t)(x >> 32))};
+    }
+    __device__ __forceinline__ static uint32_t ecode(const Word &x) { return x.hi >> 24; }
+    __device__ __forceinline__ static uint32_t link(const Word &x, uint32_t leg)
+    {
+        const uint64_t v = (uint64_t)x.lo | ((uint64_t)x.hi << 32);
+        return (uint32_t)(v >> (14u * leg)) & 0x3FFFu;
+    }
+    __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const
+    {
+        if (threadIdx.x == 0)
+            ((lds_u8 *)rec)[8u * r + 7u] = (uint8_t)e;
+    }
+    __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return (uint32_t)(rec[r] >> 56); }
+};
+
+struct Rec10 {
+    lds_u64 *lk;
+    lds_u16 *ec;
+    static constexpr bool kChecked = false;
+    struct Word {
+        uint32_t lo, hi, e;
+    };
+    __device__ __forceinline__ Word load(uint32_t r) const
+    {
+        const uint64_t x = lk[r];
+        const uint32_t e = ec[r];
+        return Word{uni((uint32_t)x), uni((uint32_t)(x >> 32)), uni(e)};
+    }
+    __device__ __forceinline__ static uint32_t ecode(const Word &x) { return x.e; }
+    __device__ __forceinline__ static uint32_t link(const Word &x, uint32_t leg)
+    {
+        const uint64_t v = (uint64_t)x.lo | ((uint64_t)x.hi << 32);
+        return (uint32_t)(v >> (16u * leg)) & 0xFFFFu;
+    }
+    __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const
+    {
+        if (threadIdx.x == 0)
+            ec[r] = (uint16_t)e;
+    }
+    __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
+};
+
+struct RecG {
+    uint32_t *ec;
+    const_u4 *lk;
+    static constexpr bool kChecked = true;
+    struct Word {
+        U4 l;
+        uint32_t e;
+    };
+    __device__ __forceinline__ Word load(uint32_t r) const
+    {
+        const_u4 &q = lk[r];
+        return Word{U4{q.x, q.y, q.z, q.w}, uni(ec[r])};
+    }
+    __device__ __forceinline__ static uint32_t ecode(const Word &x) { return x.e; }
+    __device__ __forceinline__ static uint32_t link(const Word &x, uint32_t leg)
+    {
+        return leg == 0 ? x.l.x : leg == 1 ? x.l.y : leg == 2 ? x.l.z : x.l.w;
+    }
+    // every lane stores, so each lane's own later load is ordered after it
+    __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const { ec[r] = e; }
+    __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
+};
+
+// Exit selection of one scatter on the scalar unit: m = word >> 11 against the
+// entry's thresholds by 64-bit borrow chains (s_sub/s_subb leave the borrow,
+// m < K, in SCC).  Returns out[j] for the first j with m < K[j]; ft = 1 when
+// m >= K[3] (fall-through, or an entry whose exits live in wtab).
+__device__ __forceinline__ uint32_t chase_select(uint32_t mlo, uint32_t mhi, ConstEntry *E, uint32_t &ft)
 {
-    const int lane = threadIdx.x;
+    const uint64_t K0 = E->K[0], K1 = E->K[1], K2 = E->K[2], K3 = E->K[3];
+    const uint32_t o0 = E->out[0], o1 = E->out[1], o2 = E->out[2], o3 = E->out[3];
+    uint32_t eo, t;
+    asm("s_sub_u32 %1, %3, %5\n\t"
+        "s_subb_u32 %1, %4, %6\n\t"
+        "s_cselect_b32 %0, %13, %14\n\t"
+        "s_sub_u32 %1, %3, %7\n\t"
+        "s_subb_u32 %1, %4, %8\n\t"
+        "s_cselect_b32 %0, %0, %15\n\t"
+        "s_sub_u32 %1, %3, %9\n\t"
+        "s_subb_u32 %1, %4, %10\n\t"
+        "s_cselect_b32 %0, %0, %16\n\t"
+        "s_sub_u32 %1, %3, %11\n\t"
+        "s_subb_u32 %1, %4, %12\n\t"
+        "s_cselect_b32 %2, 0, 1"
+        : "=&s"(eo), "=&s"(t), "=s"(ft)
+        : "s"(uni(mlo)), "s"(uni(mhi)), "s"((uint32_t)K0), "s"((uint32_t)(K0 >> 32)), "s"((uint32_t)K1),
+          "s"((uint32_t)(K1 >> 32)), "s"((uint32_t)K2), "s"((uint32_t)(K2 >> 32)), "s"((uint32_t)K3),
+          "s"((uint32_t)(K3 >> 32)), "s"(o0), "s"(o1), "s"(o2), "s"(o3)
+        : "scc");
+    return eo;
+}
+
+// The worm update of one walker (sse.jl:193-303), executed uniformly by the
+// whole wave.  The chase is scalar work: the record and the next Philox word
+// leave LDS together and become wave-uniform at once, the transition entry
+// arrives in one scalar load (constant address space, uniform offset), the exit
+// is selected on the scalar unit, and the closing tests compare packed keys
+// (leg id << 14 | worm-1).  Returns an error code (0 = ok).
+template <class R>
+__device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, const R &rec,
+                          const uint64_t *MASK, const uint32_t *BASE, double &total, int64_t &scat)
+{
     const int64_t L = w.L;
-    const int n = (int)w.n;
+    const uint32_t n = (uint32_t)w.n;
     const int64_t nworms = (int64_t)ceil(w.num_worms);
-    const uint32_t dnv = uni((uint32_t)m.dt_nv), dmw = uni((uint32_t)m.dt_mw);
+    const uint32_t dmw = uni((uint32_t)m.dt_mw);
+    const uint32_t lgp = uni((uint32_t)m.dt_lgp);
+    const uint32_t estride = dmw << 8;  // bytes per ecode
+    const_char *CT = (const_char *)(const void *)uni64((uint64_t)(const void *)m.dtab);
+    lds_u64 *RB = (lds_u64 *)rs.buf;
     total = 1.0;
-    int64_t sc = 0;
+    uint32_t sc = 0;
     for (int64_t k = 0; k < nworms; ++k) {
         if (n == 0)
             continue;  // worm_traverse! returns 0 without draws (sse.jl:234-236)
         int64_t p0;
-        int l0, r0;
+        uint32_t l0, r0;
         for (;;) {  // sse.jl:241-247: an operator's four legs are all linked
             p0 = (int64_t)uni64(rs.range((uint64_t)L));
-            l0 = (int)uni((uint32_t)rs.range(4));
+            l0 = uni((uint32_t)rs.range(4));
             const uint64_t mk = uni64(MASK[p0 >> 6]);
             const int bit = (int)(p0 & 63);
             if ((mk >> bit) & 1ull) {
-                r0 = (int)uni(BASE[p0 >> 6] + (uint32_t)popc(mk & below_mask(bit)));
+                r0 = uni(BASE[p0 >> 6] + (uint32_t)popc(mk & below_mask(bit)));
                 break;
             }
         }
         const int2 b0 = m.bond_rec[op_bond(w.ops[p0])];
-        const int site0 = (int)uni((uint32_t)((l0 & 1) ? b0.y : (b0.x & 0xFFFFF)));
-        const int dim0 = (int)uni(m.site_dims[site0]);
-        const int wf0 = 1 + (int)uni((uint32_t)rs.range((uint64_t)(dim0 - 1)));
-        const int winv0 = dim0 - wf0;
-        int leg_in = l0, wf = wf0, r = r0;
-        int64_t len = 1;
+        const uint32_t site0 = uni((uint32_t)((l0 & 1) ? b0.y : (b0.x & 0xFFFFF)));
+        const uint32_t dim0 = uni(m.site_dims[site0]);
+        const uint32_t wf0 = 1u + uni((uint32_t)rs.range((uint64_t)(dim0 - 1)));
+        if (wf0 - 1u >= dmw)
+            return SSE_ERR_STATE;
+        const uint32_t lid0 = 4u * r0 + l0;
+        // closing keys (sse.jl:290-297): exit (r0, l0, dim0 - wf0) or entry (r0, l0, wf0)
+        const uint64_t keyA = ((uint64_t)lid0 << 14) | (dim0 - wf0 - 1u);
+        const uint64_t keyB = ((uint64_t)lid0 << 14) | (wf0 - 1u);
+        uint32_t lid = lid0, wfm = wf0 - 1u;
+        const uint32_t sc0 = sc;
+        uint32_t endB;
         for (;;) {  // sse.jl:274-300
-            // loop-carried chase state: keep it in scalar registers
-            r = (int)uni((uint32_t)r);
-            wf = (int)uni((uint32_t)wf);
-            leg_in = (int)uni((uint32_t)leg_in);
-            rs.off = uni(rs.off);
             if (rs.off >= 128u)
                 rs.ensure(1);
-            const uint64_t word_v = rs.buf[rs.off];
-            uint32_t vt_v;
-            typename LinkType<R>::T lk;
-            rec.load(r, vt_v, lk);
-            const uint32_t vt = uni(vt_v);
-            const uint32_t vidx = (vt & 1023u) >> 1, t = vt >> 10;
-            if (vidx >= dnv || (uint32_t)(wf - 1) >= dmw)
-                return SSE_ERR_STATE;
-            const uint32_t ei = ((t * dnv + vidx) * dmw + (uint32_t)(wf - 1)) * 4u + (uint32_t)leg_in;
-            ConstEntry &de = DT[ei];
-            const uint64_t K0 = de.K[0], K1 = de.K[1], K2 = de.K[2], K3 = de.K[3];
-            const uint32_t o0 = de.out[0], o1 = de.out[1], o2 = de.out[2], o3 = de.out[3];
-            const uint32_t cnt = de.count, trw = de.trans;
-            const uint64_t word = uni64(word_v);
-            rs.off = uni(rs.off + 1u);
-            const uint64_t mm = word >> 11;  // rand() = mm * 2^-53
+            const uint32_t off = uni(rs.off);
+            const uint64_t wv = RB[off];
+            const uint32_t r = lid >> 2;
+            const typename R::Word x = rec.load(r);
+            const uint32_t wlo = uni((uint32_t)wv), whi = uni((uint32_t)(wv >> 32));
+            rs.off = off + 1u;
+            const uint32_t e = R::ecode(x);
+            ConstEntry *E = (ConstEntry *)(CT + (e * estride + (wfm << 8) + ((lid & 3u) << 6)));
+            uint32_t ft;
+            const uint64_t mm = (((uint64_t)whi << 32) | wlo) >> 11;  // rand() = mm * 2^-53
+            uint32_t eo = chase_select((uint32_t)mm, (uint32_t)(mm >> 32), E, ft);
             sc += 1;
-            // first j with mm < K[j] = number of thresholds <= mm (K non-decreasing)
-            const uint32_t j = (uint32_t)(mm >= K0) + (uint32_t)(mm >= K1) + (uint32_t)(mm >= K2) +
-                               (uint32_t)(mm >= K3);
-            // mask select: keeps all four exit words live so they arrive with the
-            // thresholds in one scalar load instead of a dependent one after the compare
-            uint32_t eo = (o0 & (0u - (uint32_t)(j == 0u))) | (o1 & (0u - (uint32_t)(j == 1u))) |
-                          (o2 & (0u - (uint32_t)(j == 2u))) | (o3 & (0u - (uint32_t)(j >= 3u)));
-            if (cnt <= 4u && j >= cnt)
-                eo = 0xFFFFFFFFu;  // fall-through (or invalid transition, count 0)
-            if (cnt > 4u) {
+            if (ft) {
+                const uint32_t cnt = uni(E->count);
+                if (cnt <= 4u)
+                    return SSE_ERR_SCATTER;  // fall-through (vertex_data.jl:124) or invalid
+                const uint32_t trw = uni(E->trans);
                 const WormTab tab = worm_tab((const char *)m.wtab, m);
-                const double x = word_to_float(word);
-                uint32_t e = trw & 0xFFFFFu;
-                const uint32_t eend = e + (trw >> 20);
-                while (e < eend && !(x < tab.cum[e]))
-                    ++e;
-                if (e >= eend)
-                    return SSE_ERR_SCATTER;
-                eo = uni(tab.eout[e]);
-            } else if (eo == 0xFFFFFFFFu) {
-                return SSE_ERR_SCATTER;  // fall-through (vertex_data.jl:124) or invalid
+                const double xr = word_to_float(((uint64_t)whi << 32) | wlo);
+                uint32_t ex = trw & 0xFFFFFu;
+                const uint32_t eend = ex + (trw >> 20);
+                for (;;) {
+                    if (ex >= eend)
+                        return SSE_ERR_SCATTER;
+                    const uint64_t cb = uni64((uint64_t)__double_as_longlong(tab.cum[ex]));
+                    if (xr < __longlong_as_double((long long)cb))
+                        break;
+                    ++ex;
+                }
+                const uint32_t vc = uni(tab.eout[ex]);
+                eo = chase_out((e >> lgp << lgp) | (vc & 1023u), (vc >> 10) & 3u, vc >> 12);
             }
-            const int leg_out = (int)((eo >> 10) & 3u);
-            const int wf_out = (int)(eo >> 12);
-            rec.store(r, (eo & 1023u) | (t << 10), lane);
-            if (r == r0 && leg_out == l0 && wf_out == winv0)
+            rec.put_ecode(r, eo >> 16);
+            if ((((uint64_t)(lid & ~3u) << 14) | (eo & 0xFFFFu)) == keyA) {
+                endB = 0;
                 break;
-            len += 1;
-            wf = wf_out;
-            const uint32_t nx = R::link(lk, leg_out);
-            leg_in = (int)(nx & 3u);
-            r = (int)(nx >> 2);
-            if (r >= n)
+            }
+            const uint32_t nx = R::link(x, (eo >> 14) & 3u);
+            wfm = eo & 0x3FFFu;
+            if (R::kChecked && (nx >> 2) >= n)
                 return SSE_ERR_STATE;
-            if (r == r0 && leg_in == l0 && wf == wf0)
+            if ((((uint64_t)nx << 14) | wfm) == keyB) {
+                endB = 1;
                 break;
+            }
+            lid = nx;
         }
-        total += (double)len;
+        // worm length: scatters + 1 when the worm closed on re-entry
+        total += (double)(sc - sc0 + endB);
     }
     scat = sc;
     return 0;
@@ -757,6 +823,7 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &re
 {
     const int lane = threadIdx.x;
     const int N = m.n_sites;
+    const uint32_t lgp = (uint32_t)m.dt_lgp;
     for (int sb = 0; sb < N; sb += 64) {
         const int s = sb + lane;
         const bool vs = s < N;
@@ -778,7 +845,7 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &re
             }
         }
         if (vs && !fr && (int64_t)(f >> 2) < w.n) {
-            const uint32_t vt = rec.get_vt(f >> 2);
+            const uint32_t vt = vt_of_ecode(rec.get_ecode((uint32_t)f >> 2), lgp);
             const DevVD &vd = m.types[vt >> 10];
             val = vd.leg_states[4 * ((vt & 1023u) >> 1) + (f & 3)];
         }
@@ -797,35 +864,115 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &re
         if (vs)
             w.spins[s] = val;
     }
-    // write the worm-updated vertex codes back into the operator string
-    for (int64_t base = 0; base < w.L; base += 64) {
-        const int64_t p = base + lane;
-        const uint64_t mk = MASK[base >> 6];
-        if ((mk >> lane) & 1ull) {
-            const int r = (int)(BASE[base >> 6] + popc(mk & below_mask(lane)));
-            const uint32_t op = w.ops[p];
-            const uint32_t nop = (op & ~(1023u << 1)) | ((rec.get_vt(r) & 1023u) << 1);
-            if (nop != op)
-                w.ops[p] = nop;
+    // write the worm-updated vertex codes back into the operator string, four
+    // 64-slot chunks per pass so their loads are in flight together
+    const int64_t L = w.L;
+    for (int64_t base = 0; base < L; base += 256) {
+        uint32_t op[4], r[4];
+        bool act[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t c = (base >> 6) + u;
+            const int64_t p = base + 64 * u + lane;
+            act[u] = false;
+            op[u] = 0;
+            r[u] = 0;
+            if (base + 64 * u < L) {
+                const uint64_t mk = MASK[c];
+                if ((mk >> lane) & 1ull) {
+                    act[u] = true;
+                    r[u] = BASE[c] + (uint32_t)popc(mk & below_mask(lane));
+                    op[u] = w.ops[p];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (act[u]) {
+                const uint32_t vc = vt_of_ecode(rec.get_ecode(r[u]), lgp) & 1023u;
+                const uint32_t nop = (op[u] & ~(1023u << 1)) | (vc << 1);
+                if (nop != op[u])
+                    w.ops[base + 64 * u + lane] = nop;
+            }
         }
     }
 }
 
-// LDS bytes the worm kernel needs to hold a walker with n operators, cutoff L
-__host__ __device__ inline size_t k2_lds_need(int64_t n, int64_t L)
+// LDS bytes the worm kernel needs for a walker with n operators, cutoff L, in
+// the Rec8 (fmt 8) or Rec10 (fmt 10) layout
+__host__ __device__ inline size_t k2_lds_need(int64_t n, int64_t L, int fmt)
 {
-    size_t nch = (size_t)((L + 63) / 64);
-    return 1024 + nch * 8 + ((nch * 4 + 15) & ~(size_t)15) + (size_t)n * 8 +
-           (((size_t)n * 2 + 15) & ~(size_t)15);
+    const size_t nch = (size_t)((L + 63) / 64);
+    size_t b = 1024 + nch * 8 + ((nch * 4 + 15) & ~(size_t)15) + (size_t)n * 8;
+    if (fmt == 10)
+        b += ((size_t)n * 2 + 15) & ~(size_t)15;
+    return b;
 }
 
-// K2: LDS layout  rbuf u64[128] | mask u64[nch] | base u32[nch] |
-//                 links u64[n] | vt u16[n]
-// Walkers whose records do not fit (or n >= 16384, or > 64 bond types) chase
-// their records in global memory instead.
+// Stage the compact records of K1 (global) into the LDS layout R, converting
+// vt to ecodes and checking every ecode and link; returns false on a bad record.
+template <int FMT>
+__device__ bool k2_stage(const DevWalker &w, const DevModel &m, lds_u64 *lk, lds_u16 *ec)
+{
+    const int lane = threadIdx.x;
+    const uint32_t n = (uint32_t)w.n, lgp = (uint32_t)m.dt_lgp, nt = (uint32_t)m.n_types;
+    const uint32_t *gvt = rec_vt(w);
+    const uint4 *glk = (const uint4 *)rec_link(w);
+    const uint32_t lmax = 4u * n;
+    bool bad = false;
+    for (uint32_t i0 = 0; i0 < n; i0 += 256) {
+        uint32_t vt[4];
+        uint4 l4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t i = i0 + 64u * u + lane;
+            if (i < n) {
+                vt[u] = gvt[i];
+                l4[u] = glk[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t i = i0 + 64u * u + lane;
+            if (i < n) {
+                const uint32_t e = ecode_of_vt(vt[u], lgp);
+                bad |= (vt[u] & 1023u) >= (1u << lgp) || (vt[u] >> 10) >= nt || l4[u].x >= lmax ||
+                       l4[u].y >= lmax || l4[u].z >= lmax || l4[u].w >= lmax;
+                if (FMT == 8) {
+                    lk[i] = (uint64_t)l4[u].x | ((uint64_t)l4[u].y << 14) | ((uint64_t)l4[u].z << 28) |
+                            ((uint64_t)l4[u].w << 42) | ((uint64_t)e << 56);
+                } else {
+                    lk[i] = (uint64_t)l4[u].x | ((uint64_t)l4[u].y << 16) | ((uint64_t)l4[u].z << 32) |
+                            ((uint64_t)l4[u].w << 48);
+                    ec[i] = (uint16_t)e;
+                }
+            }
+        }
+    }
+    return __ballot(bad) == 0;
+}
+
+// In-place vt -> ecode conversion of the global records (RecG path).
+__device__ bool k2_stage_global(const DevWalker &w, const DevModel &m)
+{
+    const int lane = threadIdx.x;
+    const uint32_t n = (uint32_t)w.n, lgp = (uint32_t)m.dt_lgp, nt = (uint32_t)m.n_types;
+    uint32_t *gvt = rec_vt(w);
+    bool bad = false;
+    for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t vt = gvt[i];
+        bad |= (vt & 1023u) >= (1u << lgp) || (vt >> 10) >= nt;
+        gvt[i] = ecode_of_vt(vt, lgp);
+    }
+    return __ballot(bad) == 0;
+}
+
+// K2.  LDS layout  rbuf u64[128] | mask u64[nch] | base u32[nch] | records.
+// rec_mode 0 picks the smallest layout that fits (Rec8, Rec10, then RecG);
+// 1 skips Rec8; 2 forces RecG.
 __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevModel *models,
                                                        DevPool *pool, int n_walkers, int full_sweep,
-                                                       int lds_bytes)
+                                                       int lds_bytes, int rec_mode)
 {
     extern __shared__ __align__(16) char lds[];
     const int wi = blockIdx.x;
@@ -841,46 +988,58 @@ __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevMo
     WaveStream128 rs;
     rs.init(rbuf, w.key, w.ctr);
     const int64_t nch = (w.L + 63) / 64;
-    const bool fits = w.n < 16384 && m.n_types <= 64 && k2_lds_need(w.n, w.L) <= (size_t)lds_bytes;
-    ConstEntry *DT = (ConstEntry *)(const void *)m.dtab;
+    const bool small_codes = ((int64_t)m.n_types << m.dt_lgp) <= 256;
+    int fmt = 0;
+    if (rec_mode == 0 && w.n <= 4096 && small_codes && k2_lds_need(w.n, w.L, 8) <= (size_t)lds_bytes)
+        fmt = 8;
+    else if (rec_mode <= 1 && w.n <= 16384 && ((int64_t)m.n_types << m.dt_lgp) <= 65536 &&
+             k2_lds_need(w.n, w.L, 10) <= (size_t)lds_bytes)
+        fmt = 10;
     double total = 1.0;
     int64_t scat = 0;
-    int err;
-    if (fits) {
+    int err = 0;
+    if (fmt) {
         char *q = lds + 1024;
         uint64_t *lmask = (uint64_t *)q;
         q += nch * 8;
         uint32_t *lbase = (uint32_t *)q;
         q += (nch * 4 + 15) & ~(int64_t)15;
-        uint64_t *llk = (uint64_t *)q;
+        lds_u64 *llk = (lds_u64 *)(uint64_t *)q;
         q += w.n * 8;
-        uint16_t *lvt = (uint16_t *)q;
+        lds_u16 *lec = (lds_u16 *)(uint16_t *)q;
         const uint64_t *gmask = rec_mask(w);
         const uint32_t *gbase = rec_base(w);
         for (int64_t i = lane; i < nch; i += 64) {
             lmask[i] = gmask[i];
             lbase[i] = gbase[i];
         }
-        const uint32_t *gvt = rec_vt(w);
-        const uint4 *glk = (const uint4 *)rec_link(w);
-        for (int64_t i = lane; i < w.n; i += 64) {
-            lvt[i] = (uint16_t)gvt[i];
-            const uint4 l4 = glk[i];
-            llk[i] = (uint64_t)(l4.x & 0xFFFFu) | ((uint64_t)(l4.y & 0xFFFFu) << 16) |
-                     ((uint64_t)(l4.z & 0xFFFFu) << 32) | ((uint64_t)(l4.w & 0xFFFFu) << 48);
+        const bool ok = fmt == 8 ? k2_stage<8>(w, m, llk, lec) : k2_stage<10>(w, m, llk, lec);
+        __syncthreads();
+        if (!ok) {
+            err = SSE_ERR_STATE;
+        } else if (fmt == 8) {
+            Rec8 rec{llk};
+            err = worm_chase(w, m, rs, rec, lmask, lbase, total, scat);
+            __syncthreads();
+            if (!err)
+                worm_finish(w, m, rs, rec, lmask, lbase);
+        } else {
+            Rec10 rec{llk, lec};
+            err = worm_chase(w, m, rs, rec, lmask, lbase, total, scat);
+            __syncthreads();
+            if (!err)
+                worm_finish(w, m, rs, rec, lmask, lbase);
         }
-        __syncthreads();
-        RecLds rec{lvt, llk};
-        err = worm_body(w, m, rs, rec, lmask, lbase, DT, total, scat);
-        __syncthreads();
-        if (!err)
-            worm_finish(w, m, rs, rec, lmask, lbase);
     } else {
-        RecGlobal rec{rec_vt(w), (const uint4 *)rec_link(w)};
-        err = worm_body(w, m, rs, rec, rec_mask(w), rec_base(w), DT, total, scat);
-        __syncthreads();
-        if (!err)
-            worm_finish(w, m, rs, rec, rec_mask(w), rec_base(w));
+        if (!k2_stage_global(w, m)) {
+            err = SSE_ERR_STATE;
+        } else {
+            RecG rec{rec_vt(w), (const_u4 *)(const void *)rec_link(w)};
+            err = worm_chase(w, m, rs, rec, rec_mask(w), rec_base(w), total, scat);
+            __syncthreads();
+            if (!err)
+                worm_finish(w, m, rs, rec, rec_mask(w), rec_base(w));
+        }
     }
     if (err) {
         if (lane == 0) {
@@ -1106,8 +1265,12 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
         c.lds_limit = (size_t)lim;
     if (c.lds_k1 > c.lds_limit)
         return SSE_ERR_INVALID;
-    for (const DevModel &m : models)
+    c.small_codes = true;
+    for (const DevModel &m : models) {
         c.max_wtab = std::max(c.max_wtab, (int)m.dtab_bytes);
+        if (((int64_t)m.n_types << m.dt_lgp) > 256)
+            c.small_codes = false;
+    }
     c.lds_k2 = 1024 + 4096;
     if (c.lds_limit > 65536) {
         (void)hipFuncSetAttribute((const void *)wave_worm_kernel,
@@ -1122,7 +1285,8 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
 // headroom; walkers that still do not fit use the global-memory chase.
 static inline void wave_resize_lds(WaveConfig &c, unsigned long long max_n, unsigned long long max_L)
 {
-    size_t need = k2_lds_need((int64_t)(max_n + max_n / 16 + 32), (int64_t)max_L);
+    const int64_t nh = (int64_t)(max_n + max_n / 16 + 32);
+    size_t need = k2_lds_need(nh, (int64_t)max_L, nh <= 4096 && c.small_codes && c.rec_mode == 0 ? 8 : 10);
     need = (need + 511) & ~(size_t)511;
     if (need > c.lds_limit)
         need = c.lds_limit;
@@ -1151,7 +1315,7 @@ static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, 
         (void)hipEventRecord(ev[1], s);
     if (phases & 4)
         hipLaunchKernelGGL(wave_worm_kernel, dim3(n), dim3(64), c.lds_k2, s, W, M, pool, n,
-                           (phases & 7) == 7 ? 1 : 0, (int)c.lds_k2);
+                           (phases & 7) == 7 ? 1 : 0, (int)c.lds_k2, c.rec_mode);
     if (ev)
         (void)hipEventRecord(ev[2], s);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;

@@ -37,11 +37,14 @@ def _compare(batch, oracles, tag):
             assert gs[key] == os_[key], f"{tag} walker {k}: {key} {gs[key]} != {os_[key]}"
 
 
-def _run_parity(sse, oracle_mod, model, T, n_walkers, sweeps, mode, seed0=0, **kw):
+def _run_parity(sse, oracle_mod, model, T, n_walkers, sweeps, mode, seed0=0, records=None, **kw):
     flat = model.flat()
     walkers = [sse.walker_desc(T=T, seed=seed0 + s, **kw) for s in range(n_walkers)]
-    _say(f"parity: mode={mode} N={flat.n_sites} walkers={n_walkers} sweeps={sweeps}")
+    _say(f"parity: mode={mode} N={flat.n_sites} walkers={n_walkers} sweeps={sweeps} "
+         f"records={records}")
     batch = sse.Batch([flat], walkers, mode=mode)
+    if records is not None:
+        batch.set_option(sse.capi.SSE_OPT_WORM_RECORDS, records)
     oracles = [oracle_mod.OracleWalker(sse.capi, flat, w) for w in walkers]
     batch.init()
     for o in oracles:
@@ -95,6 +98,16 @@ def test_parity_ladder_dimer_basis(sse_pkg, oracle_mod, mode):
 def test_parity_bilayer_cluster(sse_pkg, oracle_mod, mode):
     _run_parity(sse_pkg, oracle_mod, job_bilayer(sse_pkg, 0.2), 0.2, 4, 25, mode,
                 thermalization=10)
+
+
+@pytest.mark.parametrize("records", [1, 2])
+def test_parity_worm_record_layouts(sse_pkg, oracle_mod, records):
+    # 10-byte LDS records (1) and global-memory records (2) against the oracle;
+    # the default 8-byte layout runs in every other wave-mode test
+    _run_parity(sse_pkg, oracle_mod, square_heisenberg(sse_pkg, 4), 0.5, 6, 25, 0,
+                thermalization=10, records=records)
+    _run_parity(sse_pkg, oracle_mod, job_honeycomb(sse_pkg, 0.4), 0.4, 4, 20, 0, seed0=50,
+                thermalization=10, records=records)
 
 
 @pytest.mark.parametrize("mode", MODES)
