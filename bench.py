@@ -7,14 +7,18 @@ beta=8, Philox seeds 0..4095, random initial spins, cutoff grown by the
 reference rule, 2000 thermalization sweeps (untimed setup).  One step = one
 sweep (diagonal update + vertex list + worm update) of every walker followed
 by the measurement (energy, uniform and staggered magnetization family),
-accumulated on the device.
+accumulated on the device.  The timed steps run in calls of --chunk sweeps;
+the walkers are split into --streams groups, each swept on its own HIP stream
+so one group's slow worm-update tail overlaps the others' sweeps.
 
 Reported beside it, in the same run:
   * the plain GPU form: the same walkers (thermalized states) run one walker
     per launch with the lane kernel, and with the wave kernels;
   * the CPU baseline: the C restatement oracle (one walker per thread) on
     thermalized states copied from the GPU, for a bounded time;
-  * the roofline of the dominant kernel from HIP events on the batch stream.
+  * the roofline of the dominant kernel (the worm update): three extra
+    sweeps on one stream (untimed diagnostic) with HIP events around each
+    kernel give its launch duration for the whole batch.
 
 Vertex updates per walker-sweep U = L + W (W = scatter calls); algorithmic
 HBM bytes per walker-sweep B = 16L + 16n + 12W + 3N.
@@ -47,14 +51,6 @@ def build_workload(sse, L, beta, walkers, thermalization):
     wds = [sse.walker_desc(T=1.0 / beta, seed=s, thermalization=thermalization)
            for s in range(walkers)]
     return flat, wds
-
-
-def kernel_bytes(Ls, ns, Ws, N, walker_sweeps):
-    """Algorithmic bytes per kernel: diagonal+vertex list 8L + 4L + 16n + 2N
-    (spins in/out), worm 12W + N, measure 4L."""
-    return {"diag_vlist": 12 * Ls + 16 * ns + 2 * N * walker_sweeps,
-            "worm": 12 * Ws + N * walker_sweeps,
-            "measure": 4 * Ls}
 
 
 def plain_form(sse, flat, wds, states, sweeps, mode):
@@ -233,8 +229,12 @@ def extra_serving(sse, seconds=30.0, sweep_scale=1e-3, slots=1024):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--chunk", type=int, default=20,
+                    help="sweeps per timed call")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="walker groups, each swept on its own HIP stream")
     ap.add_argument("--walkers", type=int, default=4096)
     ap.add_argument("--lattice", type=int, default=16)
     ap.add_argument("--beta", type=float, default=8.0)
@@ -268,32 +268,51 @@ def main():
         sc, _, _ = batch.get_walker(0, with_ops=False, with_spins=False)
         log(f"thermalization {done}/{args.thermalization} sweeps ({time.perf_counter() - t0:.1f} s)"
             f" walker0 L={sc['cutoff']} n={sc['num_operators']} worms={sc['num_worms']:.2f}")
-    for _ in range(args.warmup):
-        batch.run(1, measure=True)
+    if args.warmup:
+        batch.run(args.warmup, measure=True)
     batch.reset_accumulators()
 
-    step_ms = []
-    ktimes = np.zeros(3)
-    for s in range(args.steps):
+    # timed region: `steps` sweeps (+ fused measurement) of every walker, in
+    # calls of `chunk` sweeps, walkers split over `streams` stream groups
+    batch.set_option(sse.capi.SSE_OPT_STREAMS, args.streams)
+    chunk = max(1, min(args.chunk, args.steps))
+    call_ms, dev_ms = [], 0.0
+    done = 0
+    while done < args.steps:
+        k = min(chunk, args.steps - done)
         t1 = time.perf_counter()
-        batch.run(1, measure=True)
-        step_ms.append((time.perf_counter() - t1) * 1e3)
-        ktimes += batch.phase_times()
-    total_s = sum(step_ms) / 1e3
+        batch.run(k, measure=True)
+        call_ms.append((time.perf_counter() - t1) * 1e3 / k)
+        ms, sw = batch.call_time()
+        dev_ms += ms
+        done += k
     Ls, ns, Ws, wsweeps = batch.work_counters()
+    total_s = sum(call_ms[i] * min(chunk, args.steps - i * chunk) for i in range(len(call_ms))) / 1e3
     U = Ls + Ws
     B = 16 * Ls + 16 * ns + 12 * Ws + 3 * N * wsweeps
     value = U / total_s
-    log(f"timed {args.steps} steps: {total_s:.3f} s, {wsweeps} walker-sweeps, "
-        f"<L>={Ls / wsweeps:.0f} <n>={ns / wsweeps:.0f} <W>={Ws / wsweeps:.0f}")
+    log(f"timed {args.steps} steps: {total_s:.3f} s wall, {dev_ms / 1e3:.3f} s device, "
+        f"{wsweeps} walker-sweeps, <L>={Ls / wsweeps:.0f} <n>={ns / wsweeps:.0f} "
+        f"<W>={Ws / wsweeps:.0f}")
 
-    kb = kernel_bytes(Ls, ns, Ws, N, wsweeps)
+    # roofline of the dominant kernel: three more sweeps on one stream with
+    # HIP events around each kernel (untimed diagnostic); bytes per launch
+    # from the per-walker-sweep averages of the timed region
+    batch.set_option(sse.capi.SSE_OPT_STREAMS, 1)
+    ktimes = np.zeros(3)
+    for _ in range(3):
+        batch.run(1, measure=True)
+        ktimes += batch.phase_times()
+    batch.set_option(sse.capi.SSE_OPT_STREAMS, args.streams)
     names = ["diag_vlist", "worm", "measure"]
-    kms = {n: ktimes[i] / args.steps for i, n in enumerate(names)}
+    kms = {n: ktimes[i] / 3 for i, n in enumerate(names)}
+    nw = args.walkers
+    per_launch = {"diag_vlist": nw * (12 * Ls + 16 * ns) / wsweeps + 2 * N * nw,
+                  "worm": nw * 12 * Ws / wsweeps + N * nw,
+                  "measure": nw * 4 * Ls / wsweeps}
     dom = max(names, key=lambda n: kms[n])
-    per_launch_bytes = kb[dom] / args.steps
-    achieved = per_launch_bytes / (kms[dom] * 1e-3) / 1e9
-    log(f"kernel ms/launch: " + ", ".join(f"{n}={kms[n]:.3f}" for n in names))
+    achieved = per_launch[dom] / (kms[dom] * 1e-3) / 1e9
+    log("single-stream kernels ms/launch: " + ", ".join(f"{n}={kms[n]:.3f}" for n in names))
 
     states = [batch.get_walker(k) for k in range(max(args.plain_walkers, 8))]
     baselines = {}
@@ -322,7 +341,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": float(np.mean(step_ms)),
+        "ms_per_step": total_s * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -330,17 +349,21 @@ def main():
         "data": "synthetic (Philox-seeded random initial spins, thermalized chains)",
         "config": {"workload": f"batched Heisenberg {args.lattice}x{args.lattice} beta={args.beta:g}",
                    "walkers": args.walkers, "thermalization_sweeps": args.thermalization,
-                   "parallelism": "one wavefront per walker"},
+                   "parallelism": f"one wavefront per walker, {args.streams} stream groups"},
         "sweeps_per_s": wsweeps / total_s,
         "hbm_gbs_algorithmic": B / total_s / 1e9,
         "hbm_roofline_frac": B / total_s / 1e9 / HBM_PEAK_GBS,
-        "p50_step_ms": float(np.percentile(step_ms, 50)),
-        "p99_step_ms": float(np.percentile(step_ms, 99)),
+        "step_ms_per_call": {"chunk": chunk, "p50": float(np.percentile(call_ms, 50)),
+                             "p99": float(np.percentile(call_ms, 99))},
+        "device_ms_per_step": dev_ms / args.steps,
         "per_walker_sweep": {"L": Ls / wsweeps, "n": ns / wsweeps, "W": Ws / wsweeps,
                              "U": U / wsweeps, "B": B / wsweeps},
-        "kernel_ms_per_launch": kms,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "kernel_ms_per_launch_single_stream": kms,
+        "roofline": {"bound": "hbm", "kernel": {"diag_vlist": "wave_diag_vlist_kernel",
+                                                "worm": "wave_worm_kernel",
+                                                "measure": "wave_measure_kernel"}[dom],
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
     }
     out.update(baselines)

@@ -195,6 +195,10 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links,
  * (diagonal+vertex list, worm, measure), measured with HIP events on the
  * batch stream.                                                            */
 int sse_phase_times(const sse_batch *b, double *ms3);
+/* Device time (ms, HIP events on the batch stream) of the last sse_sweep /
+ * sse_run call and its sweep count (0, 0 with timing off).  No reference
+ * counterpart: the reference runs one walker per process.                  */
+int sse_call_time(const sse_batch *b, double *ms, int64_t *sweeps);
 /* Sum over walkers of cutoff L, operators n and scatter calls W since the
  * last reset (counters taken from the chains themselves).                  */
 int sse_work_counters(sse_batch *b, int64_t *L_sum, int64_t *n_sum,
@@ -224,14 +228,32 @@ int sse_pool_usage(sse_batch *b, int64_t *used_bytes, int64_t *total_bytes);
 /* ---- options -------------------------------------------------------------
  * SSE_OPT_GRAPH  (default 1): sse_run/sse_sweep replay a captured HIP graph of
  *                up to 32 sweeps (+ measures) instead of launching each kernel.
- * SSE_OPT_TIMING (default 1): calls of <= 256 sweeps record HIP events around
- *                each kernel (read with sse_phase_times) and launch eagerly.
+ * SSE_OPT_TIMING (default 1): every sweep/run call is bracketed by HIP events
+ *                (sse_call_time); single-stream calls of <= 256 sweeps also
+ *                record events around each kernel (sse_phase_times) and
+ *                launch eagerly.
  * SSE_OPT_WORM_RECORDS (default 0): record layout of the worm kernel.  0 picks
  *                the smallest that fits (8-byte LDS records for n <= 4096 and
  *                <= 256 chase codes, 10-byte LDS records for n <= 16384, else
  *                global memory); 1 skips the 8-byte layout; 2 forces global
- *                memory.  Results are identical; this is a tuning/testing knob. */
-enum sse_option { SSE_OPT_GRAPH = 1, SSE_OPT_TIMING = 2, SSE_OPT_WORM_RECORDS = 3 };
+ *                memory.  Results are identical; this is a tuning/testing knob.
+ * SSE_OPT_PERSISTENT (default 0): persistent multi-sweep launches (workers
+ *                take (walker, sweep) tasks in order).  Disabled in this build:
+ *                setting it to 1 returns SSE_ERR_INVALID (its first GPU run
+ *                faulted and the cause is not yet found).
+ * SSE_OPT_STREAMS (default 2): wave mode splits the walkers into this many
+ *                groups, each swept on its own HIP stream (forked from and
+ *                joined into the batch stream), so the slow tail of one
+ *                group's worm update overlaps the other groups' sweeps.  1 =
+ *                one stream (graphs and per-kernel timing apply only then).
+ *                Results are identical for any value.                           */
+enum sse_option {
+    SSE_OPT_GRAPH = 1,
+    SSE_OPT_TIMING = 2,
+    SSE_OPT_WORM_RECORDS = 3,
+    SSE_OPT_PERSISTENT = 4,
+    SSE_OPT_STREAMS = 5
+};
 int sse_set_option(sse_batch *b, int32_t option, int64_t value);
 
 /* Encoding helpers (src/opercode.jl:18-71). */

@@ -21,6 +21,8 @@ SSE_MODE_LANE = 1
 SSE_OPT_GRAPH = 1
 SSE_OPT_TIMING = 2
 SSE_OPT_WORM_RECORDS = 3
+SSE_OPT_PERSISTENT = 4
+SSE_OPT_STREAMS = 5
 
 STATUS = {0: "SSE_OK", 1: "SSE_ERR_INVALID", 2: "SSE_ERR_HIP", 3: "SSE_ERR_OOM",
           4: "SSE_ERR_SCATTER", 5: "SSE_ERR_NO_DEVICE", 6: "SSE_ERR_STATE"}
@@ -185,6 +187,7 @@ def load_library(path=None):
     lib.sse_get_vertex_list.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.c_int64,
                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     lib.sse_phase_times.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.sse_call_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     lib.sse_work_counters.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.sse_opcode_to_reference.argtypes = [C.c_uint32]
@@ -210,7 +213,8 @@ EXPORTED_SYMBOLS = (
     "sse_get_walker", "sse_set_walker", "sse_sweep_phases", "sse_get_vertex_list",
     "sse_phase_times", "sse_work_counters", "sse_opcode_to_reference",
     "sse_opcode_from_reference", "sse_last_error", "sse_abi_version", "sse_add_model",
-    "sse_slot_start", "sse_slot_stop", "sse_slot_poll", "sse_pool_usage", "sse_set_option")
+    "sse_slot_start", "sse_slot_stop", "sse_slot_poll", "sse_pool_usage", "sse_set_option",
+    "sse_call_time")
 
 
 class SSEError(RuntimeError):
@@ -376,6 +380,12 @@ class Batch:
         t = np.zeros(3)
         check(self.lib, self.lib.sse_phase_times(self.handle, _ptr(t, C.c_double)))
         return t
+
+    def call_time(self):
+        """(device ms, sweeps) of the last sweep/run call (HIP events)."""
+        ms, sw = C.c_double(), C.c_int64()
+        check(self.lib, self.lib.sse_call_time(self.handle, C.byref(ms), C.byref(sw)))
+        return ms.value, sw.value
 
     def work_counters(self):
         a = [C.c_int64() for _ in range(4)]

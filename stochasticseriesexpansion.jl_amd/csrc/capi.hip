@@ -76,6 +76,16 @@ struct sse_batch {
     const DevModel *g_models = nullptr;
     double phase_ms[3] = {0, 0, 0};
     int64_t phase_launches = 0;
+    // wave mode: walkers split into groups, each swept on its own stream, so
+    // one group's sweeps overlap the slow tail of another's
+    int n_groups = 2;
+    std::vector<hipStream_t> gstreams;
+    std::vector<hipEvent_t> gjoin;
+    hipEvent_t gfork = nullptr;
+    // device time of the last sse_sweep/sse_run call (HIP events, timing on)
+    hipEvent_t call_ev[2] = {nullptr, nullptr};
+    double call_ms = 0;
+    int64_t call_sweeps = 0;
     WaveConfig wave;
 };
 
@@ -630,6 +640,15 @@ int sse_batch_destroy(sse_batch *b)
         (void)hipEventDestroy(e);
     if (b->gexec)
         (void)hipGraphExecDestroy(b->gexec);
+    for (hipStream_t gs : b->gstreams)
+        (void)hipStreamDestroy(gs);
+    for (hipEvent_t e : b->gjoin)
+        (void)hipEventDestroy(e);
+    if (b->gfork)
+        (void)hipEventDestroy(b->gfork);
+    for (int i = 0; i < 2; ++i)
+        if (b->call_ev[i])
+            (void)hipEventDestroy(b->call_ev[i]);
     if (b->stream)
         (void)hipStreamDestroy(b->stream);
     delete b;
@@ -651,7 +670,7 @@ static int launch_sweep(sse_batch *b, int phases, hipEvent_t *ev = nullptr)
             HIPCHECK(hipEventRecord(ev[2], b->stream));
         }
     } else {
-        int rc = wave_launch_sweep(b->wave, b->stream, b->d_walkers, b->d_models, b->d_pool, n,
+        int rc = wave_launch_sweep(b->wave, b->stream, b->d_walkers, b->d_models, b->d_pool, 0, n,
                                    phases, ev);
         if (rc)
             return rc;
@@ -688,6 +707,13 @@ static void collect_phase_times(sse_batch *b, int n_sweeps, bool with_measure)
     for (int i = 0; i < 3; ++i)
         b->phase_ms[i] = t[i];
     b->phase_launches = n_sweeps;
+}
+
+static void clear_phase_times(sse_batch *b)
+{
+    for (int i = 0; i < 3; ++i)
+        b->phase_ms[i] = 0;
+    b->phase_launches = 0;
 }
 
 static int launch_measure(sse_batch *b, int mode);
@@ -730,6 +756,76 @@ static int graph_run(sse_batch *b, int sweeps, int measure)
         b->g_models = b->d_models;
     }
     HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
+    return SSE_OK;
+}
+
+// Device time of a sweep/run call: stage 0 records the start event, 1 the
+// end event, 2 (after the call synchronized) reads the elapsed time.
+static int call_timer(sse_batch *b, int stage, int64_t sweeps)
+{
+    if (stage == 0) {
+        b->call_ms = 0;
+        b->call_sweeps = 0;
+    }
+    if (!b->use_timing)
+        return SSE_OK;
+    for (int i = 0; i < 2; ++i)
+        if (!b->call_ev[i])
+            HIPCHECK(hipEventCreate(&b->call_ev[i]));
+    if (stage < 2) {
+        HIPCHECK(hipEventRecord(b->call_ev[stage], b->stream));
+        if (stage == 1)
+            b->call_sweeps = sweeps;
+        return SSE_OK;
+    }
+    float t = 0;
+    HIPCHECK(hipEventElapsedTime(&t, b->call_ev[0], b->call_ev[1]));
+    b->call_ms = t;
+    return SSE_OK;
+}
+
+// Wave mode with n_groups > 1: walkers [g*n/G, (g+1)*n/G) are swept on stream
+// g.  The group streams fork from and join back into the batch stream, so
+// everything else (copies, syncs, the next call) stays ordered on it.  Sweeps
+// are issued round-robin over the groups; a group whose slowest walker is
+// still in its worm update leaves the rest of the device to the others.
+static int groups_run(sse_batch *b, int n_sweeps, int measure)
+{
+    const int n = b->n_walkers;
+    const int G = std::max(1, std::min(b->n_groups, n));
+    // group 0 runs on the batch stream itself: G groups use G streams (the
+    // device maps streams onto a few hardware queues; streams sharing one
+    // queue would serialize behind each other)
+    while ((int)b->gstreams.size() < G - 1) {
+        hipStream_t gs;
+        HIPCHECK(hipStreamCreateWithFlags(&gs, hipStreamNonBlocking));
+        b->gstreams.push_back(gs);
+        hipEvent_t e;
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->gjoin.push_back(e);
+    }
+    if (!b->gfork)
+        HIPCHECK(hipEventCreateWithFlags(&b->gfork, hipEventDisableTiming));
+    HIPCHECK(hipEventRecord(b->gfork, b->stream));
+    for (int g = 1; g < G; ++g)
+        HIPCHECK(hipStreamWaitEvent(b->gstreams[g - 1], b->gfork, 0));
+    for (int k = 0; k < n_sweeps; ++k) {
+        for (int g = 0; g < G; ++g) {
+            hipStream_t gs = g == 0 ? b->stream : b->gstreams[g - 1];
+            const int w0 = (int)((int64_t)n * g / G), w1 = (int)((int64_t)n * (g + 1) / G);
+            int rc = wave_launch_sweep(b->wave, gs, b->d_walkers, b->d_models, b->d_pool, w0, w1, 7,
+                                       nullptr);
+            if (!rc && measure)
+                rc = wave_launch_measure(b->wave, gs, b->d_walkers, b->d_models, b->d_out, b->d_sum,
+                                         b->d_sumsq, b->d_count, w0, w1, 1, b->d_done);
+            if (rc)
+                return fail(rc, "group sweep launch failed");
+        }
+    }
+    for (int g = 1; g < G; ++g) {
+        HIPCHECK(hipEventRecord(b->gjoin[g - 1], b->gstreams[g - 1]));
+        HIPCHECK(hipStreamWaitEvent(b->stream, b->gjoin[g - 1], 0));
+    }
     return SSE_OK;
 }
 
@@ -802,8 +898,16 @@ int sse_sweep(sse_batch *b, int32_t n_sweeps)
         if (rc)
             return rc;
     }
-    const bool timed = b->use_timing && n_sweeps <= 256;
-    if (timed) {
+    const bool grouped = b->mode == SSE_MODE_WAVE && b->n_groups > 1 && b->n_walkers > 1;
+    const bool timed = !grouped && b->use_timing && n_sweeps <= 256;
+    int rc0 = call_timer(b, 0, 0);
+    if (rc0)
+        return rc0;
+    if (grouped) {
+        int rc = groups_run(b, n_sweeps, 0);
+        if (rc)
+            return rc;
+    } else if (timed) {
         int rc = ensure_events(b, 4 * (size_t)n_sweeps);
         if (rc)
             return rc;
@@ -830,9 +934,16 @@ int sse_sweep(sse_batch *b, int32_t n_sweeps)
                            b->d_walkers, b->n_walkers, b->d_done);
         HIPCHECK(hipGetLastError());
     }
+    int rc1 = call_timer(b, 1, n_sweeps);
+    if (rc1)
+        return rc1;
     int rc = check_walker_errors(b);
+    if (!rc)
+        rc = call_timer(b, 2, 0);
     if (timed)
         collect_phase_times(b, n_sweeps, false);
+    else
+        clear_phase_times(b);
     return rc;
 }
 
@@ -859,8 +970,7 @@ static int launch_measure(sse_batch *b, int mode)
                            b->d_scratch, b->max_sites, n, mode, b->d_done);
     } else {
         int rc = wave_launch_measure(b->wave, b->stream, b->d_walkers, b->d_models, b->d_out,
-                                     b->d_sum, b->d_sumsq, b->d_count, b->d_scratch, b->max_sites,
-                                     n, mode, b->d_done);
+                                     b->d_sum, b->d_sumsq, b->d_count, 0, n, mode, b->d_done);
         if (rc)
             return rc;
     }
@@ -902,8 +1012,16 @@ int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure)
         if (rc)
             return rc;
     }
-    const bool timed = b->use_timing && n_sweeps <= 256;
-    if (timed) {
+    const bool grouped = b->mode == SSE_MODE_WAVE && b->n_groups > 1 && b->n_walkers > 1;
+    const bool timed = !grouped && b->use_timing && n_sweeps <= 256;
+    int rc0 = call_timer(b, 0, 0);
+    if (rc0)
+        return rc0;
+    if (grouped) {
+        int rc = groups_run(b, n_sweeps, measure ? 1 : 0);
+        if (rc)
+            return rc;
+    } else if (timed) {
         int rc = ensure_events(b, 4 * (size_t)n_sweeps);
         if (rc)
             return rc;
@@ -935,9 +1053,16 @@ int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure)
                            b->d_walkers, b->n_walkers, b->d_done);
         HIPCHECK(hipGetLastError());
     }
+    int rc1 = call_timer(b, 1, n_sweeps);
+    if (rc1)
+        return rc1;
     int rc = check_walker_errors(b);
+    if (!rc)
+        rc = call_timer(b, 2, 0);
     if (timed)
         collect_phase_times(b, n_sweeps, measure != 0);
+    else
+        clear_phase_times(b);
     return rc;
 }
 
@@ -1334,6 +1459,17 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
     case SSE_OPT_TIMING:
         b->use_timing = value != 0;
         return SSE_OK;
+    case SSE_OPT_PERSISTENT:
+        if (value != 0)  // switched off until the fault of its first GPU run is explained
+            return fail(SSE_ERR_INVALID, "SSE_OPT_PERSISTENT is disabled in this build: its first "
+                                         "GPU run ended in a memory fault whose cause is not found");
+        return SSE_OK;
+    case SSE_OPT_STREAMS:
+        if (value < 1 || value > 64)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_STREAMS takes 1..64 walker groups");
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        b->n_groups = (int)value;
+        return SSE_OK;
     case SSE_OPT_WORM_RECORDS:
         if (value < 0 || value > 2)
             return fail(SSE_ERR_INVALID, "SSE_OPT_WORM_RECORDS takes 0, 1 or 2");
@@ -1348,6 +1484,17 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
     default:
         return fail(SSE_ERR_INVALID, "unknown option");
     }
+}
+
+int sse_call_time(const sse_batch *b, double *ms, int64_t *sweeps)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    if (ms)
+        *ms = b->call_ms;
+    if (sweeps)
+        *sweeps = b->call_sweeps;
+    return SSE_OK;
 }
 
 int sse_phase_times(const sse_batch *b, double *ms3)

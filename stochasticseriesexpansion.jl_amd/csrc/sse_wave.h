@@ -231,14 +231,9 @@ __device__ void k1_serial_chunk(DevWalker &w, const DevModel &m, WaveStream &rs,
 // phases: 1 diagonal update, 2 vertex list, 8 force the slot-by-slot path.
 // warm >= 0: init warm-up round (diagonal only; walkers with fewer warm-up
 // sweeps are skipped).
-__global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const DevModel *models,
-                                                             DevPool *pool, int n_walkers,
-                                                             int phases, int warm)
+__device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
+                                          int phases, int warm, char *lds)
 {
-    extern __shared__ __align__(16) char lds[];
-    const int wi = blockIdx.x;
-    if (wi >= n_walkers)
-        return;
     const int lane = threadIdx.x;
     DevWalker w = W[wi];
     if (w.error)
@@ -567,6 +562,18 @@ __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const
         atomicMax(&pool->max_n, (unsigned long long)n0);
         atomicMax(&pool->max_L, (unsigned long long)L);
     }
+}
+
+// kernels of the per-sweep path take a walker range [w_begin, w_end), one
+// workgroup (one wave) per walker
+__global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const DevModel *models,
+                                                             DevPool *pool, int w_begin, int w_end,
+                                                             int phases, int warm)
+{
+    extern __shared__ __align__(16) char lds[];
+    const int wi = w_begin + (int)blockIdx.x;
+    if (wi < w_end)
+        k1_walker(W, models, pool, wi, phases, warm, lds);
 }
 
 // ------------------------------------------------------------------ K2
@@ -967,17 +974,12 @@ __device__ bool k2_stage_global(const DevWalker &w, const DevModel &m)
     return __ballot(bad) == 0;
 }
 
-// K2.  LDS layout  rbuf u64[128] | mask u64[nch] | base u32[nch] | records.
-// rec_mode 0 picks the smallest layout that fits (Rec8, Rec10, then RecG);
-// 1 skips Rec8; 2 forces RecG.
-__global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevModel *models,
-                                                       DevPool *pool, int n_walkers, int full_sweep,
-                                                       int lds_bytes, int rec_mode)
+// Worm update of walker wi.  LDS layout  rbuf u64[128] | mask u64[nch] |
+// base u32[nch] | records.  RecG reads links through the scalar cache, which is
+// valid because K1 wrote them in an earlier kernel launch.
+__device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
+                                          int full_sweep, int lds_bytes, int rec_mode, char *lds)
 {
-    extern __shared__ __align__(16) char lds[];
-    const int wi = blockIdx.x;
-    if (wi >= n_walkers)
-        return;
     const int lane = threadIdx.x;
     DevWalker w = W[wi];
     if (w.error || !w.active)
@@ -1073,6 +1075,18 @@ __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevMo
     }
 }
 
+// K2.  rec_mode 0 picks the smallest record layout that fits (Rec8, Rec10,
+// then RecG); 1 skips Rec8; 2 forces RecG.
+__global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevModel *models,
+                                                       DevPool *pool, int w_begin, int w_end,
+                                                       int full_sweep, int lds_bytes, int rec_mode)
+{
+    extern __shared__ __align__(16) char lds[];
+    const int wi = w_begin + (int)blockIdx.x;
+    if (wi < w_end)
+        k2_walker(W, models, pool, wi, full_sweep, lds_bytes, rec_mode, lds);
+}
+
 // ------------------------------------------------------------------ init
 __global__ __launch_bounds__(64) void wave_init_kernel(DevWalker *W, const DevModel *models, int n_walkers)
 {
@@ -1124,14 +1138,9 @@ __device__ __forceinline__ double wave_incl_scan(double x)
 
 // mode 0: explicit measure of every walker; mode 1: thermalized walkers only,
 // accumulated into the per-walker sums (fused sweep+measure loop).
-__global__ __launch_bounds__(64) void wave_measure_kernel(DevWalker *W, const DevModel *models,
-                                                          double *out, double *sum, double *sumsq,
-                                                          int64_t *count, int n_walkers, int mode,
-                                                          int32_t *done)
+__device__ __forceinline__ void k3_walker(DevWalker *W, const DevModel *models, double *out, double *sum,
+                                          double *sumsq, int64_t *count, int wi, int mode, int32_t *done)
 {
-    const int wi = blockIdx.x;
-    if (wi >= n_walkers)
-        return;
     const int lane = threadIdx.x;
     const DevWalker w = W[wi];
     if (w.error || (mode == 1 && !w.active))
@@ -1241,6 +1250,16 @@ __global__ __launch_bounds__(64) void wave_measure_kernel(DevWalker *W, const De
     }
 }
 
+__global__ __launch_bounds__(64) void wave_measure_kernel(DevWalker *W, const DevModel *models,
+                                                          double *out, double *sum, double *sumsq,
+                                                          int64_t *count, int w_begin, int w_end,
+                                                          int mode, int32_t *done)
+{
+    const int wi = w_begin + (int)blockIdx.x;
+    if (wi < w_end)
+        k3_walker(W, models, out, sum, sumsq, count, wi, mode, done);
+}
+
 // pending walkers become active once initialised
 __global__ void activate_pending_kernel(DevWalker *W, int n_walkers)
 {
@@ -1299,22 +1318,25 @@ static inline int wave_launch_init(WaveConfig &c, hipStream_t s, DevWalker *W, c
 {
     hipLaunchKernelGGL(wave_init_kernel, dim3(n), dim3(64), 0, s, W, M, n);
     for (int r = 0; r < max_warmup; ++r)
-        hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, n, 1, r);
+        hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, 0, n, 1, r);
     hipLaunchKernelGGL(activate_pending_kernel, dim3((n + 63) / 64), dim3(64), 0, s, W, n);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
 
+// one sweep (phase mask) of walkers [w0, w1) on stream s
 static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, const DevModel *M,
-                                    DevPool *pool, int n, int phases, hipEvent_t *ev)
+                                    DevPool *pool, int w0, int w1, int phases, hipEvent_t *ev)
 {
+    const int n = w1 - w0;
     const int k1 = phases & (1 | 2 | 8);
     if (k1 & 3) {
-        hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, n, k1, -1);
+        hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, w0, w1, k1,
+                           -1);
     }
     if (ev)
         (void)hipEventRecord(ev[1], s);
     if (phases & 4)
-        hipLaunchKernelGGL(wave_worm_kernel, dim3(n), dim3(64), c.lds_k2, s, W, M, pool, n,
+        hipLaunchKernelGGL(wave_worm_kernel, dim3(n), dim3(64), c.lds_k2, s, W, M, pool, w0, w1,
                            (phases & 7) == 7 ? 1 : 0, (int)c.lds_k2, c.rec_mode);
     if (ev)
         (void)hipEventRecord(ev[2], s);
@@ -1323,10 +1345,10 @@ static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, 
 
 static inline int wave_launch_measure(WaveConfig &, hipStream_t s, DevWalker *W, const DevModel *M,
                                       double *out, double *sum, double *sumsq, int64_t *count,
-                                      uint8_t *, int, int n, int mode, int32_t *done)
+                                      int w0, int w1, int mode, int32_t *done)
 {
-    hipLaunchKernelGGL(wave_measure_kernel, dim3(n), dim3(64), 0, s, W, M, out, sum, sumsq, count, n,
-                       mode, done);
+    hipLaunchKernelGGL(wave_measure_kernel, dim3(w1 - w0), dim3(64), 0, s, W, M, out, sum, sumsq, count,
+                       w0, w1, mode, done);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
 

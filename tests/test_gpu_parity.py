@@ -287,6 +287,8 @@ def test_graph_replay_equals_eager(sse_pkg, oracle_mod, mode):
     _say(f"graph replay: mode={mode}")
     a = sse_pkg.Batch([flat], wd, mode=mode)
     b = sse_pkg.Batch([flat], wd, mode=mode)
+    for x in (a, b):
+        x.set_option(sse_pkg.capi.SSE_OPT_STREAMS, 1)
     a.set_option(sse_pkg.capi.SSE_OPT_TIMING, 0)      # graph path
     b.set_option(sse_pkg.capi.SSE_OPT_GRAPH, 0)       # eager path
     b.set_option(sse_pkg.capi.SSE_OPT_TIMING, 0)
@@ -311,5 +313,140 @@ def test_graph_replay_equals_eager(sse_pkg, oracle_mod, mode):
     o.sweep(280)
     sa, oa, pa = a.get_walker(0)
     assert np.array_equal(oa, o.ops()) and sa["rng_counter"] == o.scalars()["rng_counter"]
+    a.close()
+    b.close()
+
+
+def _eager(sse, batch):
+    """one stream, no graphs, no per-kernel events: plain kernel launches"""
+    batch.set_option(sse.capi.SSE_OPT_STREAMS, 1)
+    for opt in (sse.capi.SSE_OPT_GRAPH, sse.capi.SSE_OPT_TIMING):
+        batch.set_option(opt, 0)
+
+
+def _assert_same_walkers(a, b, n, tag):
+    for k in range(n):
+        sa, oa, pa = a.get_walker(k)
+        sb, ob, pb = b.get_walker(k)
+        assert np.array_equal(oa, ob), f"{tag} walker {k}: ops"
+        assert np.array_equal(pa, pb), f"{tag} walker {k}: spins"
+        for key in KEYS:
+            assert sa[key] == sb[key], f"{tag} walker {k}: {key}"
+
+
+@pytest.mark.skip(reason="persistent multi-sweep kernel disabled: its first GPU run ended in a "
+                  "memory fault whose cause is not found; the option is rejected by the library")
+def test_persistent_equals_per_sweep_kernels(sse_pkg, oracle_mod):
+    """Persistent multi-sweep launches (several chunks per call, fused
+    measurement, calls without measurement) give the per-sweep kernels'
+    result bit for bit, on a batch mixing two models."""
+    sq = square_heisenberg(sse_pkg, 6).flat()
+    hc = job_honeycomb(sse_pkg, 0.5).flat()
+    wd = [sse_pkg.walker_desc(model=s % 2, T=0.3 + 0.05 * (s % 3), seed=300 + s, thermalization=12)
+          for s in range(10)]
+    _say("persistent vs per-sweep kernels: 10 walkers, 2 models")
+    a = sse_pkg.Batch([sq, hc], wd)
+    b = sse_pkg.Batch([sq, hc], wd)
+    _eager(sse_pkg, b)
+    a.init()
+    b.init()
+    for rep, (k, meas) in enumerate(((37, True), (9, False), (70, True), (2, True))):
+        a.run(k, measure=meas)
+        b.run(k, measure=meas)
+        ms, sw = a.call_time()
+        assert sw == k and ms > 0.0
+        _assert_same_walkers(a, b, len(wd), f"call {rep}")
+    for (s1, q1, c1), (s2, q2, c2) in zip(a.accumulators(), b.accumulators()):
+        assert c1 == c2 and np.array_equal(s1, s2) and np.array_equal(q1, q2)
+    # one walker of each model against the oracle
+    for w in (0, 1):
+        o = oracle_mod.OracleWalker(sse_pkg.capi, [sq, hc][w], sse_pkg.walker_desc(
+            T=wd[w].T, seed=wd[w].seed, thermalization=12))
+        o.init()
+        o.sweep(37 + 9 + 70 + 2)
+        gs, gops, gspins = a.get_walker(w)
+        os_, oops, ospins = o.state()
+        assert np.array_equal(gops, oops) and np.array_equal(gspins, ospins)
+        for key in KEYS:
+            assert gs[key] == os_[key]
+    a.close()
+    b.close()
+
+
+@pytest.mark.skip(reason="persistent multi-sweep kernel disabled: its first GPU run ended in a "
+                  "memory fault whose cause is not found; the option is rejected by the library")
+def test_persistent_many_walkers_waits(sse_pkg):
+    """More walkers than fit at once and tiny sweeps, so workers often take a
+    walker's next sweep while its previous one still runs and must wait."""
+    flat = square_heisenberg(sse_pkg, 4).flat()
+    wd = [sse_pkg.walker_desc(T=0.5, seed=900 + s, thermalization=3) for s in range(3000)]
+    _say("persistent: 3000 walkers x 12 sweeps vs per-sweep kernels")
+    a = sse_pkg.Batch([flat], wd)
+    b = sse_pkg.Batch([flat], wd)
+    _eager(sse_pkg, b)
+    a.init()
+    b.init()
+    a.run(12, measure=True)
+    b.run(12, measure=True)
+    _assert_same_walkers(a, b, len(wd), "3000 walkers")
+    for (s1, q1, c1), (s2, q2, c2) in zip(a.accumulators(), b.accumulators()):
+        assert c1 == c2 and np.array_equal(s1, s2) and np.array_equal(q1, q2)
+    a.close()
+    b.close()
+
+
+def test_walker_groups_equal_single_stream(sse_pkg, oracle_mod):
+    """Walker groups on separate streams (default) give the single-stream
+    result bit for bit: calls of several lengths with and without the fused
+    measurement, a batch mixing two models, and the oracle for two walkers."""
+    sq = square_heisenberg(sse_pkg, 6).flat()
+    hc = job_honeycomb(sse_pkg, 0.5).flat()
+    wd = [sse_pkg.walker_desc(model=s % 2, T=0.3 + 0.05 * (s % 3), seed=300 + s, thermalization=12)
+          for s in range(10)]
+    _say("walker groups vs one stream: 10 walkers, 2 models")
+    a = sse_pkg.Batch([sq, hc], wd)
+    b = sse_pkg.Batch([sq, hc], wd)
+    a.set_option(sse_pkg.capi.SSE_OPT_STREAMS, 3)
+    _eager(sse_pkg, b)
+    a.init()
+    b.init()
+    for rep, (k, meas) in enumerate(((37, True), (9, False), (70, True), (1, True))):
+        a.run(k, measure=meas)
+        b.run(k, measure=meas)
+        ms, sw = a.call_time()
+        assert sw == k and ms > 0.0
+        _assert_same_walkers(a, b, len(wd), f"call {rep}")
+    for (s1, q1, c1), (s2, q2, c2) in zip(a.accumulators(), b.accumulators()):
+        assert c1 == c2 and np.array_equal(s1, s2) and np.array_equal(q1, q2)
+    for w in (0, 1):
+        o = oracle_mod.OracleWalker(sse_pkg.capi, [sq, hc][w], sse_pkg.walker_desc(
+            T=wd[w].T, seed=wd[w].seed, thermalization=12))
+        o.init()
+        o.sweep(37 + 9 + 70 + 1)
+        gs, gops, gspins = a.get_walker(w)
+        os_, oops, ospins = o.state()
+        assert np.array_equal(gops, oops) and np.array_equal(gspins, ospins)
+        for key in KEYS:
+            assert gs[key] == os_[key]
+    a.close()
+    b.close()
+
+
+def test_walker_groups_many_walkers(sse_pkg):
+    """3000 walkers in 7 groups against one stream, bit for bit."""
+    flat = square_heisenberg(sse_pkg, 4).flat()
+    wd = [sse_pkg.walker_desc(T=0.5, seed=900 + s, thermalization=3) for s in range(3000)]
+    _say("walker groups: 3000 walkers x 12 sweeps, 7 groups vs one stream")
+    a = sse_pkg.Batch([flat], wd)
+    b = sse_pkg.Batch([flat], wd)
+    a.set_option(sse_pkg.capi.SSE_OPT_STREAMS, 7)
+    _eager(sse_pkg, b)
+    a.init()
+    b.init()
+    a.run(12, measure=True)
+    b.run(12, measure=True)
+    _assert_same_walkers(a, b, len(wd), "3000 walkers")
+    for (s1, q1, c1), (s2, q2, c2) in zip(a.accumulators(), b.accumulators()):
+        assert c1 == c2 and np.array_equal(s1, s2) and np.array_equal(q1, q2)
     a.close()
     b.close()
