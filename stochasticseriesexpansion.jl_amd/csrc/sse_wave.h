@@ -90,6 +90,13 @@ struct WaveStreamT {
             wave_refill<NW>(buf, key, bbase);
         }
     }
+    // the buffer is used up (off == NW): load the next NW words
+    __device__ __forceinline__ void refill_next()
+    {
+        bbase += off;
+        off = 0;
+        wave_refill<NW>(buf, key, bbase);
+    }
     // word k positions after the next one (after ensure)
     __device__ __forceinline__ uint64_t peek(uint32_t k) const { return buf[off + k]; }
     __device__ __forceinline__ void advance(uint32_t k) { off += k; }
@@ -765,10 +772,14 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
         const uint64_t keyB = ((uint64_t)lid0 << 14) | (wf0 - 1u);
         uint32_t lid = lid0, wfm = wf0 - 1u;
         const uint32_t sc0 = sc;
-        uint32_t endB;
+        uint32_t endB = 0, err = 0;
         for (;;) {  // sse.jl:274-300
-            if (rs.off >= 128u)
-                rs.ensure(1);
+            if (rs.off >= 128u) {
+                if (rs.off == 128u)
+                    rs.refill_next();
+                else
+                    rs.ensure(1);
+            }
             const uint32_t off = uni(rs.off);
             const uint64_t wv = RB[off];
             const uint32_t r = lid >> 2;
@@ -782,40 +793,51 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
             uint32_t eo = chase_select((uint32_t)mm, (uint32_t)(mm >> 32), E, ft);
             sc += 1;
             if (ft) {
+                // more than four exits: walk the exit list (vertex_data.jl:113-125);
+                // a fall-through or an invalid transition is an error
                 const uint32_t cnt = uni(E->count);
-                if (cnt <= 4u)
-                    return SSE_ERR_SCATTER;  // fall-through (vertex_data.jl:124) or invalid
-                const uint32_t trw = uni(E->trans);
-                const WormTab tab = worm_tab((const char *)m.wtab, m);
-                const double xr = word_to_float(((uint64_t)whi << 32) | wlo);
-                uint32_t ex = trw & 0xFFFFFu;
-                const uint32_t eend = ex + (trw >> 20);
-                for (;;) {
-                    if (ex >= eend)
-                        return SSE_ERR_SCATTER;
-                    const uint64_t cb = uni64((uint64_t)__double_as_longlong(tab.cum[ex]));
-                    if (xr < __longlong_as_double((long long)cb))
-                        break;
-                    ++ex;
+                eo = e << 16;  // rewrites the unchanged code if the scatter fails
+                if (cnt <= 4u) {
+                    err = SSE_ERR_SCATTER;
+                } else {
+                    const uint32_t trw = uni(E->trans);
+                    const WormTab tab = worm_tab((const char *)m.wtab, m);
+                    const double xr = word_to_float(((uint64_t)whi << 32) | wlo);
+                    uint32_t ex = trw & 0xFFFFFu;
+                    const uint32_t eend = ex + (trw >> 20);
+                    for (;;) {
+                        if (ex >= eend) {
+                            err = SSE_ERR_SCATTER;
+                            break;
+                        }
+                        const uint64_t cb = uni64((uint64_t)__double_as_longlong(tab.cum[ex]));
+                        if (xr < __longlong_as_double((long long)cb)) {
+                            const uint32_t vc = uni(tab.eout[ex]);
+                            eo = chase_out((e >> lgp << lgp) | (vc & 1023u), (vc >> 10) & 3u, vc >> 12);
+                            break;
+                        }
+                        ++ex;
+                    }
                 }
-                const uint32_t vc = uni(tab.eout[ex]);
-                eo = chase_out((e >> lgp << lgp) | (vc & 1023u), (vc >> 10) & 3u, vc >> 12);
             }
             rec.put_ecode(r, eo >> 16);
-            if ((((uint64_t)(lid & ~3u) << 14) | (eo & 0xFFFFu)) == keyA) {
-                endB = 0;
-                break;
-            }
+            // closing tests (sse.jl:290-297) folded into one loop exit: the worm
+            // ends leaving through (r0, l0, dim0 - wf0), or entering (r0, l0, wf0)
             const uint32_t nx = R::link(x, (eo >> 14) & 3u);
-            wfm = eo & 0x3FFFu;
-            if (R::kChecked && (nx >> 2) >= n)
-                return SSE_ERR_STATE;
-            if ((((uint64_t)nx << 14) | wfm) == keyB) {
-                endB = 1;
+            const uint32_t wfn = eo & 0x3FFFu;
+            const bool endA = ((((uint64_t)(lid & ~3u) << 14) | (eo & 0xFFFFu)) == keyA);
+            const bool hitB = ((((uint64_t)nx << 14) | wfn) == keyB);
+            if (R::kChecked && !endA && (nx >> 2) >= n)
+                err = SSE_ERR_STATE;
+            if (endA || hitB || err) {
+                endB = !endA && hitB;
                 break;
             }
             lid = nx;
+            wfm = wfn;
         }
+        if (err)
+            return (int)err;
         // worm length: scatters + 1 when the worm closed on re-entry
         total += (double)(sc - sc0 + endB);
     }
@@ -974,8 +996,8 @@ __device__ bool k2_stage_global(const DevWalker &w, const DevModel &m)
     return __ballot(bad) == 0;
 }
 
-// Worm update of walker wi.  LDS layout  rbuf u64[128] | mask u64[nch] |
-// base u32[nch] | records.  RecG reads links through the scalar cache, which is
+// Worm update of walker wi.  LDS layout  rbuf u64[128] | records |
+// mask u64[nch] | base u32[nch].  RecG reads links through the scalar cache, which is
 // valid because K1 wrote them in an earlier kernel launch.
 __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
                                           int full_sweep, int lds_bytes, int rec_mode, char *lds)
@@ -1001,14 +1023,17 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
     int64_t scat = 0;
     int err = 0;
     if (fmt) {
+        // records first, at a fixed offset, so the chase addresses them with
+        // the offset folded into the LDS instruction
         char *q = lds + 1024;
-        uint64_t *lmask = (uint64_t *)q;
-        q += nch * 8;
-        uint32_t *lbase = (uint32_t *)q;
-        q += (nch * 4 + 15) & ~(int64_t)15;
         lds_u64 *llk = (lds_u64 *)(uint64_t *)q;
         q += w.n * 8;
         lds_u16 *lec = (lds_u16 *)(uint16_t *)q;
+        if (fmt == 10)
+            q += (w.n * 2 + 15) & ~(int64_t)15;
+        uint64_t *lmask = (uint64_t *)q;
+        q += nch * 8;
+        uint32_t *lbase = (uint32_t *)q;
         const uint64_t *gmask = rec_mask(w);
         const uint32_t *gbase = rec_base(w);
         for (int64_t i = lane; i < nch; i += 64) {
@@ -1138,6 +1163,112 @@ __device__ __forceinline__ double wave_incl_scan(double x)
 
 // mode 0: explicit measure of every walker; mode 1: thermalized walkers only,
 // accumulated into the per-walker sums (fused sweep+measure loop).
+// One pass over the operator string: the sign (sse.jl:305-314) when
+// want_sign, and estimators e0 .. e0+ne-1 (ne <= K3_MAXE) of the
+// MagnetizationEstimator family (magnetization_estimator.jl:96-230), whose
+// running sums over the operator positions are wave prefix sums.  Writes the
+// five outputs of each of its estimators.
+constexpr int K3_MAXE = 4;
+__device__ __forceinline__ void k3_pass(const DevWalker &w, const DevModel &m, int e0, int ne,
+                                        bool want_sign, double &sign, double *o)
+{
+    const int lane = threadIdx.x;
+    const int64_t L = w.L;
+    const int N = m.n_sites, D = m.est_max_dim;
+    const double *coef[K3_MAXE], *mv[K3_MAXE];
+    double tmp0[K3_MAXE], run[K3_MAXE], mag[K3_MAXE], absmag[K3_MAXE], mag2[K3_MAXE], mag4[K3_MAXE];
+#pragma unroll
+    for (int k = 0; k < K3_MAXE; ++k) {
+        coef[k] = m.est_coef + (size_t)(e0 + k) * N;
+        mv[k] = m.est_mval + (size_t)(e0 + k) * N * D;
+        double t0 = 0.0;
+        if (k < ne)
+            for (int i = lane; i < N; i += 64)
+                t0 += coef[k][i] * mv[k][i * D + w.spins[i] - 1];
+        tmp0[k] = k < ne ? wave_sum(t0) : 0.0;
+        run[k] = tmp0[k];
+        mag[k] = absmag[k] = mag2[k] = mag4[k] = 0.0;
+    }
+    int neg = 0;
+    int64_t cnt = 0;
+    for (int64_t base = 0; base < L; base += 64) {
+        const int64_t p = base + lane;
+        const uint32_t op = p < L ? w.ops[p] : 0u;
+        const uint32_t vc = op_vcode(op);
+        int2 br = make_int2(0, 0);
+        const DevVD *vd = m.types;
+        if (op != 0u) {
+            br = m.bond_rec[op_bond(op)];
+            vd = &m.types[br.x >> 20];
+        }
+        if (want_sign)
+            neg += popc(__ballot(op != 0u && vd->signs[vc >> 1] < 0));
+        cnt += popc(__ballot(op != 0u));
+        const bool offd = op != 0u && !(vc & 1u);
+        uint32_t l0 = 0, l1 = 0, l2 = 0, l3 = 0;
+        if (offd) {
+            const uint8_t *ls = vd->leg_states + 4 * (vc >> 1);
+            l0 = ls[0];
+            l1 = ls[1];
+            l2 = ls[2];
+            l3 = ls[3];
+        }
+        const int sa = br.x & 0xFFFFF, sb = br.y;
+#pragma unroll
+        for (int k = 0; k < K3_MAXE; ++k) {
+            if (k < ne) {
+                double delta = 0.0;
+                if (offd) {
+                    delta = coef[k][sa] * (mv[k][sa * D + l2 - 1] - mv[k][sa * D + l0 - 1]);
+                    delta += coef[k][sb] * (mv[k][sb * D + l3 - 1] - mv[k][sb * D + l1 - 1]);
+                }
+                const double inc = wave_incl_scan(delta);
+                if (op != 0u) {
+                    const double t = run[k] + inc;
+                    const double t2 = t * t;
+                    mag[k] += t;
+                    absmag[k] += fabs(t);
+                    mag2[k] += t2;
+                    mag4[k] += t2 * t2;
+                }
+                run[k] += __shfl(inc, 63);
+            }
+        }
+    }
+    if (want_sign)
+        sign = (neg & 1) ? -1.0 : 1.0;
+    const double Nn = m.norm;
+    const double en = 1.0 + (double)cnt;
+    const double norm = 1.0 / Nn;
+#pragma unroll
+    for (int k = 0; k < K3_MAXE; ++k) {
+        if (k < ne) {
+            double smag = tmp0[k] + wave_sum(mag[k]);
+            double sabs = fabs(tmp0[k]) + wave_sum(absmag[k]);
+            double s2 = tmp0[k] * tmp0[k];
+            double smag2 = s2 + wave_sum(mag2[k]);
+            double smag4 = s2 * s2 + wave_sum(mag4[k]);
+            smag *= norm;
+            sabs *= norm;
+            smag2 *= norm * norm;
+            smag4 *= (norm * norm) * (norm * norm);
+            if (lane == 0) {
+                double *oe = o + 5 + 5 * (e0 + k);
+                oe[0] = sign * smag / en;
+                oe[1] = sign * sabs / en;
+                oe[2] = sign * smag2 / en;
+                oe[3] = sign * smag4 / en;
+                const double chi = 1.0 / w.T / (en + 1.0) / en * (smag * smag + smag2) * Nn;
+                oe[4] = sign * chi;
+            }
+        }
+    }
+}
+
+// measure (sse.jl:70-87, 305-376) of walker wi.  mode 0: explicit measure;
+// mode 1: thermalized walkers only, accumulated into the per-walker sums
+// (fused sweep+measure loop).  Estimators are taken K3_MAXE per pass over the
+// operator string; the first pass also computes the sign.
 __device__ __forceinline__ void k3_walker(DevWalker *W, const DevModel *models, double *out, double *sum,
                                           double *sumsq, int64_t *count, int wi, int mode, int32_t *done)
 {
@@ -1154,83 +1285,21 @@ __device__ __forceinline__ void k3_walker(DevWalker *W, const DevModel *models, 
         return;
     }
     const DevModel &m = models[w.model];
-    const int64_t L = w.L;
-    // sign (sse.jl:305-314)
-    int neg = 0;
-    for (int64_t base = 0; base < L; base += 64) {
-        const int64_t p = base + lane;
-        const uint32_t op = p < L ? w.ops[p] : 0u;
-        bool ng = false;
-        if (op != 0u)
-            ng = m.types[m.bond_rec[op_bond(op)].x >> 20].signs[op_vcode(op) >> 1] < 0;
-        neg += popc(__ballot(ng));
-    }
-    const double sign = (neg & 1) ? -1.0 : 1.0;
-    const double nn = (double)w.n;
-    const double Nn = m.norm;
     double *o = out + w.meas_offset;
+    double sign = 1.0;
+    int e0 = 0;
+    do {
+        const int ne = m.n_est - e0 < K3_MAXE ? m.n_est - e0 : K3_MAXE;
+        k3_pass(w, m, e0, ne, e0 == 0, sign, o);
+        e0 += K3_MAXE;
+    } while (e0 < m.n_est);
+    const double nn = (double)w.n;
     if (lane == 0) {
         o[0] = sign;
         o[1] = nn;
         o[2] = sign * nn;
         o[3] = sign * (nn * nn);
-        o[4] = ((-sign) * (nn * w.T + m.energy_offset)) / Nn;
-    }
-    const int N = m.n_sites, D = m.est_max_dim;
-    for (int e = 0; e < m.n_est; ++e) {
-        const double *coef = m.est_coef + (size_t)e * N;
-        const double *mv = m.est_mval + (size_t)e * N * D;
-        double t0 = 0.0;
-        for (int i = lane; i < N; i += 64)
-            t0 += coef[i] * mv[i * D + w.spins[i] - 1];
-        const double tmp0 = wave_sum(t0);
-        double run = tmp0;
-        double mag = 0.0, absmag = 0.0, mag2 = 0.0, mag4 = 0.0;
-        int64_t cnt = 0;
-        for (int64_t base = 0; base < L; base += 64) {
-            const int64_t p = base + lane;
-            const uint32_t op = p < L ? w.ops[p] : 0u;
-            double delta = 0.0;
-            const uint32_t vc = op_vcode(op);
-            if (op != 0u && !(vc & 1u)) {
-                const int2 br = m.bond_rec[op_bond(op)];
-                const uint8_t *ls = m.types[br.x >> 20].leg_states + 4 * (vc >> 1);
-                const int sa = br.x & 0xFFFFF, sb = br.y;
-                delta = coef[sa] * (mv[sa * D + ls[2] - 1] - mv[sa * D + ls[0] - 1]);
-                delta += coef[sb] * (mv[sb * D + ls[3] - 1] - mv[sb * D + ls[1] - 1]);
-            }
-            const double inc = wave_incl_scan(delta);
-            if (op != 0u) {
-                const double t = run + inc;
-                const double t2 = t * t;
-                mag += t;
-                absmag += fabs(t);
-                mag2 += t2;
-                mag4 += t2 * t2;
-            }
-            cnt += popc(__ballot(op != 0u));
-            run += __shfl(inc, 63);
-        }
-        double smag = tmp0 + wave_sum(mag);
-        double sabs = fabs(tmp0) + wave_sum(absmag);
-        double s2 = tmp0 * tmp0;
-        double smag2 = s2 + wave_sum(mag2);
-        double smag4 = s2 * s2 + wave_sum(mag4);
-        const double en = 1.0 + (double)cnt;
-        const double norm = 1.0 / Nn;
-        smag *= norm;
-        sabs *= norm;
-        smag2 *= norm * norm;
-        smag4 *= (norm * norm) * (norm * norm);
-        if (lane == 0) {
-            double *oe = o + 5 + 5 * e;
-            oe[0] = sign * smag / en;
-            oe[1] = sign * sabs / en;
-            oe[2] = sign * smag2 / en;
-            oe[3] = sign * smag4 / en;
-            const double chi = 1.0 / w.T / (en + 1.0) / en * (smag * smag + smag2) * Nn;
-            oe[4] = sign * chi;
-        }
+        o[4] = ((-sign) * (nn * w.T + m.energy_offset)) / m.norm;
     }
     if (mode == 1) {
         __syncthreads();
