@@ -320,6 +320,31 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
         }
     }
     size_t o_dtab = put(dt.data(), sizeof(DirectEntry) * dt.size());
+    // diagonal-update table (layout: DevModel::k1tab)
+    std::vector<char> k1((size_t)16 * m.n_types);
+    for (int t = 0; t < m.n_types; ++t) {
+        const sse_vertex_data_desc &vd = m.types[t];
+        auto align8 = [&]() { k1.resize((k1.size() + 7) & ~(size_t)7); };
+        align8();
+        const uint32_t w_off = (uint32_t)k1.size();
+        k1.resize(k1.size() + 8 * (size_t)vd.n_vertices);
+        memcpy(k1.data() + w_off, vd.weights, 8 * (size_t)vd.n_vertices);
+        const uint32_t dv_off = (uint32_t)k1.size();
+        const int ndv = vd.dims[0] * vd.dims[1];
+        k1.resize(k1.size() + 2 * (size_t)ndv);
+        for (int i = 0; i < ndv; ++i) {
+            const int16_t c = (int16_t)vd.diagonal_vertices[i];
+            memcpy(k1.data() + dv_off + 2 * i, &c, 2);
+        }
+        const uint32_t ls_off = (uint32_t)k1.size();
+        k1.resize(k1.size() + 4 * (size_t)vd.n_vertices);
+        memcpy(k1.data() + ls_off, vd.leg_states, 4 * (size_t)vd.n_vertices);
+        const uint32_t h[4] = {w_off, dv_off, ls_off,
+                               (uint32_t)vd.n_vertices | ((uint32_t)vd.dims[0] << 16)};
+        memcpy(k1.data() + 16 * (size_t)t, h, 16);
+    }
+    k1.resize((k1.size() + 15) & ~(size_t)15);
+    size_t o_k1 = put(k1.data(), k1.size());
 
     char *d = nullptr;
     int rc = dev_alloc(b, &d, blob.size() + 16);
@@ -374,6 +399,9 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
     dm.dt_mw = dmw;
     dm.dt_lgp = lgp;
     dm.dtab_bytes = (int32_t)(sizeof(DirectEntry) * dt.size());
+    dm.k1tab = (const uint8_t *)(d + o_k1);
+    dm.k1tab_bytes = (int32_t)k1.size();
+    dm.pad_k1 = 0;
     return SSE_OK;
 }
 

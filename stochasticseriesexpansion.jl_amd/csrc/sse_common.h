@@ -62,6 +62,12 @@ struct DevModel {
     // (2^dt_lgp >= 2*dt_nv); up to 4 exits inline, more are looked up in wtab
     const struct DirectEntry *dtab;
     int32_t dt_nv, dt_mw, dtab_bytes, dt_lgp;
+    // diagonal-update table, copied to LDS by K1 next to the bond records:
+    //   per type a 16-byte header {weights off, diag-vertex off, leg-state off,
+    //   nv | dims0 << 16}, then f64 weights[nv], i16 diagonal vertex codes
+    //   [dims0*dims1] (or -1), u8 leg states [nv][4]; offsets from the table start
+    const uint8_t *k1tab;
+    int32_t k1tab_bytes, pad_k1;
 };
 
 // Exit j is taken by the first j with r < cum[j], r = (word >> 11) * 2^-53.
@@ -93,6 +99,22 @@ __host__ __device__ inline uint32_t ecode_of_vt(uint32_t vt, uint32_t lgp)
 __host__ __device__ inline uint32_t vt_of_ecode(uint32_t e, uint32_t lgp)
 {
     return (e & ((1u << lgp) - 1u)) | ((e >> lgp) << 10);
+}
+
+// Pointers known to address global memory.  Loads and stores through them are
+// global (not flat) instructions, which count only against the vector-memory
+// counter: an LDS or scalar-load wait then does not also wait for them.
+template <class T>
+using gmem = __attribute__((address_space(1))) T;
+template <class T>
+__device__ __forceinline__ gmem<T> *as_global(T *p)
+{
+    return (gmem<T> *)p;
+}
+template <class T>
+__device__ __forceinline__ const gmem<T> *as_global(const T *p)
+{
+    return (const gmem<T> *)p;
 }
 
 struct DevWalker {

@@ -146,6 +146,68 @@ __host__ __device__ inline size_t k1_lds_bytes(int N)
     return (b + 15) & ~(size_t)15;
 }
 
+// Orders the LDS accesses of the lanes of a one-wave workgroup.  The LDS runs
+// a wave's instructions in order, so only code motion must be prevented; unlike
+// __syncthreads this does not wait for global loads still in flight.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Model lookups of the diagonal update / vertex list: from the LDS copies of
+// the bond records and of DevModel::k1tab (K1Staged), or from global memory.
+struct K1Global {
+    const DevModel *m;
+    __device__ __forceinline__ void stage(const DevModel &) const {}
+    __device__ __forceinline__ int2 bond(uint32_t b) const { return m->bond_rec[b]; }
+    __device__ __forceinline__ int nv(int t) const { return m->types[t].nv; }
+    __device__ __forceinline__ int dims0(int t) const { return m->types[t].dims0; }
+    __device__ __forceinline__ int diag(int t, int i) const { return m->types[t].diag_vertex[i]; }
+    __device__ __forceinline__ double weight(int t, int v) const { return m->types[t].weights[v]; }
+    __device__ __forceinline__ uint32_t upper(int t, int v) const
+    {
+        const uint8_t *ls = m->types[t].leg_states + 4 * v;
+        return ls[2] | ((uint32_t)ls[3] << 8);
+    }
+};
+struct K1Staged {
+    const int2 *br;
+    const uint8_t *kt;
+    // copy the bond records and the model's k1tab into LDS (whole wave)
+    __device__ __forceinline__ void stage(const DevModel &m) const
+    {
+        int2 *b = (int2 *)br;
+        for (int i = threadIdx.x; i < m.n_bonds; i += 64)
+            b[i] = m.bond_rec[i];
+        uint32_t *k = (uint32_t *)kt;
+        const uint32_t *g = (const uint32_t *)m.k1tab;
+        for (int i = threadIdx.x; i < m.k1tab_bytes / 4; i += 64)
+            k[i] = g[i];
+    }
+    __device__ __forceinline__ int2 bond(uint32_t b) const { return br[b]; }
+    __device__ __forceinline__ uint4 hdr(int t) const { return ((const uint4 *)kt)[t]; }
+    __device__ __forceinline__ int nv(int t) const { return (int)(hdr(t).w & 0xFFFFu); }
+    __device__ __forceinline__ int dims0(int t) const { return (int)(hdr(t).w >> 16); }
+    __device__ __forceinline__ int diag(int t, int i) const { return ((const int16_t *)(kt + hdr(t).y))[i]; }
+    __device__ __forceinline__ double weight(int t, int v) const
+    {
+        return ((const double *)(kt + hdr(t).x))[v];
+    }
+    __device__ __forceinline__ uint32_t upper(int t, int v) const
+    {
+        const uint8_t *ls = kt + hdr(t).z + 4 * v;
+        return ls[2] | ((uint32_t)ls[3] << 8);
+    }
+};
+
+// LDS bytes of K1 with the model's lookups staged: the base layout, then the
+// bond records and the k1tab of the model
+__host__ __device__ inline size_t k1_lds_staged(int N, int n_bonds, int k1tab_bytes)
+{
+    return k1_lds_bytes(N) + ((8 * (size_t)n_bonds + 15) & ~(size_t)15) + (size_t)k1tab_bytes;
+}
+
 // Slot-by-slot processing of one chunk (reference order).  Used when a bond
 // draw of the chunk hits a Lemire rejection, and when forced for validation.
 // Emits the same compact records as the parallel path.
@@ -238,8 +300,9 @@ __device__ void k1_serial_chunk(DevWalker &w, const DevModel &m, WaveStream &rs,
 // phases: 1 diagonal update, 2 vertex list, 8 force the slot-by-slot path.
 // warm >= 0: init warm-up round (diagonal only; walkers with fewer warm-up
 // sweeps are skipped).
-__device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
-                                          int phases, int warm, char *lds)
+template <class Tab>
+__device__ __forceinline__ void k1_walker_t(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
+                                            int phases, int warm, char *lds, Tab tab)
 {
     const int lane = threadIdx.x;
     DevWalker w = W[wi];
@@ -248,6 +311,7 @@ __device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, 
     if (warm >= 0 ? (warm >= w.warmup || !w.pending) : !w.active)
         return;
     const DevModel &m = models[w.model];
+    tab.stage(m);
     const int N = m.n_sites;
     uint64_t *rbuf = (uint64_t *)lds;
     uint64_t *tm0 = (uint64_t *)(lds + 2048);
@@ -309,15 +373,19 @@ __device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, 
     int64_t n0 = w.n;
     int64_t nr = 0;  // operators (ranks) emitted so far
     const uint64_t below = below_mask(lane), above = above_mask(lane);
-    uint32_t *gvt = rec_vt(w);
-    uint32_t *glk = rec_link(w);
-    uint64_t *gmask = rec_mask(w);
-    uint32_t *gbase = rec_base(w);
+    gmem<uint32_t> *gvt = as_global(rec_vt(w));
+    gmem<uint32_t> *glk = as_global(rec_link(w));
+    gmem<uint64_t> *gmask = as_global(rec_mask(w));
+    gmem<uint32_t> *gbase = as_global(rec_base(w));
+    gmem<uint32_t> *ops = as_global(w.ops);
 
+    // the next chunk's operators are loaded while this chunk is processed
+    uint32_t op_next = (int64_t)lane < L ? ops[lane] : 0u;
     for (int64_t base = 0; base < L; base += 64) {
         const int64_t p = base + lane;
         const bool valid = p < L;
-        const uint32_t op = valid ? w.ops[p] : 0u;
+        const uint32_t op = op_next;
+        op_next = p + 64 < L ? ops[p + 64] : 0u;
         const uint32_t vc = op_vcode(op);
         const bool ident = valid && op == 0u;
         const bool isop = valid && op != 0u;
@@ -331,7 +399,7 @@ __device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, 
             if (isop) {
                 bad = b >= (uint32_t)m.n_bonds;
                 if (!bad)
-                    bad = (int)(vc >> 1) >= m.types[m.bond_rec[b].x >> 20].nv;
+                    bad = (int)(vc >> 1) >= tab.nv(tab.bond(b).x >> 20);
             }
             if (__ballot(bad)) {
                 if (lane == 0) {
@@ -374,7 +442,7 @@ __device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, 
         const bool has = isop || (ident && do_diag);
         int s0 = 0, s1 = 0, type = 0;
         if (has) {
-            int2 br = m.bond_rec[b];
+            int2 br = tab.bond(b);
             s0 = br.x & 0xFFFFF;
             s1 = br.y;
             type = br.x >> 20;
@@ -385,10 +453,9 @@ __device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, 
             atomicOr(&tm1w[2 * s1 + (lane >> 5)], 1u << (lane & 31));
         }
         if (offd) {
-            const uint8_t *ls = m.types[type].leg_states + 4 * (vc >> 1);
-            cups[lane] = (uint16_t)(ls[2] | (ls[3] << 8));
+            cups[lane] = (uint16_t)tab.upper(type, vc >> 1);
         }
-        __syncthreads();
+        wave_lds_sync();
         uint64_t a0 = 0, a1 = 0, c0 = 0, c1 = 0;  // site0: pos-0/pos-1 masks; site1: same
         if (has) {
             a0 = tm0[s0];
@@ -418,11 +485,10 @@ __device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, 
                 } else {
                     v1 = sp[s1];
                 }
-                const DevVD &vd = m.types[type];
-                dv = vd.diag_vertex[(v0 - 1) + vd.dims0 * (v1 - 1)];
-                wgt = dv < 0 ? 0.0 : vd.weights[dv >> 1];
+                dv = tab.diag(type, (v0 - 1) + tab.dims0(type) * (v1 - 1));
+                wgt = dv < 0 ? 0.0 : tab.weight(type, dv >> 1);
             } else if (dia) {
-                wgt = m.types[type].weights[vc >> 1];
+                wgt = tab.weight(type, vc >> 1);
             }
             bool amb = false, dec = false;
             if (ident) {
@@ -468,7 +534,7 @@ __device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, 
             if (dia && ((minus >> lane) & 1ull))
                 fop = 0u;
             if (valid && fop != op)
-                w.ops[p] = fop;
+                ops[p] = fop;
         }
 
         // vertex list over the final non-identity operators of the chunk, in
@@ -501,7 +567,7 @@ __device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, 
             last0 = (m0 & above) == 0ull;
             last1 = (m1 & above) == 0ull;
         }
-        __syncthreads();
+        wave_lds_sync();
         if (do_vl && isni) {
             const int32_t me0 = 4 * rank, me1 = 4 * rank + 1;
             if (pred0 >= 0) {
@@ -538,7 +604,7 @@ __device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, 
             tm0[s0] = 0ull;
             tm1[s1] = 0ull;
         }
-        __syncthreads();
+        wave_lds_sync();
     }
 
     // periodic closure (vertex_list.jl:46-51) and write-back
@@ -571,16 +637,33 @@ __device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, 
     }
 }
 
+// K1 of walker wi; the model's lookups come from LDS when the launch's LDS
+// (lds_bytes) holds them
+__device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
+                                          int phases, int warm, char *lds, int lds_bytes)
+{
+    const DevModel &m = models[W[wi].model];
+    const size_t base = k1_lds_bytes(m.n_sites);
+    if (k1_lds_staged(m.n_sites, m.n_bonds, m.k1tab_bytes) <= (size_t)lds_bytes) {
+        K1Staged tab{(const int2 *)(lds + base),
+                     (const uint8_t *)(lds + base + ((8 * (size_t)m.n_bonds + 15) & ~(size_t)15))};
+        k1_walker_t(W, models, pool, wi, phases, warm, lds, tab);
+    } else {
+        K1Global tab{&m};
+        k1_walker_t(W, models, pool, wi, phases, warm, lds, tab);
+    }
+}
+
 // kernels of the per-sweep path take a walker range [w_begin, w_end), one
 // workgroup (one wave) per walker
 __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const DevModel *models,
                                                              DevPool *pool, int w_begin, int w_end,
-                                                             int phases, int warm)
+                                                             int phases, int warm, int lds_bytes)
 {
     extern __shared__ __align__(16) char lds[];
     const int wi = w_begin + (int)blockIdx.x;
     if (wi < w_end)
-        k1_walker(W, models, pool, wi, phases, warm, lds);
+        k1_walker(W, models, pool, wi, phases, warm, lds, lds_bytes);
 }
 
 // ------------------------------------------------------------------ K2
@@ -896,6 +979,7 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &re
     // write the worm-updated vertex codes back into the operator string, four
     // 64-slot chunks per pass so their loads are in flight together
     const int64_t L = w.L;
+    gmem<uint32_t> *ops = as_global(w.ops);
     for (int64_t base = 0; base < L; base += 256) {
         uint32_t op[4], r[4];
         bool act[4];
@@ -911,7 +995,7 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &re
                 if ((mk >> lane) & 1ull) {
                     act[u] = true;
                     r[u] = BASE[c] + (uint32_t)popc(mk & below_mask(lane));
-                    op[u] = w.ops[p];
+                    op[u] = ops[p];
                 }
             }
         }
@@ -921,7 +1005,7 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &re
                 const uint32_t vc = vt_of_ecode(rec.get_ecode(r[u]), lgp) & 1023u;
                 const uint32_t nop = (op[u] & ~(1023u << 1)) | (vc << 1);
                 if (nop != op[u])
-                    w.ops[base + 64 * u + lane] = nop;
+                    ops[base + 64 * u + lane] = nop;
             }
         }
     }
@@ -945,8 +1029,8 @@ __device__ bool k2_stage(const DevWalker &w, const DevModel &m, lds_u64 *lk, lds
 {
     const int lane = threadIdx.x;
     const uint32_t n = (uint32_t)w.n, lgp = (uint32_t)m.dt_lgp, nt = (uint32_t)m.n_types;
-    const uint32_t *gvt = rec_vt(w);
-    const uint4 *glk = (const uint4 *)rec_link(w);
+    const gmem<uint32_t> *gvt = as_global((const uint32_t *)rec_vt(w));
+    const gmem<U4> *glk = as_global((const U4 *)rec_link(w));
     const uint32_t lmax = 4u * n;
     bool bad = false;
     for (uint32_t i0 = 0; i0 < n; i0 += 256) {
@@ -957,7 +1041,8 @@ __device__ bool k2_stage(const DevWalker &w, const DevModel &m, lds_u64 *lk, lds
             const uint32_t i = i0 + 64u * u + lane;
             if (i < n) {
                 vt[u] = gvt[i];
-                l4[u] = glk[i];
+                const gmem<U4> &q = glk[i];
+                l4[u] = make_uint4(q.x, q.y, q.z, q.w);
             }
         }
 #pragma unroll
@@ -1191,23 +1276,29 @@ __device__ __forceinline__ void k3_pass(const DevWalker &w, const DevModel &m, i
     }
     int neg = 0;
     int64_t cnt = 0;
+    const gmem<uint32_t> *ops = as_global((const uint32_t *)w.ops);
+    const gmem<uint64_t> *brec = as_global((const uint64_t *)m.bond_rec);  // int2 pairs
+    // the next chunk's operators are loaded while this chunk is processed
+    uint32_t op_next = (int64_t)lane < L ? ops[lane] : 0u;
     for (int64_t base = 0; base < L; base += 64) {
         const int64_t p = base + lane;
-        const uint32_t op = p < L ? w.ops[p] : 0u;
+        const uint32_t op = op_next;
+        op_next = p + 64 < L ? ops[p + 64] : 0u;
         const uint32_t vc = op_vcode(op);
         int2 br = make_int2(0, 0);
         const DevVD *vd = m.types;
         if (op != 0u) {
-            br = m.bond_rec[op_bond(op)];
+            const uint64_t bv = brec[op_bond(op)];
+            br = make_int2((int)(uint32_t)bv, (int)(uint32_t)(bv >> 32));
             vd = &m.types[br.x >> 20];
         }
         if (want_sign)
-            neg += popc(__ballot(op != 0u && vd->signs[vc >> 1] < 0));
+            neg += popc(__ballot(op != 0u && as_global(vd->signs)[vc >> 1] < 0));
         cnt += popc(__ballot(op != 0u));
         const bool offd = op != 0u && !(vc & 1u);
         uint32_t l0 = 0, l1 = 0, l2 = 0, l3 = 0;
         if (offd) {
-            const uint8_t *ls = vd->leg_states + 4 * (vc >> 1);
+            const gmem<uint8_t> *ls = as_global(vd->leg_states) + 4 * (vc >> 1);
             l0 = ls[0];
             l1 = ls[1];
             l2 = ls[2];
@@ -1219,8 +1310,9 @@ __device__ __forceinline__ void k3_pass(const DevWalker &w, const DevModel &m, i
             if (k < ne) {
                 double delta = 0.0;
                 if (offd) {
-                    delta = coef[k][sa] * (mv[k][sa * D + l2 - 1] - mv[k][sa * D + l0 - 1]);
-                    delta += coef[k][sb] * (mv[k][sb * D + l3 - 1] - mv[k][sb * D + l1 - 1]);
+                    const gmem<double> *cf = as_global(coef[k]), *mk = as_global(mv[k]);
+                    delta = cf[sa] * (mk[sa * D + l2 - 1] - mk[sa * D + l0 - 1]);
+                    delta += cf[sb] * (mk[sb * D + l3 - 1] - mk[sb * D + l1 - 1]);
                 }
                 const double inc = wave_incl_scan(delta);
                 if (op != 0u) {
@@ -1346,6 +1438,13 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
 {
     c.max_sites = max_sites;
     c.lds_k1 = k1_lds_bytes(max_sites);
+    // stage each model's lookups in LDS if that keeps K1 within 32 KiB (high
+    // occupancy matters more than the lookups of larger models)
+    for (const DevModel &m : models) {
+        const size_t st = (k1_lds_staged(m.n_sites, m.n_bonds, m.k1tab_bytes) + 15) & ~(size_t)15;
+        if (st <= 32768 && st > c.lds_k1)
+            c.lds_k1 = st;
+    }
     int dev = 0, lim = 65536;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
@@ -1387,7 +1486,8 @@ static inline int wave_launch_init(WaveConfig &c, hipStream_t s, DevWalker *W, c
 {
     hipLaunchKernelGGL(wave_init_kernel, dim3(n), dim3(64), 0, s, W, M, n);
     for (int r = 0; r < max_warmup; ++r)
-        hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, 0, n, 1, r);
+        hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, 0, n, 1, r,
+                           (int)c.lds_k1);
     hipLaunchKernelGGL(activate_pending_kernel, dim3((n + 63) / 64), dim3(64), 0, s, W, n);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
@@ -1400,7 +1500,7 @@ static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, 
     const int k1 = phases & (1 | 2 | 8);
     if (k1 & 3) {
         hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, w0, w1, k1,
-                           -1);
+                           -1, (int)c.lds_k1);
     }
     if (ev)
         (void)hipEventRecord(ev[1], s);
