@@ -114,6 +114,42 @@ def cpu_baseline(sse, flat, wds, states, seconds, threads):
             "ns_per_operator_sweep": wall * threads / nops * 1e9 if nops else None}
 
 
+def measured_traffic(kernel, walkers, lattice, beta):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated) when it covers this
+    workload, else None."""
+    import re
+    path = os.path.join(ROOT, "profiles",
+                        "worm_kernel_hbm_traffic_heisenberg16x16_beta8_4096walkers.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    want = f"traffic_workload: walkers={walkers} lattice={lattice} beta={beta:g} kernel={kernel}"
+    m = re.search(r"traffic_bytes_per_launch: (\d+)", text)
+    return int(m.group(1)) if (want in text and m) else None
+
+
+def copy_calibration(mb):
+    """Device-to-device hipMemcpy of `mb` MiB, three times: a copy kernel of
+    known traffic (mb MiB read + mb MiB written per launch) that scales the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: 3 x {mb} MiB device-to-device")
+
+
 # ---------------------------------------------------------------- extras
 def _xxz_types(args):
     """VertexData of the two square-lattice bond types for one walker's Jz
@@ -244,6 +280,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="end with three device-to-device copies of this many MiB "
+                         "(reference kernel for HBM traffic counters)")
     ap.add_argument("--extra", action="store_true",
                     help="also run the heterogeneous, ladder and serving workloads")
     args = ap.parse_args()
@@ -311,6 +350,8 @@ def main():
                   "worm": nw * 12 * Ws / wsweeps + N * nw,
                   "measure": nw * 4 * Ls / wsweeps}
     dom = max(names, key=lambda n: kms[n])
+    kname = {"diag_vlist": "wave_diag_vlist_kernel", "worm": "wave_worm_kernel",
+             "measure": "wave_measure_kernel"}[dom]
     achieved = per_launch[dom] / (kms[dom] * 1e-3) / 1e9
     log("single-stream kernels ms/launch: " + ", ".join(f"{n}={kms[n]:.3f}" for n in names))
 
@@ -359,11 +400,11 @@ def main():
         "per_walker_sweep": {"L": Ls / wsweeps, "n": ns / wsweeps, "W": Ws / wsweeps,
                              "U": U / wsweeps, "B": B / wsweeps},
         "kernel_ms_per_launch_single_stream": kms,
-        "roofline": {"bound": "hbm", "kernel": {"diag_vlist": "wave_diag_vlist_kernel",
-                                                "worm": "wave_worm_kernel",
-                                                "measure": "wave_measure_kernel"}[dom],
+        "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(kname, args.walkers, args.lattice, args.beta),
+                     "algorithmic_bytes_per_launch": per_launch[dom]},
         "cpu_baseline": cpu,
     }
     out.update(baselines)
@@ -374,6 +415,8 @@ def main():
             log(f"extra workload: {name}")
             extras[name] = fn(sse)
         out["extra_workloads"] = extras
+    if args.calibrate_copy_mb:
+        copy_calibration(args.calibrate_copy_mb)
     print(json.dumps(out), flush=True)
 
 
