@@ -693,7 +693,8 @@ __device__ __forceinline__ WormTab worm_tab(const char *base, const DevModel &m)
 //   Rec10  u64 links in 16-bit fields + a u16 ecode array (n <= 16384)
 //   RecG   global memory: ecode rewritten in place over vt, links read through
 //          the scalar cache (K2 never writes them)
-// load() returns wave-uniform words; put_ecode() is called by the whole wave.
+// load() returns wave-uniform words; put_ecode() is called by the whole wave
+// with one value (all lanes store it).
 typedef __attribute__((address_space(3))) uint64_t lds_u64;
 typedef __attribute__((address_space(3))) uint16_t lds_u16;
 typedef __attribute__((address_space(3))) uint8_t lds_u8;
@@ -707,6 +708,7 @@ typedef const __attribute__((address_space(4))) DirectEntry ConstEntry;
 struct Rec8 {
     lds_u64 *rec;
     static constexpr bool kChecked = false;
+    typedef uint32_t Key;  // leg id < 2^16: (leg id << 14 | worm-1) fits 32 bits
     struct Word {
         uint32_t lo, hi;
     };
@@ -721,10 +723,10 @@ struct Rec8 {
         const uint64_t v = (uint64_t)x.lo | ((uint64_t)x.hi << 32);
         return (uint32_t)(v >> (14u * leg)) & 0x3FFFu;
     }
+    // every lane writes the same byte (no exec-mask switch in the chase loop)
     __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const
     {
-        if (threadIdx.x == 0)
-            ((lds_u8 *)rec)[8u * r + 7u] = (uint8_t)e;
+        ((lds_u8 *)rec)[8u * r + 7u] = (uint8_t)e;
     }
     __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return (uint32_t)(rec[r] >> 56); }
 };
@@ -733,6 +735,7 @@ struct Rec10 {
     lds_u64 *lk;
     lds_u16 *ec;
     static constexpr bool kChecked = false;
+    typedef uint32_t Key;
     struct Word {
         uint32_t lo, hi, e;
     };
@@ -748,11 +751,7 @@ struct Rec10 {
         const uint64_t v = (uint64_t)x.lo | ((uint64_t)x.hi << 32);
         return (uint32_t)(v >> (16u * leg)) & 0xFFFFu;
     }
-    __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const
-    {
-        if (threadIdx.x == 0)
-            ec[r] = (uint16_t)e;
-    }
+    __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const { ec[r] = (uint16_t)e; }
     __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
 };
 
@@ -760,6 +759,7 @@ struct RecG {
     uint32_t *ec;
     const_u4 *lk;
     static constexpr bool kChecked = true;
+    typedef uint64_t Key;
     struct Word {
         U4 l;
         uint32_t e;
@@ -814,7 +814,7 @@ __device__ __forceinline__ uint32_t chase_select(uint32_t mlo, uint32_t mhi, Con
 // arrives in one scalar load (constant address space, uniform offset), the exit
 // is selected on the scalar unit, and the closing tests compare packed keys
 // (leg id << 14 | worm-1).  Returns an error code (0 = ok).
-template <class R>
+template <class R, bool kOneWorm>
 __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, const R &rec,
                           const uint64_t *MASK, const uint32_t *BASE, double &total, int64_t &scat)
 {
@@ -851,8 +851,9 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
             return SSE_ERR_STATE;
         const uint32_t lid0 = 4u * r0 + l0;
         // closing keys (sse.jl:290-297): exit (r0, l0, dim0 - wf0) or entry (r0, l0, wf0)
-        const uint64_t keyA = ((uint64_t)lid0 << 14) | (dim0 - wf0 - 1u);
-        const uint64_t keyB = ((uint64_t)lid0 << 14) | (wf0 - 1u);
+        typedef typename R::Key Key;
+        const Key keyA = ((Key)lid0 << 14) | (dim0 - wf0 - 1u);
+        const Key keyB = ((Key)lid0 << 14) | (wf0 - 1u);
         uint32_t lid = lid0, wfm = wf0 - 1u;
         const uint32_t sc0 = sc;
         uint32_t endB = 0, err = 0;
@@ -870,7 +871,9 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
             const uint32_t wlo = uni((uint32_t)wv), whi = uni((uint32_t)(wv >> 32));
             rs.off = off + 1u;
             const uint32_t e = R::ecode(x);
-            ConstEntry *E = (ConstEntry *)(CT + (e * estride + (wfm << 8) + ((lid & 3u) << 6)));
+            const uint32_t eoff = kOneWorm ? (e << 8) | ((lid & 3u) << 6)
+                                           : e * estride + (wfm << 8) + ((lid & 3u) << 6);
+            ConstEntry *E = (ConstEntry *)(CT + eoff);
             uint32_t ft;
             const uint64_t mm = (((uint64_t)whi << 32) | wlo) >> 11;  // rand() = mm * 2^-53
             uint32_t eo = chase_select((uint32_t)mm, (uint32_t)(mm >> 32), E, ft);
@@ -879,9 +882,13 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
                 // more than four exits: walk the exit list (vertex_data.jl:113-125);
                 // a fall-through or an invalid transition is an error
                 const uint32_t cnt = uni(E->count);
-                eo = e << 16;  // rewrites the unchanged code if the scatter fails
+                // on failure: rewrite the unchanged code and leave the loop through
+                // the exit test (lid and the exit word are made to match key A)
+                const uint32_t efail = (e << 16) | (l0 << 14) | (dim0 - wf0 - 1u);
+                eo = efail;
                 if (cnt <= 4u) {
                     err = SSE_ERR_SCATTER;
+                    lid = lid0;
                 } else {
                     const uint32_t trw = uni(E->trans);
                     const WormTab tab = worm_tab((const char *)m.wtab, m);
@@ -891,6 +898,7 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
                     for (;;) {
                         if (ex >= eend) {
                             err = SSE_ERR_SCATTER;
+                            lid = lid0;
                             break;
                         }
                         const uint64_t cb = uni64((uint64_t)__double_as_longlong(tab.cum[ex]));
@@ -908,12 +916,17 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
             // ends leaving through (r0, l0, dim0 - wf0), or entering (r0, l0, wf0)
             const uint32_t nx = R::link(x, (eo >> 14) & 3u);
             const uint32_t wfn = eo & 0x3FFFu;
-            const bool endA = ((((uint64_t)(lid & ~3u) << 14) | (eo & 0xFFFFu)) == keyA);
-            const bool hitB = ((((uint64_t)nx << 14) | wfn) == keyB);
-            if (R::kChecked && !endA && (nx >> 2) >= n)
+            const Key kA = ((Key)(lid & ~3u) << 14) | (eo & 0xFFFFu);
+            const Key kB = ((Key)nx << 14) | wfn;
+            // one exit: d == 0 when either closing test holds (a failed scatter
+            // above forced the exit test to hold)
+            Key d = min(kA ^ keyA, kB ^ keyB);
+            if (R::kChecked && (nx >> 2) >= n && kA != keyA) {
                 err = SSE_ERR_STATE;
-            if (endA || hitB || err) {
-                endB = !endA && hitB;
+                d = 0;
+            }
+            if (d == 0) {
+                endB = kA != keyA;  // closed on re-entry (the exit test has priority)
                 break;
             }
             lid = nx;
@@ -1131,13 +1144,15 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
             err = SSE_ERR_STATE;
         } else if (fmt == 8) {
             Rec8 rec{llk};
-            err = worm_chase(w, m, rs, rec, lmask, lbase, total, scat);
+            err = m.dt_mw == 1 ? worm_chase<Rec8, true>(w, m, rs, rec, lmask, lbase, total, scat)
+                               : worm_chase<Rec8, false>(w, m, rs, rec, lmask, lbase, total, scat);
             __syncthreads();
             if (!err)
                 worm_finish(w, m, rs, rec, lmask, lbase);
         } else {
             Rec10 rec{llk, lec};
-            err = worm_chase(w, m, rs, rec, lmask, lbase, total, scat);
+            err = m.dt_mw == 1 ? worm_chase<Rec10, true>(w, m, rs, rec, lmask, lbase, total, scat)
+                               : worm_chase<Rec10, false>(w, m, rs, rec, lmask, lbase, total, scat);
             __syncthreads();
             if (!err)
                 worm_finish(w, m, rs, rec, lmask, lbase);
@@ -1147,7 +1162,7 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
             err = SSE_ERR_STATE;
         } else {
             RecG rec{rec_vt(w), (const_u4 *)(const void *)rec_link(w)};
-            err = worm_chase(w, m, rs, rec, rec_mask(w), rec_base(w), total, scat);
+            err = worm_chase<RecG, false>(w, m, rs, rec, rec_mask(w), rec_base(w), total, scat);
             __syncthreads();
             if (!err)
                 worm_finish(w, m, rs, rec, rec_mask(w), rec_base(w));
