@@ -234,9 +234,12 @@ int sse_pool_usage(sse_batch *b, int64_t *used_bytes, int64_t *total_bytes);
  *                launch eagerly.
  * SSE_OPT_WORM_RECORDS (default 0): record layout of the worm kernel.  0 picks
  *                the smallest that fits (8-byte LDS records for n <= 4096 and
- *                <= 256 chase codes, 10-byte LDS records for n <= 16384, else
- *                global memory); 1 skips the 8-byte layout; 2 forces global
- *                memory.  Results are identical; this is a tuning/testing knob.
+ *                <= 256 chase codes, 10-byte LDS records for n <= 16384, chase
+ *                codes in LDS with links read from global memory, else global
+ *                memory); the kernel's LDS per walker is capped at 64 KiB so
+ *                large walkers take the later layouts; 1 skips the 8-byte
+ *                layout; 2 forces global memory; 3 forces the hybrid.  Results
+ *                are identical; this is a tuning/testing knob.
  * SSE_OPT_PERSISTENT (default 0): persistent multi-sweep launches (workers
  *                take (walker, sweep) tasks in order).  Disabled in this build:
  *                setting it to 1 returns SSE_ERR_INVALID (its first GPU run

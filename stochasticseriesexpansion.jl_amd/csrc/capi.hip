@@ -1499,10 +1499,10 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         b->n_groups = (int)value;
         return SSE_OK;
     case SSE_OPT_WORM_RECORDS:
-        if (value < 0 || value > 2)
-            return fail(SSE_ERR_INVALID, "SSE_OPT_WORM_RECORDS takes 0, 1 or 2");
+        if (value < 0 || value > 3)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_WORM_RECORDS takes 0, 1, 2 or 3");
         b->wave.rec_mode = (int)value;
-        if (value == 1) {  // Rec10 needs more LDS than the Rec8 sizing gave
+        if (value == 1 || value == 3) {  // other layouts than the sizing assumed
             DevPool p;
             HIPCHECK(hipStreamSynchronize(b->stream));
             HIPCHECK(hipMemcpy(&p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));

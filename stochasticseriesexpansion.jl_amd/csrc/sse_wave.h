@@ -779,6 +779,31 @@ struct RecG {
     __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
 };
 
+// Hybrid for walkers too large for Rec8/Rec10: chase codes in LDS (u16), links
+// in global memory through the scalar cache (K2 never writes them)
+struct RecH {
+    lds_u16 *ec;
+    const_u4 *lk;
+    static constexpr bool kChecked = true;
+    typedef uint64_t Key;
+    struct Word {
+        U4 l;
+        uint32_t e;
+    };
+    __device__ __forceinline__ Word load(uint32_t r) const
+    {
+        const_u4 &q = lk[r];
+        return Word{U4{q.x, q.y, q.z, q.w}, uni(ec[r])};
+    }
+    __device__ __forceinline__ static uint32_t ecode(const Word &x) { return x.e; }
+    __device__ __forceinline__ static uint32_t link(const Word &x, uint32_t leg)
+    {
+        return leg == 0 ? x.l.x : leg == 1 ? x.l.y : leg == 2 ? x.l.z : x.l.w;
+    }
+    __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const { ec[r] = (uint16_t)e; }
+    __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
+};
+
 // Exit selection of one scatter on the scalar unit: m = word >> 11 against the
 // entry's thresholds by 64-bit borrow chains (s_sub/s_subb leave the borrow,
 // m < K, in SCC).  Returns out[j] for the first j with m < K[j]; ft = 1 when
@@ -1029,11 +1054,18 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &re
 __host__ __device__ inline size_t k2_lds_need(int64_t n, int64_t L, int fmt)
 {
     const size_t nch = (size_t)((L + 63) / 64);
-    size_t b = 1024 + nch * 8 + ((nch * 4 + 15) & ~(size_t)15) + (size_t)n * 8;
+    size_t b = 1024 + nch * 8 + ((nch * 4 + 15) & ~(size_t)15);
+    if (fmt == 2)  // RecH: chase codes only
+        return b + (((size_t)n * 2 + 15) & ~(size_t)15);
+    b += (size_t)n * 8;
     if (fmt == 10)
         b += ((size_t)n * 2 + 15) & ~(size_t)15;
     return b;
 }
+
+// LDS per walker the worm kernel is sized for at most: beyond it a walker uses
+// the hybrid or global records instead of lowering every walker's residency
+constexpr size_t K2_LDS_CAP = 65536;
 
 // Stage the compact records of K1 (global) into the LDS layout R, converting
 // vt to ecodes and checking every ecode and link; returns false on a bad record.
@@ -1079,6 +1111,21 @@ __device__ bool k2_stage(const DevWalker &w, const DevModel &m, lds_u64 *lk, lds
     return __ballot(bad) == 0;
 }
 
+// vt -> ecode into an LDS u16 array (RecH path), checking every code.
+__device__ bool k2_stage_codes(const DevWalker &w, const DevModel &m, lds_u16 *ec)
+{
+    const int lane = threadIdx.x;
+    const uint32_t n = (uint32_t)w.n, lgp = (uint32_t)m.dt_lgp, nt = (uint32_t)m.n_types;
+    const gmem<uint32_t> *gvt = as_global((const uint32_t *)rec_vt(w));
+    bool bad = false;
+    for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t vt = gvt[i];
+        bad |= (vt & 1023u) >= (1u << lgp) || (vt >> 10) >= nt;
+        ec[i] = (uint16_t)ecode_of_vt(vt, lgp);
+    }
+    return __ballot(bad) == 0;
+}
+
 // In-place vt -> ecode conversion of the global records (RecG path).
 __device__ bool k2_stage_global(const DevWalker &w, const DevModel &m)
 {
@@ -1111,16 +1158,43 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
     rs.init(rbuf, w.key, w.ctr);
     const int64_t nch = (w.L + 63) / 64;
     const bool small_codes = ((int64_t)m.n_types << m.dt_lgp) <= 256;
+    const bool u16_codes = ((int64_t)m.n_types << m.dt_lgp) <= 65536;
     int fmt = 0;
     if (rec_mode == 0 && w.n <= 4096 && small_codes && k2_lds_need(w.n, w.L, 8) <= (size_t)lds_bytes)
         fmt = 8;
-    else if (rec_mode <= 1 && w.n <= 16384 && ((int64_t)m.n_types << m.dt_lgp) <= 65536 &&
-             k2_lds_need(w.n, w.L, 10) <= (size_t)lds_bytes)
+    else if (rec_mode <= 1 && w.n <= 16384 && u16_codes && k2_lds_need(w.n, w.L, 10) <= (size_t)lds_bytes)
         fmt = 10;
+    else if ((rec_mode <= 1 || rec_mode == 3) && u16_codes && k2_lds_need(w.n, w.L, 2) <= (size_t)lds_bytes)
+        fmt = 2;
     double total = 1.0;
     int64_t scat = 0;
     int err = 0;
-    if (fmt) {
+    if (fmt == 2) {
+        char *q = lds + 1024;
+        lds_u16 *lec = (lds_u16 *)(uint16_t *)q;
+        q += (w.n * 2 + 15) & ~(int64_t)15;
+        uint64_t *lmask = (uint64_t *)q;
+        q += nch * 8;
+        uint32_t *lbase = (uint32_t *)q;
+        const uint64_t *gmask = rec_mask(w);
+        const uint32_t *gbase = rec_base(w);
+        for (int64_t i = lane; i < nch; i += 64) {
+            lmask[i] = gmask[i];
+            lbase[i] = gbase[i];
+        }
+        const bool ok = k2_stage_codes(w, m, lec);
+        __syncthreads();
+        if (!ok) {
+            err = SSE_ERR_STATE;
+        } else {
+            RecH rec{lec, (const_u4 *)(const void *)rec_link(w)};
+            err = m.dt_mw == 1 ? worm_chase<RecH, true>(w, m, rs, rec, lmask, lbase, total, scat)
+                               : worm_chase<RecH, false>(w, m, rs, rec, lmask, lbase, total, scat);
+            __syncthreads();
+            if (!err)
+                worm_finish(w, m, rs, rec, lmask, lbase);
+        }
+    } else if (fmt) {
         // records first, at a fixed offset, so the chase addresses them with
         // the offset folded into the LDS instruction
         char *q = lds + 1024;
@@ -1201,7 +1275,7 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
 }
 
 // K2.  rec_mode 0 picks the smallest record layout that fits (Rec8, Rec10,
-// then RecG); 1 skips Rec8; 2 forces RecG.
+// RecH, then RecG); 1 skips Rec8; 2 forces RecG; 3 forces RecH.
 __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevModel *models,
                                                        DevPool *pool, int w_begin, int w_end,
                                                        int full_sweep, int lds_bytes, int rec_mode)
@@ -1488,7 +1562,11 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
 static inline void wave_resize_lds(WaveConfig &c, unsigned long long max_n, unsigned long long max_L)
 {
     const int64_t nh = (int64_t)(max_n + max_n / 16 + 32);
-    size_t need = k2_lds_need(nh, (int64_t)max_L, nh <= 4096 && c.small_codes && c.rec_mode == 0 ? 8 : 10);
+    size_t need = k2_lds_need(nh, (int64_t)max_L, nh <= 4096 && c.small_codes && c.rec_mode == 0 ? 8
+                                                  : c.rec_mode == 3                           ? 2
+                                                                                              : 10);
+    if (need > K2_LDS_CAP)  // larger walkers take the hybrid or global records
+        need = K2_LDS_CAP;
     need = (need + 511) & ~(size_t)511;
     if (need > c.lds_limit)
         need = c.lds_limit;

@@ -100,10 +100,11 @@ def test_parity_bilayer_cluster(sse_pkg, oracle_mod, mode):
                 thermalization=10)
 
 
-@pytest.mark.parametrize("records", [1, 2])
+@pytest.mark.parametrize("records", [1, 2, 3])
 def test_parity_worm_record_layouts(sse_pkg, oracle_mod, records):
-    # 10-byte LDS records (1) and global-memory records (2) against the oracle;
-    # the default 8-byte layout runs in every other wave-mode test
+    # 10-byte LDS records (1), global-memory records (2) and the hybrid (3: codes
+    # in LDS, links in global memory) against the oracle; the default 8-byte
+    # layout runs in every other wave-mode test
     _run_parity(sse_pkg, oracle_mod, square_heisenberg(sse_pkg, 4), 0.5, 6, 25, 0,
                 thermalization=10, records=records)
     _run_parity(sse_pkg, oracle_mod, job_honeycomb(sse_pkg, 0.4), 0.4, 4, 20, 0, seed0=50,
