@@ -879,8 +879,12 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
         typedef typename R::Key Key;
         const Key keyA = ((Key)lid0 << 14) | (dim0 - wf0 - 1u);
         const Key keyB = ((Key)lid0 << 14) | (wf0 - 1u);
-        uint32_t lid = lid0, wfm = wf0 - 1u;
-        const uint32_t sc0 = sc;
+        // current leg id kept shifted (lid << 14): the record index, the entry's leg
+        // and both closing keys come from it with one operation each
+        Key lid14 = (Key)lid0 << 14;
+        uint32_t wfm = wf0 - 1u;
+        // one Philox word per step: the step count is the stream advance
+        const uint64_t pos0 = rs.bbase + rs.off;
         uint32_t endB = 0, err = 0;
         for (;;) {  // sse.jl:274-300
             if (rs.off >= 128u) {
@@ -891,18 +895,17 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
             }
             const uint32_t off = uni(rs.off);
             const uint64_t wv = RB[off];
-            const uint32_t r = lid >> 2;
+            const uint32_t r = (uint32_t)(lid14 >> 16);
             const typename R::Word x = rec.load(r);
             const uint32_t wlo = uni((uint32_t)wv), whi = uni((uint32_t)(wv >> 32));
             rs.off = off + 1u;
             const uint32_t e = R::ecode(x);
-            const uint32_t eoff = kOneWorm ? (e << 8) | ((lid & 3u) << 6)
-                                           : e * estride + (wfm << 8) + ((lid & 3u) << 6);
+            const uint32_t legbits = (uint32_t)(lid14 >> 8) & 0xC0u;  // leg_in << 6
+            const uint32_t eoff = kOneWorm ? (e << 8) | legbits : e * estride + (wfm << 8) + legbits;
             ConstEntry *E = (ConstEntry *)(CT + eoff);
             uint32_t ft;
             const uint64_t mm = (((uint64_t)whi << 32) | wlo) >> 11;  // rand() = mm * 2^-53
             uint32_t eo = chase_select((uint32_t)mm, (uint32_t)(mm >> 32), E, ft);
-            sc += 1;
             if (ft) {
                 // more than four exits: walk the exit list (vertex_data.jl:113-125);
                 // a fall-through or an invalid transition is an error
@@ -913,7 +916,7 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
                 eo = efail;
                 if (cnt <= 4u) {
                     err = SSE_ERR_SCATTER;
-                    lid = lid0;
+                    lid14 = (Key)lid0 << 14;
                 } else {
                     const uint32_t trw = uni(E->trans);
                     const WormTab tab = worm_tab((const char *)m.wtab, m);
@@ -923,7 +926,7 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
                     for (;;) {
                         if (ex >= eend) {
                             err = SSE_ERR_SCATTER;
-                            lid = lid0;
+                            lid14 = (Key)lid0 << 14;
                             break;
                         }
                         const uint64_t cb = uni64((uint64_t)__double_as_longlong(tab.cum[ex]));
@@ -941,8 +944,9 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
             // ends leaving through (r0, l0, dim0 - wf0), or entering (r0, l0, wf0)
             const uint32_t nx = R::link(x, (eo >> 14) & 3u);
             const uint32_t wfn = eo & 0x3FFFu;
-            const Key kA = ((Key)(lid & ~3u) << 14) | (eo & 0xFFFFu);
-            const Key kB = ((Key)nx << 14) | wfn;
+            const Key kA = (lid14 & ~(Key)0xC000u) | (eo & 0xFFFFu);
+            const Key nx14 = (Key)nx << 14;
+            const Key kB = nx14 | wfn;
             // one exit: d == 0 when either closing test holds (a failed scatter
             // above forced the exit test to hold)
             Key d = min(kA ^ keyA, kB ^ keyB);
@@ -954,13 +958,15 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
                 endB = kA != keyA;  // closed on re-entry (the exit test has priority)
                 break;
             }
-            lid = nx;
+            lid14 = nx14;
             wfm = wfn;
         }
         if (err)
             return (int)err;
+        const uint32_t steps = (uint32_t)(rs.bbase + rs.off - pos0);
+        sc += steps;
         // worm length: scatters + 1 when the worm closed on re-entry
-        total += (double)(sc - sc0 + endB);
+        total += (double)(steps + endB);
     }
     scat = sc;
     return 0;
