@@ -343,6 +343,7 @@ def main():
         batch.run(1, measure=True)
         ktimes += batch.phase_times()
     batch.set_option(sse.capi.SSE_OPT_STREAMS, args.streams)
+    lds = batch.lds_bytes()
     names = ["diag_vlist", "worm", "measure"]
     kms = {n: ktimes[i] / 3 for i, n in enumerate(names)}
     nw = args.walkers
@@ -400,6 +401,7 @@ def main():
         "per_walker_sweep": {"L": Ls / wsweeps, "n": ns / wsweeps, "W": Ws / wsweeps,
                              "U": U / wsweeps, "B": B / wsweeps},
         "kernel_ms_per_launch_single_stream": kms,
+        "lds_bytes_per_walker": {"diag_vlist": lds[0], "worm": lds[1], "worm_walkers_per_cu": lds[2]},
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -199,6 +199,11 @@ int sse_phase_times(const sse_batch *b, double *ms3);
  * sse_run call and its sweep count (0, 0 with timing off).  No reference
  * counterpart: the reference runs one walker per process.                  */
 int sse_call_time(const sse_batch *b, double *ms, int64_t *sweeps);
+/* Dynamic LDS bytes per walker of the current diagonal-update and worm-kernel
+ * launches (wave mode; sized from the batch's largest operator count) and the
+ * worm-kernel workgroups a CU can hold at that size (HIP occupancy query).
+ * No reference counterpart (tuning introspection).                         */
+int sse_lds_bytes(const sse_batch *b, int64_t *k1, int64_t *k2, int32_t *k2_per_cu);
 /* Sum over walkers of cutoff L, operators n and scatter calls W since the
  * last reset (counters taken from the chains themselves).                  */
 int sse_work_counters(sse_batch *b, int64_t *L_sum, int64_t *n_sum,
@@ -249,13 +254,18 @@ int sse_pool_usage(sse_batch *b, int64_t *used_bytes, int64_t *total_bytes);
  *                joined into the batch stream), so the slow tail of one
  *                group's worm update overlaps the other groups' sweeps.  1 =
  *                one stream (graphs and per-kernel timing apply only then).
- *                Results are identical for any value.                           */
+ *                Results are identical for any value.
+ * SSE_OPT_K2_LDS (default 0): LDS bytes per walker of the worm kernel; 0 =
+ *                sized from the batch's largest operator count (then widened to
+ *                the largest size with the same walkers per CU).  Walkers that
+ *                do not fit take a slower record layout; results are identical. */
 enum sse_option {
     SSE_OPT_GRAPH = 1,
     SSE_OPT_TIMING = 2,
     SSE_OPT_WORM_RECORDS = 3,
     SSE_OPT_PERSISTENT = 4,
-    SSE_OPT_STREAMS = 5
+    SSE_OPT_STREAMS = 5,
+    SSE_OPT_K2_LDS = 6
 };
 int sse_set_option(sse_batch *b, int32_t option, int64_t value);
 

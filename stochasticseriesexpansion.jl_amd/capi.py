@@ -23,6 +23,7 @@ SSE_OPT_TIMING = 2
 SSE_OPT_WORM_RECORDS = 3
 SSE_OPT_PERSISTENT = 4
 SSE_OPT_STREAMS = 5
+SSE_OPT_K2_LDS = 6
 
 STATUS = {0: "SSE_OK", 1: "SSE_ERR_INVALID", 2: "SSE_ERR_HIP", 3: "SSE_ERR_OOM",
           4: "SSE_ERR_SCATTER", 5: "SSE_ERR_NO_DEVICE", 6: "SSE_ERR_STATE"}
@@ -188,6 +189,8 @@ def load_library(path=None):
                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     lib.sse_phase_times.argtypes = [vp, C.POINTER(C.c_double)]
     lib.sse_call_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    lib.sse_lds_bytes.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int32)]
     lib.sse_work_counters.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.sse_opcode_to_reference.argtypes = [C.c_uint32]
@@ -214,7 +217,7 @@ EXPORTED_SYMBOLS = (
     "sse_phase_times", "sse_work_counters", "sse_opcode_to_reference",
     "sse_opcode_from_reference", "sse_last_error", "sse_abi_version", "sse_add_model",
     "sse_slot_start", "sse_slot_stop", "sse_slot_poll", "sse_pool_usage", "sse_set_option",
-    "sse_call_time")
+    "sse_call_time", "sse_lds_bytes")
 
 
 class SSEError(RuntimeError):
@@ -386,6 +389,12 @@ class Batch:
         ms, sw = C.c_double(), C.c_int64()
         check(self.lib, self.lib.sse_call_time(self.handle, C.byref(ms), C.byref(sw)))
         return ms.value, sw.value
+
+    def lds_bytes(self):
+        """(K1 bytes, K2 bytes, K2 walkers per CU) of the current launches."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int32()
+        check(self.lib, self.lib.sse_lds_bytes(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def work_counters(self):
         a = [C.c_int64() for _ in range(4)]

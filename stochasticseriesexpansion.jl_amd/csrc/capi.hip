@@ -79,6 +79,7 @@ struct sse_batch {
     // wave mode: walkers split into groups, each swept on its own stream, so
     // one group's sweeps overlap the slow tail of another's
     int n_groups = 2;
+    unsigned long long seen_max_n = 0, seen_max_L = 0;  // operator counts behind the LDS size
     std::vector<hipStream_t> gstreams;
     std::vector<hipEvent_t> gjoin;
     hipEvent_t gfork = nullptr;
@@ -455,8 +456,15 @@ static int check_walker_errors(sse_batch *b)
     HIPCHECK(hipMemcpy(&p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));
     if (p.overflow)
         return fail(SSE_ERR_OOM, "slab pool exhausted; raise SSE_POOL_MB");
-    if (b->mode == SSE_MODE_WAVE)
-        wave_resize_lds(b->wave, p.max_n, p.max_L);
+    if (b->mode == SSE_MODE_WAVE && (p.max_n || p.max_L)) {
+        // size the worm kernel's LDS from the operator counts of this call's
+        // sweeps (the device maxima restart at zero for the next call)
+        b->seen_max_n = p.max_n;
+        b->seen_max_L = p.max_L;
+        const unsigned long long zero[2] = {0, 0};
+        HIPCHECK(hipMemcpy(&b->d_pool->max_n, zero, sizeof zero, hipMemcpyHostToDevice));
+        wave_resize_lds(b->wave, b->seen_max_n, b->seen_max_L);
+    }
     if (p.status) {
         if (p.status & (1 << SSE_ERR_SCATTER))
             return fail(SSE_ERR_SCATTER, "scatter fell through (r >= last cumprob) or hit an "
@@ -1331,9 +1339,7 @@ int sse_add_model(sse_batch *b, const sse_model_desc *model, int32_t *index)
     b->h_models.push_back(dm);
     if (((int64_t)dm.n_types << dm.dt_lgp) > 256 && b->wave.small_codes) {
         b->wave.small_codes = false;  // 10-byte worm records from now on: size LDS for them
-        DevPool p;
-        HIPCHECK(hipMemcpy(&p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));
-        wave_resize_lds(b->wave, p.max_n, p.max_L);
+        wave_resize_lds(b->wave, b->seen_max_n, b->seen_max_L);
     }
     DevModel *nd = nullptr;
     rc = dev_alloc(b, &nd, b->h_models.size());
@@ -1492,6 +1498,14 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
             return fail(SSE_ERR_INVALID, "SSE_OPT_PERSISTENT is disabled in this build: its first "
                                          "GPU run ended in a memory fault whose cause is not found");
         return SSE_OK;
+    case SSE_OPT_K2_LDS: {
+        if (value != 0 && (value < 8 * (int64_t)K2Stream::kNW + 256 || value > (int64_t)b->wave.lds_limit))
+            return fail(SSE_ERR_INVALID, "SSE_OPT_K2_LDS: 0 or a size the device allows");
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        b->wave.lds_k2_fixed = (size_t)value & ~(size_t)15;
+        wave_resize_lds(b->wave, b->seen_max_n, b->seen_max_L);
+        return SSE_OK;
+    }
     case SSE_OPT_STREAMS:
         if (value < 1 || value > 64)
             return fail(SSE_ERR_INVALID, "SSE_OPT_STREAMS takes 1..64 walker groups");
@@ -1502,16 +1516,29 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         if (value < 0 || value > 3)
             return fail(SSE_ERR_INVALID, "SSE_OPT_WORM_RECORDS takes 0, 1, 2 or 3");
         b->wave.rec_mode = (int)value;
-        if (value == 1 || value == 3) {  // other layouts than the sizing assumed
-            DevPool p;
-            HIPCHECK(hipStreamSynchronize(b->stream));
-            HIPCHECK(hipMemcpy(&p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));
-            wave_resize_lds(b->wave, p.max_n, p.max_L);
-        }
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        wave_resize_lds(b->wave, b->seen_max_n, b->seen_max_L);  // layouts differ in size
         return SSE_OK;
     default:
         return fail(SSE_ERR_INVALID, "unknown option");
     }
+}
+
+int sse_lds_bytes(const sse_batch *b, int64_t *k1, int64_t *k2, int32_t *k2_per_cu)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    if (k1)
+        *k1 = (int64_t)b->wave.lds_k1;
+    if (k2)
+        *k2 = (int64_t)b->wave.lds_k2;
+    if (k2_per_cu) {
+        int nb = 0;
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)wave_worm_kernel, 64,
+                                                              b->wave.lds_k2));
+        *k2_per_cu = nb;
+    }
+    return SSE_OK;
 }
 
 int sse_call_time(const sse_batch *b, double *ms, int64_t *sweeps)

@@ -36,6 +36,7 @@ struct WaveConfig {
     size_t lds_limit = 65536;  // device limit on dynamic LDS per workgroup
     bool small_codes = true;   // every model has <= 256 chase codes (Rec8 records)
     int rec_mode = 0;          // worm-kernel record layout: 0 auto, 1 no Rec8, 2 global
+    size_t lds_k2_fixed = 0;   // worm-kernel LDS per walker set by the caller (0 = sized here)
 };
 
 __device__ __forceinline__ uint64_t below_mask(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
@@ -52,8 +53,7 @@ __device__ __noinline__ void wave_refill(uint64_t *buf, uint64_t key, uint64_t b
     const int l = threadIdx.x;
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < NW / 128; ++r) {
-        int j = l + 64 * r;
+    for (int j = l; j < NW / 2; j += 64) {
         uint32_t o0, o1, o2, o3;
         philox_block((base >> 1) + (uint64_t)j, key, o0, o1, o2, o3);
         buf[2 * j] = (uint64_t)o0 | ((uint64_t)o1 << 32);
@@ -64,6 +64,7 @@ __device__ __noinline__ void wave_refill(uint64_t *buf, uint64_t key, uint64_t b
 
 template <int NW>
 struct WaveStreamT {
+    static constexpr uint32_t kNW = NW;
     uint64_t *buf;
     uint64_t key, bbase;
     uint32_t off;  // next word is bbase + off; off > 256 marks an empty buffer
@@ -126,6 +127,8 @@ struct WaveStreamT {
 
 typedef WaveStreamT<256> WaveStream;
 typedef WaveStreamT<128> WaveStream128;
+// the worm kernel's stream: 64 words (512 bytes of LDS per walker)
+typedef WaveStreamT<64> K2Stream;
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ uint64_t uni64(uint64_t x)
@@ -840,8 +843,9 @@ __device__ __forceinline__ uint32_t chase_select(uint32_t mlo, uint32_t mhi, Con
 // is selected on the scalar unit, and the closing tests compare packed keys
 // (leg id << 14 | worm-1).  Returns an error code (0 = ok).
 template <class R, bool kOneWorm>
-__device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, const R &rec,
-                          const uint64_t *MASK, const uint32_t *BASE, double &total, int64_t &scat)
+__device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R &rec,
+                          const gmem<uint64_t> *MASK, const gmem<uint32_t> *BASE, double &total,
+                          int64_t &scat)
 {
     const int64_t L = w.L;
     const uint32_t n = (uint32_t)w.n;
@@ -887,8 +891,8 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
         const uint64_t pos0 = rs.bbase + rs.off;
         uint32_t endB = 0, err = 0;
         for (;;) {  // sse.jl:274-300
-            if (rs.off >= 128u) {
-                if (rs.off == 128u)
+            if (rs.off >= K2Stream::kNW) {
+                if (rs.off == K2Stream::kNW)
                     rs.refill_next();
                 else
                     rs.ensure(1);
@@ -975,14 +979,16 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, WaveStream128 &rs, co
 // state refresh (sse.jl:219-228) and operator write-back after the worms
 template <class R, class RS>
 __device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &rec,
-                            const uint64_t *MASK, const uint32_t *BASE)
+                            const gmem<uint64_t> *MASK, const gmem<uint32_t> *BASE)
 {
     const int lane = threadIdx.x;
     const int N = m.n_sites;
     const uint32_t lgp = (uint32_t)m.dt_lgp;
-    for (int sb = 0; sb < N; sb += 64) {
+    // sites in blocks of 32 so that one block draws at most 32 words (the
+    // stream holds 64)
+    for (int sb = 0; sb < N; sb += 32) {
         const int s = sb + lane;
-        const bool vs = s < N;
+        const bool vs = lane < 32 && s < N;
         const int32_t f = vs ? w.first[s] : 0;
         const bool fr = vs && f < 0;
         const uint64_t frm = __ballot(fr);
@@ -1006,7 +1012,7 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &re
             val = vd.leg_states[4 * ((vt & 1023u) >> 1) + (f & 3)];
         }
         if (__ballot(reject)) {
-            for (int i = 0; i < 64 && sb + i < N; ++i) {
+            for (int i = 0; i < 32 && sb + i < N; ++i) {
                 const bool fi = (frm >> i) & 1ull;
                 uint8_t vi = 0;
                 if (fi)
@@ -1057,16 +1063,12 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &re
 
 // LDS bytes the worm kernel needs for a walker with n operators, cutoff L, in
 // the Rec8 (fmt 8) or Rec10 (fmt 10) layout
-__host__ __device__ inline size_t k2_lds_need(int64_t n, int64_t L, int fmt)
+__host__ __device__ inline size_t k2_lds_need(int64_t n, int64_t, int fmt)
 {
-    const size_t nch = (size_t)((L + 63) / 64);
-    size_t b = 1024 + nch * 8 + ((nch * 4 + 15) & ~(size_t)15);
-    if (fmt == 2)  // RecH: chase codes only
+    const size_t b = 8 * K2Stream::kNW;  // Philox words
+    if (fmt == 2)                        // RecH: chase codes only
         return b + (((size_t)n * 2 + 15) & ~(size_t)15);
-    b += (size_t)n * 8;
-    if (fmt == 10)
-        b += ((size_t)n * 2 + 15) & ~(size_t)15;
-    return b;
+    return b + (size_t)n * 8 + (fmt == 10 ? (((size_t)n * 2 + 15) & ~(size_t)15) : 0);
 }
 
 // LDS per walker the worm kernel is sized for at most: beyond it a walker uses
@@ -1147,8 +1149,7 @@ __device__ bool k2_stage_global(const DevWalker &w, const DevModel &m)
     return __ballot(bad) == 0;
 }
 
-// Worm update of walker wi.  LDS layout  rbuf u64[128] | records |
-// mask u64[nch] | base u32[nch].  RecG reads links through the scalar cache, which is
+// Worm update of walker wi.  LDS layout  Philox words u64[64] | records.  RecG reads links through the scalar cache, which is
 // valid because K1 wrote them in an earlier kernel launch.
 __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
                                           int full_sweep, int lds_bytes, int rec_mode, char *lds)
@@ -1160,9 +1161,8 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
     const DevModel &m = models[w.model];
     const bool therm = w.sweeps > w.thermalization;
     uint64_t *rbuf = (uint64_t *)lds;
-    WaveStream128 rs;
+    K2Stream rs;
     rs.init(rbuf, w.key, w.ctr);
-    const int64_t nch = (w.L + 63) / 64;
     const bool small_codes = ((int64_t)m.n_types << m.dt_lgp) <= 256;
     const bool u16_codes = ((int64_t)m.n_types << m.dt_lgp) <= 65536;
     int fmt = 0;
@@ -1172,80 +1172,60 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
         fmt = 10;
     else if ((rec_mode <= 1 || rec_mode == 3) && u16_codes && k2_lds_need(w.n, w.L, 2) <= (size_t)lds_bytes)
         fmt = 2;
+    // chunk masks and bases stay in global memory (worm starts and write-back)
+    const gmem<uint64_t> *gmask = as_global((const uint64_t *)rec_mask(w));
+    const gmem<uint32_t> *gbase = as_global((const uint32_t *)rec_base(w));
     double total = 1.0;
     int64_t scat = 0;
     int err = 0;
+    // records at a fixed offset after the Philox words, so the chase addresses
+    // them with the offset folded into the LDS instruction
+    char *q = lds + 8 * K2Stream::kNW;
     if (fmt == 2) {
-        char *q = lds + 1024;
         lds_u16 *lec = (lds_u16 *)(uint16_t *)q;
-        q += (w.n * 2 + 15) & ~(int64_t)15;
-        uint64_t *lmask = (uint64_t *)q;
-        q += nch * 8;
-        uint32_t *lbase = (uint32_t *)q;
-        const uint64_t *gmask = rec_mask(w);
-        const uint32_t *gbase = rec_base(w);
-        for (int64_t i = lane; i < nch; i += 64) {
-            lmask[i] = gmask[i];
-            lbase[i] = gbase[i];
-        }
         const bool ok = k2_stage_codes(w, m, lec);
         __syncthreads();
         if (!ok) {
             err = SSE_ERR_STATE;
         } else {
             RecH rec{lec, (const_u4 *)(const void *)rec_link(w)};
-            err = m.dt_mw == 1 ? worm_chase<RecH, true>(w, m, rs, rec, lmask, lbase, total, scat)
-                               : worm_chase<RecH, false>(w, m, rs, rec, lmask, lbase, total, scat);
+            err = m.dt_mw == 1 ? worm_chase<RecH, true>(w, m, rs, rec, gmask, gbase, total, scat)
+                               : worm_chase<RecH, false>(w, m, rs, rec, gmask, gbase, total, scat);
             __syncthreads();
             if (!err)
-                worm_finish(w, m, rs, rec, lmask, lbase);
+                worm_finish(w, m, rs, rec, gmask, gbase);
         }
     } else if (fmt) {
-        // records first, at a fixed offset, so the chase addresses them with
-        // the offset folded into the LDS instruction
-        char *q = lds + 1024;
         lds_u64 *llk = (lds_u64 *)(uint64_t *)q;
-        q += w.n * 8;
-        lds_u16 *lec = (lds_u16 *)(uint16_t *)q;
-        if (fmt == 10)
-            q += (w.n * 2 + 15) & ~(int64_t)15;
-        uint64_t *lmask = (uint64_t *)q;
-        q += nch * 8;
-        uint32_t *lbase = (uint32_t *)q;
-        const uint64_t *gmask = rec_mask(w);
-        const uint32_t *gbase = rec_base(w);
-        for (int64_t i = lane; i < nch; i += 64) {
-            lmask[i] = gmask[i];
-            lbase[i] = gbase[i];
-        }
+        lds_u16 *lec = (lds_u16 *)(uint16_t *)(q + w.n * 8);
         const bool ok = fmt == 8 ? k2_stage<8>(w, m, llk, lec) : k2_stage<10>(w, m, llk, lec);
         __syncthreads();
         if (!ok) {
             err = SSE_ERR_STATE;
         } else if (fmt == 8) {
             Rec8 rec{llk};
-            err = m.dt_mw == 1 ? worm_chase<Rec8, true>(w, m, rs, rec, lmask, lbase, total, scat)
-                               : worm_chase<Rec8, false>(w, m, rs, rec, lmask, lbase, total, scat);
+            err = m.dt_mw == 1 ? worm_chase<Rec8, true>(w, m, rs, rec, gmask, gbase, total, scat)
+                               : worm_chase<Rec8, false>(w, m, rs, rec, gmask, gbase, total, scat);
             __syncthreads();
             if (!err)
-                worm_finish(w, m, rs, rec, lmask, lbase);
+                worm_finish(w, m, rs, rec, gmask, gbase);
         } else {
             Rec10 rec{llk, lec};
-            err = m.dt_mw == 1 ? worm_chase<Rec10, true>(w, m, rs, rec, lmask, lbase, total, scat)
-                               : worm_chase<Rec10, false>(w, m, rs, rec, lmask, lbase, total, scat);
+            err = m.dt_mw == 1 ? worm_chase<Rec10, true>(w, m, rs, rec, gmask, gbase, total, scat)
+                               : worm_chase<Rec10, false>(w, m, rs, rec, gmask, gbase, total, scat);
             __syncthreads();
             if (!err)
-                worm_finish(w, m, rs, rec, lmask, lbase);
+                worm_finish(w, m, rs, rec, gmask, gbase);
         }
     } else {
         if (!k2_stage_global(w, m)) {
             err = SSE_ERR_STATE;
         } else {
             RecG rec{rec_vt(w), (const_u4 *)(const void *)rec_link(w)};
-            err = worm_chase<RecG, false>(w, m, rs, rec, rec_mask(w), rec_base(w), total, scat);
+            err = worm_chase<RecG, false>(w, m, rs, rec, gmask, gbase, total, scat);
             __syncthreads();
             if (!err)
-                worm_finish(w, m, rs, rec, rec_mask(w), rec_base(w));
+                worm_finish(w, m, rs, rec, gmask, gbase);
         }
     }
     if (err) {
@@ -1553,7 +1533,7 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
         if (((int64_t)m.n_types << m.dt_lgp) > 256)
             c.small_codes = false;
     }
-    c.lds_k2 = 1024 + 4096;
+    c.lds_k2 = 8 * K2Stream::kNW + 4096;
     if (c.lds_limit > 65536) {
         (void)hipFuncSetAttribute((const void *)wave_worm_kernel,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_limit);
@@ -1563,21 +1543,37 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
     return SSE_OK;
 }
 
-// Size the worm kernel's LDS for the largest walker seen (max n, max L), with
-// headroom; walkers that still do not fit use the global-memory chase.
+// Size the worm kernel's LDS for the largest walker of the last call (max n,
+// max L) with a little headroom; walkers that do not fit take a slower record
+// layout for that sweep.
+// The allocation is then widened to the largest size that keeps the same
+// number of walkers per CU (more headroom at no cost in residency).
 static inline void wave_resize_lds(WaveConfig &c, unsigned long long max_n, unsigned long long max_L)
 {
-    const int64_t nh = (int64_t)(max_n + max_n / 16 + 32);
+    if (c.lds_k2_fixed) {
+        c.lds_k2 = c.lds_k2_fixed;
+        return;
+    }
+    const int64_t nh = (int64_t)(max_n + 16);
     size_t need = k2_lds_need(nh, (int64_t)max_L, nh <= 4096 && c.small_codes && c.rec_mode == 0 ? 8
                                                   : c.rec_mode == 3                           ? 2
                                                                                               : 10);
     if (need > K2_LDS_CAP)  // larger walkers take the hybrid or global records
         need = K2_LDS_CAP;
-    need = (need + 511) & ~(size_t)511;
+    need = (need + 255) & ~(size_t)255;
     if (need > c.lds_limit)
         need = c.lds_limit;
-    if (need > c.lds_k2)
-        c.lds_k2 = need;
+    // Residency measured on MI355X: a workgroup takes round512(bytes + 256) of
+    // the CU's LDS and about 162816 bytes are usable per CU (26880 B runs 6
+    // walkers per CU, 27136-30720 B run 5, 32256 B and up only 4, whatever the
+    // occupancy query says).
+    const size_t per_cu = std::min(c.lds_limit, (size_t)162816);
+    auto alloc = [](size_t b) { return (b + 256 + 511) & ~(size_t)511; };
+    const size_t k = std::max((size_t)1, per_cu / alloc(need));  // walkers per CU
+    size_t wide = ((per_cu / k) & ~(size_t)511) - 256;
+    if (wide < need)
+        wide = need;
+    c.lds_k2 = wide;
 }
 
 static inline int wave_launch_init(WaveConfig &c, hipStream_t s, DevWalker *W, const DevModel *M,
