@@ -230,6 +230,39 @@ int sse_slot_poll(sse_batch *b, int32_t *done_slots, int32_t capacity, int32_t *
 /* Bytes of the slab pool in use (opstring + records of every walker).       */
 int sse_pool_usage(sse_batch *b, int64_t *used_bytes, int64_t *total_bytes);
 
+/* ---- parallel tempering in T ---------------------------------------------
+ * The reference's two exchange hooks, which Carlo's parallel-tempering loop
+ * calls per walker on the host:                                              */
+/* Carlo.parallel_tempering_log_weight_ratio(mc, :T, new_T) (src/sse.jl:390-396):
+ * -n log(new_T / T) of the walker's current configuration.                 */
+int sse_pt_log_weight_ratio(sse_batch *b, int32_t walker, double new_T, double *ratio);
+/* Carlo.parallel_tempering_change_parameter!(mc, :T, new_T) (src/sse.jl:398-405):
+ * set the walker's T; its configuration and worm controller stay.            */
+int sse_pt_change_parameter(sse_batch *b, int32_t walker, double new_T);
+/* The walker's current T (read back for checkpoints and checks).           */
+int sse_pt_get_parameter(sse_batch *b, int32_t walker, double *T);
+/* The exchange loop itself, on the device (replaces Carlo's host-side loop
+ * over those hooks; no reference counterpart in this package).  slot_walkers
+ * holds n_ladders ladders of ladder_len walkers each, in temperature-slot
+ * order; every walker at most once, one model per ladder, no serving slots.
+ * Round r proposes the neighbour pairs (k, k+1) with k % 2 == r % 2 of every
+ * ladder and accepts a pair with probability min(1, exp(lwr(a->T_b) +
+ * lwr(b->T_a))) (uniforms from a Philox stream keyed by seed); an accepted
+ * pair swaps the walkers' temperatures, slots and accumulator rows, so row w
+ * of sse_read_accumulators / sse_measure keeps collecting the temperature
+ * walker w was created with.  interval > 0 runs one round after every
+ * interval-th sweep of sse_sweep/sse_run (counted across calls); 0 leaves
+ * rounds to sse_pt_exchange.  n_ladders = 0 removes the ladders.  Counters
+ * and the round number restart at 0.                                        */
+int sse_pt_set_ladders(sse_batch *b, const int32_t *slot_walkers, int32_t n_ladders,
+                       int32_t ladder_len, uint64_t seed, int32_t interval);
+/* One exchange round now (on the batch stream).                             */
+int sse_pt_exchange(sse_batch *b);
+/* Current slot order, accepted and proposed swaps per lower slot (each
+ * n_ladders * ladder_len entries; may be NULL) and rounds run so far.       */
+int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
+                       int64_t *proposed, int64_t *rounds);
+
 /* ---- options -------------------------------------------------------------
  * SSE_OPT_GRAPH  (default 1): sse_run/sse_sweep replay a captured HIP graph of
  *                up to 32 sweeps (+ measures) instead of launching each kernel.

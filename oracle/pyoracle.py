@@ -67,6 +67,17 @@ def lib(capi):
     L.so_run_series.argtypes = [vp, C.c_int64, C.POINTER(C.c_double)]
     L.so_run_series.restype = C.c_int64
     L.so_work_counters.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.so_log.argtypes = [C.c_double]
+    L.so_log.restype = C.c_double
+    L.so_pt_log_weight_ratio.argtypes = [vp, C.c_double]
+    L.so_pt_log_weight_ratio.restype = C.c_double
+    L.so_pt_change_parameter.argtypes = [vp, C.c_double]
+    L.so_pt_get_T.argtypes = [vp]
+    L.so_pt_get_T.restype = C.c_double
+    L.so_pt_uniform.argtypes = [C.c_uint64, C.c_int32, C.c_int64, C.c_int32]
+    L.so_pt_uniform.restype = C.c_double
+    L.so_pt_exchange.argtypes = [C.POINTER(vp), C.POINTER(C.c_int32), C.c_int32, C.c_uint64,
+                                 C.c_int32, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     _lib = L
     return L
 
@@ -160,6 +171,18 @@ class OracleWalker:
         self.L.so_work_counters(self.h, _p(out, C.c_int64))
         return tuple(int(x) for x in out)
 
+    def pt_log_weight_ratio(self, new_T):
+        """Carlo.parallel_tempering_log_weight_ratio (sse.jl:390-396)."""
+        return self.L.so_pt_log_weight_ratio(self.h, new_T)
+
+    def pt_change_parameter(self, new_T):
+        """Carlo.parallel_tempering_change_parameter! (sse.jl:398-405)."""
+        self.L.so_pt_change_parameter(self.h, new_T)
+
+    @property
+    def T(self):
+        return self.L.so_pt_get_T(self.h)
+
     def run_timed(self, n_sweeps, measure=True):
         sums = np.zeros(self.flat.measure_width)
         t = self.L.so_run_timed(self.h, n_sweeps, int(measure), _p(sums, C.c_double))
@@ -186,3 +209,23 @@ def philox_block(capi, ctr, key):
     o = (C.c_uint32 * 4)()
     L_.so_philox4x32_10(c, k, o)
     return tuple(o)
+
+
+def log(capi, x):
+    """The series logarithm both sides use in the exchange test."""
+    return lib(capi).so_log(x)
+
+
+def pt_uniform(capi, seed, ladder, rnd, slot):
+    return lib(capi).so_pt_uniform(seed, ladder, rnd, slot)
+
+
+def pt_exchange(capi, walkers, slot_walker, seed, ladder, rnd, accepted, proposed):
+    """One exchange round over one ladder; slot_walker, accepted and proposed
+    (int32 / int64 numpy arrays) are updated in place."""
+    L_ = lib(capi)
+    hs = (C.c_void_p * len(walkers))(*[w.h for w in walkers])
+    assert slot_walker.dtype == np.int32 and accepted.dtype == np.int64
+    assert proposed.dtype == np.int64
+    L_.so_pt_exchange(hs, _p(slot_walker, C.c_int32), len(slot_walker), seed, ladder, rnd,
+                      _p(accepted, C.c_int64), _p(proposed, C.c_int64))

@@ -121,6 +121,34 @@ double so_tanh(double x)
     return copysign(t, x);
 }
 
+/* Natural logarithm for the exchange test of parallel tempering, restated
+ * from the textbook series log(m) = 2 atanh(s) = 2 sum s^(2j+1)/(2j+1),
+ * s = (m-1)/(m+1), after splitting x = m 2^e with m in (sqrt(1/2), sqrt(2)].
+ * |s| <= 0.1716, so twelve terms leave < 1e-19 relative.  The GPU evaluates
+ * the same operation sequence, so both sides decide identically.  x must be
+ * positive, finite and normal.                                             */
+double so_log(double x)
+{
+    uint64_t bits;
+    memcpy(&bits, &x, sizeof bits);
+    int e = (int)((bits >> 52) & 0x7ff) - 1023;
+    uint64_t mb = (bits & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    double m;
+    memcpy(&m, &mb, sizeof m);
+    if (m > 0x1.6a09e667f3bcdp+0) {
+        m = m * 0.5;
+        e += 1;
+    }
+    double s = (m - 1.0) / (m + 1.0);
+    double z = s * s;
+    double p = 1.0 / 23.0;
+    for (int k = 21; k >= 1; k -= 2)
+        p = p * z + 1.0 / (double)k;
+    double lm = 2.0 * s * p;
+    double ef = (double)e;
+    return ef * LN2_HI + (lm + ef * LN2_LO);
+}
+
 /* ------------------------------------------------------------ walker */
 struct so_walker {
     const sse_model_desc *m;
@@ -581,4 +609,65 @@ void so_work_counters(so_walker *w, int64_t *out4)
     out4[2] = w->acc_W;
     out4[3] = w->acc_sweeps;
     w->acc_L = w->acc_n = w->acc_W = w->acc_sweeps = 0;
+}
+
+/* ------------------------------------------------------------ parallel tempering */
+/* Carlo.parallel_tempering_log_weight_ratio (sse.jl:390-396): moving this
+ * walker's configuration from T to new_T changes its log weight by
+ * -n log(new_T / T).                                                       */
+double so_pt_log_weight_ratio(const so_walker *w, double new_T)
+{
+    return -(double)w->n * so_log(new_T / w->T);
+}
+
+/* Carlo.parallel_tempering_change_parameter! (sse.jl:398-405): only T
+ * changes; the configuration and the worm controller stay.                */
+void so_pt_change_parameter(so_walker *w, double new_T) { w->T = new_T; }
+
+double so_pt_get_T(const so_walker *w) { return w->T; }
+
+/* Uniform of the exchange test of pair (slot, slot+1) of one ladder in one
+ * round: Philox4x32-10 block (round lo, round hi, slot | 2^31, ladder) under
+ * the ladder seed; words 0|1 -> (w >> 11) 2^-53.  The set bit 31 keeps the
+ * block apart from every walker stream (whose third word is 0).            */
+double so_pt_uniform(uint64_t seed, int32_t ladder, int64_t round, int32_t slot)
+{
+    uint32_t ctr[4] = {(uint32_t)round, (uint32_t)((uint64_t)round >> 32),
+                       (uint32_t)slot | 0x80000000u, (uint32_t)ladder};
+    uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    so_philox4x32_10(ctr, k, o);
+    return so_word_to_float((uint64_t)o[0] | ((uint64_t)o[1] << 32));
+}
+
+/* One replica-exchange round (Hukushima-Nemoto, neighbour pairs of one
+ * parity) over one temperature ladder, the exchange step Carlo drives
+ * through the two hooks above.  slot_walker[k] is the walker at temperature
+ * slot k.  Round r proposes the pairs (k, k+1) with k % 2 == r % 2; the
+ * swap is accepted iff log(u) < lwr(a -> T_b) + lwr(b -> T_a), i.e. with
+ * probability min(1, exp(sum)); an accepted swap exchanges the walkers'
+ * temperatures (change_parameter on both) and their slots.  accepted and
+ * proposed count per lower slot.  Walkers in an error state are skipped.   */
+void so_pt_exchange(so_walker *const *walkers, int32_t *slot_walker, int32_t len, uint64_t seed,
+                    int32_t ladder, int64_t round, int64_t *accepted, int64_t *proposed)
+{
+    for (int32_t k = (int32_t)(round & 1); k + 1 < len; k += 2) {
+        so_walker *a = walkers[slot_walker[k]];
+        so_walker *b = walkers[slot_walker[k + 1]];
+        if (a->error || b->error)
+            continue;
+        double Ta = a->T, Tb = b->T;
+        double d = so_pt_log_weight_ratio(a, Tb) + so_pt_log_weight_ratio(b, Ta);
+        double u = so_pt_uniform(seed, ladder, round, k);
+        int acc = d >= 0.0 || u == 0.0 || so_log(u) < d;
+        proposed[k] += 1;
+        if (acc) {
+            accepted[k] += 1;
+            so_pt_change_parameter(a, Tb);
+            so_pt_change_parameter(b, Ta);
+            int32_t t = slot_walker[k];
+            slot_walker[k] = slot_walker[k + 1];
+            slot_walker[k + 1] = t;
+        }
+    }
 }

@@ -49,6 +49,15 @@ double so_run_timed(so_walker *w, int64_t n_sweeps, int measure, double *sums);
 int64_t so_run_series(so_walker *w, int64_t n_sweeps, double *out);
 void so_work_counters(so_walker *w, int64_t *out4);
 
+/* Parallel tempering in T (sse.jl:390-405) and one exchange round. */
+double so_log(double x);
+double so_pt_log_weight_ratio(const so_walker *w, double new_T);
+void so_pt_change_parameter(so_walker *w, double new_T);
+double so_pt_get_T(const so_walker *w);
+double so_pt_uniform(uint64_t seed, int32_t ladder, int64_t round, int32_t slot);
+void so_pt_exchange(so_walker *const *walkers, int32_t *slot_walker, int32_t len, uint64_t seed,
+                    int32_t ladder, int64_t round, int64_t *accepted, int64_t *proposed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -203,6 +203,14 @@ def load_library(path=None):
     lib.sse_set_option.argtypes = [vp, C.c_int32, C.c_int64]
     lib.sse_slot_poll.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]
     lib.sse_pool_usage.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    lib.sse_pt_log_weight_ratio.argtypes = [vp, C.c_int32, C.c_double, C.POINTER(C.c_double)]
+    lib.sse_pt_change_parameter.argtypes = [vp, C.c_int32, C.c_double]
+    lib.sse_pt_get_parameter.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    lib.sse_pt_set_ladders.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_uint64,
+                                       C.c_int32]
+    lib.sse_pt_exchange.argtypes = [vp]
+    lib.sse_pt_get_ladders.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.sse_last_error.restype = C.c_char_p
     lib.sse_abi_version.restype = C.c_int32
     if path is None:
@@ -217,7 +225,8 @@ EXPORTED_SYMBOLS = (
     "sse_phase_times", "sse_work_counters", "sse_opcode_to_reference",
     "sse_opcode_from_reference", "sse_last_error", "sse_abi_version", "sse_add_model",
     "sse_slot_start", "sse_slot_stop", "sse_slot_poll", "sse_pool_usage", "sse_set_option",
-    "sse_call_time", "sse_lds_bytes")
+    "sse_call_time", "sse_lds_bytes", "sse_pt_log_weight_ratio", "sse_pt_change_parameter",
+    "sse_pt_get_parameter", "sse_pt_set_ladders", "sse_pt_exchange", "sse_pt_get_ladders")
 
 
 class SSEError(RuntimeError):
@@ -395,6 +404,48 @@ class Batch:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int32()
         check(self.lib, self.lib.sse_lds_bytes(self.handle, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    # ---- parallel tempering in T (src/sse.jl:390-405 + the exchange loop)
+    def pt_log_weight_ratio(self, walker, new_T):
+        """Carlo.parallel_tempering_log_weight_ratio(mc, :T, new_T): -n log(new_T/T)."""
+        r = C.c_double()
+        check(self.lib, self.lib.sse_pt_log_weight_ratio(self.handle, walker, new_T, C.byref(r)))
+        return r.value
+
+    def pt_change_parameter(self, walker, new_T):
+        """Carlo.parallel_tempering_change_parameter!(mc, :T, new_T)."""
+        check(self.lib, self.lib.sse_pt_change_parameter(self.handle, walker, new_T))
+
+    def pt_get_parameter(self, walker):
+        t = C.c_double()
+        check(self.lib, self.lib.sse_pt_get_parameter(self.handle, walker, C.byref(t)))
+        return t.value
+
+    def pt_set_ladders(self, ladders, seed, interval=1):
+        """ladders: n_ladders x ladder_len walker indices in temperature-slot
+        order (an empty list removes the ladders)."""
+        lad = np.ascontiguousarray(np.asarray(ladders, dtype=np.int32).reshape(len(ladders), -1)
+                                   if len(ladders) else np.zeros((0, 0), np.int32))
+        self._pt_shape = lad.shape
+        check(self.lib, self.lib.sse_pt_set_ladders(self.handle, _ptr(lad, C.c_int32),
+                                                    lad.shape[0], lad.shape[1] if lad.size else 0,
+                                                    seed, interval))
+
+    def pt_exchange(self):
+        check(self.lib, self.lib.sse_pt_exchange(self.handle))
+
+    def pt_ladders(self):
+        """(slot_walkers, accepted, proposed) as n_ladders x ladder_len arrays,
+        and the number of rounds run."""
+        shape = getattr(self, "_pt_shape", (0, 0))
+        sw = np.zeros(shape, dtype=np.int32)
+        acc = np.zeros(shape, dtype=np.int64)
+        prop = np.zeros(shape, dtype=np.int64)
+        r = C.c_int64()
+        check(self.lib, self.lib.sse_pt_get_ladders(self.handle, _ptr(sw, C.c_int32),
+                                                    _ptr(acc, C.c_int64), _ptr(prop, C.c_int64),
+                                                    C.byref(r)))
+        return sw, acc, prop, r.value
 
     def work_counters(self):
         a = [C.c_int64() for _ in range(4)]

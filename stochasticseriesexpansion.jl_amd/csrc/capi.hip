@@ -17,6 +17,7 @@
 #include "sse_common.h"
 #include "sse_lane.h"
 #include "sse_wave.h"
+#include "sse_pt.h"
 
 namespace {
 
@@ -88,6 +89,15 @@ struct sse_batch {
     double call_ms = 0;
     int64_t call_sweeps = 0;
     WaveConfig wave;
+    // replica exchange in T: ladders of walkers in slot order, one exchange
+    // round after every pt_interval-th sweep (0: only on sse_pt_exchange)
+    int32_t pt_ladders = 0, pt_len = 0, pt_interval = 0, pt_since = 0;
+    uint64_t pt_seed = 0;
+    int64_t pt_round = 0;
+    size_t pt_cap = 0;
+    int32_t *d_pt_slot = nullptr;
+    int64_t *d_pt_acc = nullptr, *d_pt_prop = nullptr;
+    std::vector<uint8_t> pt_member;
 };
 
 template <class T>
@@ -634,6 +644,7 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
             x.last = b->d_last + (size_t)w * b->max_sites;
             x.meas_offset = b->offsets[w];
             x.meas_width = b->widths[w];
+            x.acc_slot = w;
             x.active = 0;
             x.pending = 1;
             x.remaining = -1;
@@ -921,14 +932,45 @@ int sse_init(sse_batch *b)
     return SSE_OK;
 }
 
-int sse_sweep(sse_batch *b, int32_t n_sweeps)
+static int launch_pt_exchange(sse_batch *b);
+
+// The sweeps of one call between exchanges: grouped streams, per-sweep timing
+// events (event slots from k0), graph replays or plain launches.
+static int issue_sweeps(sse_batch *b, int n_sweeps, int measure, bool grouped, bool timed, int k0)
 {
-    if (!b)
-        return fail(SSE_ERR_INVALID, "null batch");
-    if (!b->inited)
-        return fail(SSE_ERR_STATE, "sse_sweep before sse_init");
-    if (n_sweeps < 0)
-        return fail(SSE_ERR_INVALID, "negative sweep count");
+    if (grouped)
+        return groups_run(b, n_sweeps, measure);
+    if (timed) {
+        for (int k = 0; k < n_sweeps; ++k) {
+            hipEvent_t *e = &b->evpool[4 * (size_t)(k0 + k)];
+            HIPCHECK(hipEventRecord(e[0], b->stream));
+            int rc = launch_sweep(b, 7, e);
+            if (!rc && measure)
+                rc = launch_measure(b, 1);
+            if (rc)
+                return rc;
+            HIPCHECK(hipEventRecord(e[3], b->stream));
+        }
+        return SSE_OK;
+    }
+    if (b->use_graph)
+        return graph_run_many(b, n_sweeps, measure);
+    for (int k = 0; k < n_sweeps; ++k) {
+        int rc = launch_sweep(b, 7);
+        if (!rc && measure)
+            rc = launch_measure(b, 1);
+        if (rc)
+            return rc;
+    }
+    return SSE_OK;
+}
+
+// Body of sse_sweep (measure 0) and sse_run: n_sweeps sweeps of every active
+// walker, each followed by the measurement of the thermalized ones when
+// measure is set, and an exchange round after every pt_interval-th sweep
+// (counted across calls) when temperature ladders are set.
+static int sweep_call(sse_batch *b, int32_t n_sweeps, int32_t measure)
+{
     if (b->any_pending) {
         int rc = launch_init_pending(b);
         if (rc)
@@ -936,36 +978,34 @@ int sse_sweep(sse_batch *b, int32_t n_sweeps)
     }
     const bool grouped = b->mode == SSE_MODE_WAVE && b->n_groups > 1 && b->n_walkers > 1;
     const bool timed = !grouped && b->use_timing && n_sweeps <= 256;
+    const bool pt = b->pt_ladders > 0 && b->pt_interval > 0;
     int rc0 = call_timer(b, 0, 0);
     if (rc0)
         return rc0;
-    if (grouped) {
-        int rc = groups_run(b, n_sweeps, 0);
-        if (rc)
-            return rc;
-    } else if (timed) {
+    if (timed) {
         int rc = ensure_events(b, 4 * (size_t)n_sweeps);
         if (rc)
             return rc;
-        for (int k = 0; k < n_sweeps; ++k) {
-            hipEvent_t *e = &b->evpool[4 * (size_t)k];
-            HIPCHECK(hipEventRecord(e[0], b->stream));
-            int rc2 = launch_sweep(b, 7, e);
-            if (rc2)
-                return rc2;
-        }
-    } else if (b->use_graph) {
-        int rc = graph_run_many(b, n_sweeps, 0);
+    }
+    for (int done = 0; done < n_sweeps;) {
+        int seg = n_sweeps - done;
+        if (pt)
+            seg = std::min(seg, b->pt_interval - b->pt_since);
+        int rc = issue_sweeps(b, seg, measure ? 1 : 0, grouped, timed, done);
         if (rc)
             return rc;
-    } else {
-        for (int k = 0; k < n_sweeps; ++k) {
-            int rc = launch_sweep(b, 7);
-            if (rc)
-                return rc;
+        done += seg;
+        if (pt) {
+            b->pt_since += seg;
+            if (b->pt_since >= b->pt_interval) {
+                b->pt_since = 0;
+                rc = launch_pt_exchange(b);
+                if (rc)
+                    return rc;
+            }
         }
     }
-    if (b->any_budget) {
+    if (!measure && b->any_budget) {
         hipLaunchKernelGGL(retire_kernel, dim3((b->n_walkers + 63) / 64), dim3(64), 0, b->stream,
                            b->d_walkers, b->n_walkers, b->d_done);
         HIPCHECK(hipGetLastError());
@@ -977,10 +1017,21 @@ int sse_sweep(sse_batch *b, int32_t n_sweeps)
     if (!rc)
         rc = call_timer(b, 2, 0);
     if (timed)
-        collect_phase_times(b, n_sweeps, false);
+        collect_phase_times(b, n_sweeps, measure != 0);
     else
         clear_phase_times(b);
     return rc;
+}
+
+int sse_sweep(sse_batch *b, int32_t n_sweeps)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    if (!b->inited)
+        return fail(SSE_ERR_STATE, "sse_sweep before sse_init");
+    if (n_sweeps < 0)
+        return fail(SSE_ERR_INVALID, "negative sweep count");
+    return sweep_call(b, n_sweeps, 0);
 }
 
 int sse_sweep_phases(sse_batch *b, int32_t phases)
@@ -1043,63 +1094,9 @@ int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure)
         return fail(SSE_ERR_INVALID, "null batch");
     if (!b->inited)
         return fail(SSE_ERR_STATE, "run before init");
-    if (b->any_pending) {
-        int rc = launch_init_pending(b);
-        if (rc)
-            return rc;
-    }
-    const bool grouped = b->mode == SSE_MODE_WAVE && b->n_groups > 1 && b->n_walkers > 1;
-    const bool timed = !grouped && b->use_timing && n_sweeps <= 256;
-    int rc0 = call_timer(b, 0, 0);
-    if (rc0)
-        return rc0;
-    if (grouped) {
-        int rc = groups_run(b, n_sweeps, measure ? 1 : 0);
-        if (rc)
-            return rc;
-    } else if (timed) {
-        int rc = ensure_events(b, 4 * (size_t)n_sweeps);
-        if (rc)
-            return rc;
-        for (int k = 0; k < n_sweeps; ++k) {
-            hipEvent_t *e = &b->evpool[4 * (size_t)k];
-            HIPCHECK(hipEventRecord(e[0], b->stream));
-            int rc2 = launch_sweep(b, 7, e);
-            if (!rc2 && measure)
-                rc2 = launch_measure(b, 1);
-            if (rc2)
-                return rc2;
-            HIPCHECK(hipEventRecord(e[3], b->stream));
-        }
-    } else if (b->use_graph) {
-        int rc = graph_run_many(b, n_sweeps, measure ? 1 : 0);
-        if (rc)
-            return rc;
-    } else {
-        for (int k = 0; k < n_sweeps; ++k) {
-            int rc = launch_sweep(b, 7);
-            if (!rc && measure)
-                rc = launch_measure(b, 1);
-            if (rc)
-                return rc;
-        }
-    }
-    if (!measure && b->any_budget) {
-        hipLaunchKernelGGL(retire_kernel, dim3((b->n_walkers + 63) / 64), dim3(64), 0, b->stream,
-                           b->d_walkers, b->n_walkers, b->d_done);
-        HIPCHECK(hipGetLastError());
-    }
-    int rc1 = call_timer(b, 1, n_sweeps);
-    if (rc1)
-        return rc1;
-    int rc = check_walker_errors(b);
-    if (!rc)
-        rc = call_timer(b, 2, 0);
-    if (timed)
-        collect_phase_times(b, n_sweeps, measure != 0);
-    else
-        clear_phase_times(b);
-    return rc;
+    if (n_sweeps < 0)
+        return fail(SSE_ERR_INVALID, "negative sweep count");
+    return sweep_call(b, n_sweeps, measure);
 }
 
 int sse_read_accumulators(sse_batch *b, double *sums, double *sumsq, int64_t *counts)
@@ -1363,6 +1360,8 @@ int sse_slot_start(sse_batch *b, int32_t slot, const sse_walker_desc *wd, int64_
     int width = 5 + 5 * m.n_est;
     if (width > b->widths[slot])
         return fail(SSE_ERR_INVALID, "sse_slot_start: model has more estimators than the slot");
+    if (!b->pt_member.empty() && b->pt_member[slot])
+        return fail(SSE_ERR_INVALID, "sse_slot_start: slot is in a temperature ladder");
     HIPCHECK(hipStreamSynchronize(b->stream));
     int rc = sync_pool_from_device(b);
     if (rc)
@@ -1414,7 +1413,9 @@ int sse_slot_start(sse_batch *b, int32_t slot, const sse_walker_desc *wd, int64_
     x.error = 0;
     x.n_sites = m.n_sites;
     x.acc_L = x.acc_n = x.acc_W = x.acc_sweeps = 0;
+    x.meas_offset = b->offsets[slot];
     x.meas_width = width;
+    x.acc_slot = slot;
     x.active = 0;
     x.pending = 1;
     x.retire = 0;
@@ -1585,6 +1586,188 @@ int sse_work_counters(sse_batch *b, int64_t *L_sum, int64_t *n_sum, int64_t *W_s
         *W_sum = d;
     if (walker_sweeps)
         *walker_sweeps = e;
+    return SSE_OK;
+}
+
+// ------------------------------------------------------------ parallel tempering
+
+static int launch_pt_exchange(sse_batch *b)
+{
+    const int pairs = b->pt_ladders * (b->pt_len / 2);
+    if (pairs <= 0)
+        return SSE_OK;
+    hipLaunchKernelGGL(pt_exchange_kernel, dim3((pairs + 63) / 64), dim3(64), 0, b->stream,
+                       b->d_walkers, b->d_pt_slot, b->d_pt_acc, b->d_pt_prop, b->pt_ladders,
+                       b->pt_len, b->pt_seed, b->pt_round);
+    HIPCHECK(hipGetLastError());
+    b->pt_round += 1;
+    return SSE_OK;
+}
+
+static void release_alloc(sse_batch *b, void *p)
+{
+    if (!p)
+        return;
+    for (size_t i = 0; i < b->allocs.size(); ++i)
+        if (b->allocs[i] == p) {
+            (void)hipFree(p);
+            b->allocs.erase(b->allocs.begin() + (long)i);
+            return;
+        }
+}
+
+int sse_pt_set_ladders(sse_batch *b, const int32_t *slot_walkers, int32_t n_ladders,
+                       int32_t ladder_len, uint64_t seed, int32_t interval)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    if (n_ladders < 0 || interval < 0 || (n_ladders > 0 && (ladder_len < 2 || !slot_walkers)))
+        return fail(SSE_ERR_INVALID, "sse_pt_set_ladders: bad ladder shape");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    if (n_ladders == 0) {
+        b->pt_ladders = 0;
+        b->pt_len = 0;
+        b->pt_member.assign(b->n_walkers, 0);
+        return SSE_OK;
+    }
+    const size_t total = (size_t)n_ladders * (size_t)ladder_len;
+    if (total > (size_t)b->n_walkers)
+        return fail(SSE_ERR_INVALID, "sse_pt_set_ladders: more ladder slots than walkers");
+    std::vector<DevWalker> h(b->n_walkers);
+    HIPCHECK(hipMemcpy(h.data(), b->d_walkers, sizeof(DevWalker) * b->n_walkers,
+                       hipMemcpyDeviceToHost));
+    std::vector<uint8_t> member(b->n_walkers, 0);
+    for (int32_t l = 0; l < n_ladders; ++l) {
+        const int32_t *sw = slot_walkers + (size_t)l * ladder_len;
+        for (int32_t k = 0; k < ladder_len; ++k) {
+            const int32_t w = sw[k];
+            if (w < 0 || w >= b->n_walkers)
+                return fail(SSE_ERR_INVALID, "sse_pt_set_ladders: walker index out of range");
+            if (member[w])
+                return fail(SSE_ERR_INVALID, "sse_pt_set_ladders: walker in two ladder slots");
+            member[w] = 1;
+            if (h[w].model != h[sw[0]].model || h[w].meas_width != h[sw[0]].meas_width)
+                return fail(SSE_ERR_INVALID,
+                            "sse_pt_set_ladders: walkers of one ladder must share one model");
+            if (b->wdesc[w].model != h[w].model || h[w].remaining >= 0)
+                return fail(SSE_ERR_INVALID,
+                            "sse_pt_set_ladders: serving slots cannot join a ladder");
+        }
+    }
+    if (total > b->pt_cap) {
+        release_alloc(b, b->d_pt_slot);
+        release_alloc(b, b->d_pt_acc);
+        release_alloc(b, b->d_pt_prop);
+        b->d_pt_slot = nullptr;
+        b->d_pt_acc = b->d_pt_prop = nullptr;
+        b->pt_cap = 0;
+        int rc = dev_alloc(b, &b->d_pt_slot, total);
+        if (!rc)
+            rc = dev_alloc(b, &b->d_pt_acc, total);
+        if (!rc)
+            rc = dev_alloc(b, &b->d_pt_prop, total);
+        if (rc)
+            return rc;
+        b->pt_cap = total;
+    }
+    HIPCHECK(hipMemcpy(b->d_pt_slot, slot_walkers, sizeof(int32_t) * total, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(b->d_pt_acc, 0, sizeof(int64_t) * total));
+    HIPCHECK(hipMemset(b->d_pt_prop, 0, sizeof(int64_t) * total));
+    b->pt_ladders = n_ladders;
+    b->pt_len = ladder_len;
+    b->pt_interval = interval;
+    b->pt_since = 0;
+    b->pt_seed = seed;
+    b->pt_round = 0;
+    b->pt_member = member;
+    return SSE_OK;
+}
+
+int sse_pt_exchange(sse_batch *b)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    if (b->pt_ladders <= 0)
+        return fail(SSE_ERR_STATE, "sse_pt_exchange: no temperature ladders set");
+    if (!b->inited)
+        return fail(SSE_ERR_STATE, "sse_pt_exchange before sse_init");
+    if (b->any_pending) {
+        int rc = launch_init_pending(b);
+        if (rc)
+            return rc;
+    }
+    int rc = launch_pt_exchange(b);
+    if (rc)
+        return rc;
+    return check_walker_errors(b);
+}
+
+int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted, int64_t *proposed,
+                       int64_t *rounds)
+{
+    if (!b)
+        return fail(SSE_ERR_INVALID, "null batch");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    const size_t total = (size_t)b->pt_ladders * (size_t)b->pt_len;
+    if (total) {
+        if (slot_walkers)
+            HIPCHECK(hipMemcpy(slot_walkers, b->d_pt_slot, sizeof(int32_t) * total,
+                               hipMemcpyDeviceToHost));
+        if (accepted)
+            HIPCHECK(hipMemcpy(accepted, b->d_pt_acc, sizeof(int64_t) * total,
+                               hipMemcpyDeviceToHost));
+        if (proposed)
+            HIPCHECK(hipMemcpy(proposed, b->d_pt_prop, sizeof(int64_t) * total,
+                               hipMemcpyDeviceToHost));
+    }
+    if (rounds)
+        *rounds = b->pt_round;
+    return SSE_OK;
+}
+
+static int read_walker(sse_batch *b, int32_t walker, DevWalker *w)
+{
+    if (!b || walker < 0 || walker >= b->n_walkers)
+        return fail(SSE_ERR_INVALID, "bad walker");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    HIPCHECK(hipMemcpy(w, b->d_walkers + walker, sizeof *w, hipMemcpyDeviceToHost));
+    return SSE_OK;
+}
+
+int sse_pt_log_weight_ratio(sse_batch *b, int32_t walker, double new_T, double *ratio)
+{
+    if (!ratio || !(new_T > 0) || !std::isfinite(new_T))
+        return fail(SSE_ERR_INVALID, "sse_pt_log_weight_ratio: bad arguments");
+    DevWalker w;
+    int rc = read_walker(b, walker, &w);
+    if (rc)
+        return rc;
+    *ratio = -(double)w.n * std::log(new_T / w.T);
+    return SSE_OK;
+}
+
+int sse_pt_change_parameter(sse_batch *b, int32_t walker, double new_T)
+{
+    if (!(new_T > 0) || !std::isfinite(new_T))
+        return fail(SSE_ERR_INVALID, "sse_pt_change_parameter: T must be positive and finite");
+    DevWalker w;
+    int rc = read_walker(b, walker, &w);
+    if (rc)
+        return rc;
+    w.T = new_T;
+    HIPCHECK(hipMemcpy(b->d_walkers + walker, &w, sizeof w, hipMemcpyHostToDevice));
+    return SSE_OK;
+}
+
+int sse_pt_get_parameter(sse_batch *b, int32_t walker, double *T)
+{
+    if (!T)
+        return fail(SSE_ERR_INVALID, "sse_pt_get_parameter: null output");
+    DevWalker w;
+    int rc = read_walker(b, walker, &w);
+    if (rc)
+        return rc;
+    *T = w.T;
     return SSE_OK;
 }
 

@@ -145,7 +145,8 @@ struct DevWalker {
     // (remaining >= 0) retires the walker after its last sweep + measure
     int32_t active, pending;
     int64_t remaining;
-    int32_t retire, slot_pad;
+    int32_t retire;
+    int32_t acc_slot;  // accumulator row (count index) of the walker's temperature slot
 };
 
 struct DevPool {
@@ -278,6 +279,55 @@ __device__ __forceinline__ double sse_tanh(double x)
     double e = sse_expm1(2.0 * ax);
     double t = e / (e + 2.0);
     return copysign(t, x);
+}
+
+// Natural logarithm of the parallel-tempering exchange test: x = m 2^e with
+// m in (sqrt(1/2), sqrt(2)], log(m) = 2 atanh(s) = 2 sum_j s^(2j+1)/(2j+1),
+// s = (m-1)/(m+1), twelve terms (|s| <= 0.1716).  Exactly this sequence of
+// IEEE operations, so the host-side checker decides every exchange the same
+// way.  x positive, finite, normal.
+__device__ __forceinline__ double sse_log(double x)
+{
+    const uint64_t bits = (uint64_t)__double_as_longlong(x);
+    int e = (int)((bits >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((long long)((bits & 0x000fffffffffffffull) | 0x3ff0000000000000ull));
+    if (m > 0x1.6a09e667f3bcdp+0) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double z = s * s;
+    double p = 1.0 / 23.0;
+#pragma unroll
+    for (int k = 21; k >= 1; k -= 2)
+        p = p * z + 1.0 / (double)k;
+    const double lm = 2.0 * s * p;
+    const double ef = (double)e;
+    return ef * 0x1.62e42fee00000p-1 + (lm + ef * 0x1.a39ef35793c76p-33);
+}
+
+// Philox4x32-10 of a full 128-bit counter block.
+__device__ __forceinline__ void philox_block4(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint64_t key, uint32_t &o0, uint32_t &o1)
+{
+    uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    o0 = c0;
+    o1 = c1;
 }
 
 // Worm-number controller and WormLengthFraction (sse.jl:200-217).

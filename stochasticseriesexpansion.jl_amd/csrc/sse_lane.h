@@ -367,7 +367,7 @@ __global__ void lane_measure_kernel(DevWalker *W, const DevModel *models, double
         measure_walker_serial(w, m, o, scratch_spins + (size_t)wi * max_sites);
         if (mode == 1) {
             accumulate(sum + w.meas_offset, sumsq + w.meas_offset, o, w.meas_width);
-            count[wi] += 1;
+            count[w.acc_slot] += 1;
         }
     }
     if (mode == 1 && w.retire) {

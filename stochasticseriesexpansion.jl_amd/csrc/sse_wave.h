@@ -1476,7 +1476,7 @@ __device__ __forceinline__ void k3_walker(DevWalker *W, const DevModel *models, 
             sumsq[w.meas_offset + j] += x * x;
         }
         if (lane == 0) {
-            count[wi] += 1;
+            count[w.acc_slot] += 1;
             if (w.retire) {
                 W[wi].retire = 0;
                 W[wi].active = 0;
