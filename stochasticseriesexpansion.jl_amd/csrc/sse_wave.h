@@ -127,8 +127,97 @@ struct WaveStreamT {
 
 typedef WaveStreamT<256> WaveStream;
 typedef WaveStreamT<128> WaveStream128;
-// the worm kernel's stream: 64 words (512 bytes of LDS per walker)
-typedef WaveStreamT<64> K2Stream;
+// The worm kernel's Philox words in registers: lane l holds word bbase + l of
+// the walker's stream (bbase even) and that word's 53-bit mantissa
+// (word >> 11, the integer behind rand()).  A wave-uniform draw is two
+// v_readlane with a scalar index; lanes that draw different words read them
+// with ds_bpermute.  No LDS and no barrier; all members except the four
+// per-lane words are wave-uniform and every call is made by the whole wave.
+struct K2RegStream {
+    static constexpr uint32_t kNW = 64;
+    uint32_t wlo, whi, mlo, mhi;
+    uint64_t key, bbase;
+    uint32_t off;  // next word is bbase + off; off >= 1024 marks an empty buffer
+    __device__ void init(uint64_t k, uint64_t c)
+    {
+        key = k;
+        bbase = c;
+        off = 1024;
+    }
+    __device__ __forceinline__ void fill()
+    {
+        const uint32_t lane = threadIdx.x;
+        uint32_t o0, o1, o2, o3;
+        philox_block((bbase >> 1) + (lane >> 1), key, o0, o1, o2, o3);
+        wlo = (lane & 1u) ? o2 : o0;
+        whi = (lane & 1u) ? o3 : o1;
+        const uint64_t mm = (((uint64_t)whi << 32) | wlo) >> 11;
+        mlo = (uint32_t)mm;
+        mhi = (uint32_t)(mm >> 32);
+    }
+    __device__ __forceinline__ uint64_t ctr() const { return bbase + off - (off >= 1024 ? 1024 : 0); }
+    // make words [ctr, ctr+need) resident (need <= 63)
+    __device__ __forceinline__ void ensure(uint32_t need)
+    {
+        if (off >= 1024) {
+            const uint64_t c = bbase;
+            bbase = c & ~1ull;
+            off = (uint32_t)(c - bbase);
+            fill();
+        } else if (off + need > kNW) {
+            const uint64_t c = bbase + off;
+            bbase = c & ~1ull;
+            off = (uint32_t)(c - bbase);
+            fill();
+        }
+    }
+    // the buffer is used up (off == kNW): load the next kNW words
+    __device__ __forceinline__ void refill_next()
+    {
+        bbase += off;
+        off = 0;
+        fill();
+    }
+    // word k (wave-uniform, < kNW) of the buffer and its mantissa
+    __device__ __forceinline__ uint64_t word_at(uint32_t k) const
+    {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)wlo, (int)k);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)whi, (int)k);
+        return ((uint64_t)hi << 32) | lo;
+    }
+    // word k positions after the next one, k per lane (after ensure)
+    __device__ __forceinline__ uint64_t peek(uint32_t k) const
+    {
+        const int a = (int)((off + k) << 2);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)wlo);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)whi);
+        return ((uint64_t)hi << 32) | lo;
+    }
+    __device__ __forceinline__ void advance(uint32_t k) { off += k; }
+    __device__ __forceinline__ uint64_t next()
+    {
+        ensure(1);
+        const uint64_t x = word_at(off);
+        ++off;
+        return x;
+    }
+    __device__ __forceinline__ double next_float() { return word_to_float(next()); }
+    __device__ __forceinline__ uint64_t range(uint64_t s)
+    {
+        uint64_t x = next();
+        uint64_t lo = x * s, hi = __umul64hi(x, s);
+        if (lo < s) {
+            uint64_t t = (0ull - s) % s;
+            while (lo < t) {
+                x = next();
+                lo = x * s;
+                hi = __umul64hi(x, s);
+            }
+        }
+        return hi;
+    }
+};
+typedef K2RegStream K2Stream;
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ uint64_t uni64(uint64_t x)
@@ -854,7 +943,6 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
     const uint32_t lgp = uni((uint32_t)m.dt_lgp);
     const uint32_t estride = dmw << 8;  // bytes per ecode
     const_char *CT = (const_char *)(const void *)uni64((uint64_t)(const void *)m.dtab);
-    lds_u64 *RB = (lds_u64 *)rs.buf;
     total = 1.0;
     uint32_t sc = 0;
     for (int64_t k = 0; k < nworms; ++k) {
@@ -898,18 +986,18 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
                     rs.ensure(1);
             }
             const uint32_t off = uni(rs.off);
-            const uint64_t wv = RB[off];
             const uint32_t r = (uint32_t)(lid14 >> 16);
             const typename R::Word x = rec.load(r);
-            const uint32_t wlo = uni((uint32_t)wv), whi = uni((uint32_t)(wv >> 32));
+            // rand() = mm * 2^-53: the mantissa of this step's word
+            const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)rs.mlo, (int)off);
+            const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)rs.mhi, (int)off);
             rs.off = off + 1u;
             const uint32_t e = R::ecode(x);
             const uint32_t legbits = (uint32_t)(lid14 >> 8) & 0xC0u;  // leg_in << 6
             const uint32_t eoff = kOneWorm ? (e << 8) | legbits : e * estride + (wfm << 8) + legbits;
             ConstEntry *E = (ConstEntry *)(CT + eoff);
             uint32_t ft;
-            const uint64_t mm = (((uint64_t)whi << 32) | wlo) >> 11;  // rand() = mm * 2^-53
-            uint32_t eo = chase_select((uint32_t)mm, (uint32_t)(mm >> 32), E, ft);
+            uint32_t eo = chase_select(mlo, mhi, E, ft);
             if (ft) {
                 // more than four exits: walk the exit list (vertex_data.jl:113-125);
                 // a fall-through or an invalid transition is an error
@@ -924,7 +1012,7 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
                 } else {
                     const uint32_t trw = uni(E->trans);
                     const WormTab tab = worm_tab((const char *)m.wtab, m);
-                    const double xr = word_to_float(((uint64_t)whi << 32) | wlo);
+                    const double xr = (double)(((uint64_t)mhi << 32) | mlo) * 0x1.0p-53;
                     uint32_t ex = trw & 0xFFFFFu;
                     const uint32_t eend = ex + (trw >> 20);
                     for (;;) {
@@ -998,8 +1086,10 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &re
         uint8_t val = 0;
         if (D > 0) {
             rs.ensure((uint32_t)D);
+            // every lane takes part in the cross-lane read; lane s of the
+            // refreshed sites uses word off + (its rank among them)
+            const uint64_t x = rs.peek(fr ? (uint32_t)popc(frm & below_mask(lane)) : 0u);
             if (fr) {
-                uint64_t x = rs.peek((uint32_t)popc(frm & below_mask(lane)));
                 uint64_t lo = x * dim;
                 if (lo < dim && lo < (0ull - dim) % dim)
                     reject = true;
@@ -1065,7 +1155,7 @@ __device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &re
 // the Rec8 (fmt 8) or Rec10 (fmt 10) layout
 __host__ __device__ inline size_t k2_lds_need(int64_t n, int64_t, int fmt)
 {
-    const size_t b = 8 * K2Stream::kNW;  // Philox words
+    const size_t b = 0;  // the Philox words live in registers
     if (fmt == 2)                        // RecH: chase codes only
         return b + (((size_t)n * 2 + 15) & ~(size_t)15);
     return b + (size_t)n * 8 + (fmt == 10 ? (((size_t)n * 2 + 15) & ~(size_t)15) : 0);
@@ -1149,7 +1239,8 @@ __device__ bool k2_stage_global(const DevWalker &w, const DevModel &m)
     return __ballot(bad) == 0;
 }
 
-// Worm update of walker wi.  LDS layout  Philox words u64[64] | records.  RecG reads links through the scalar cache, which is
+// Worm update of walker wi.  LDS holds the records only (the Philox words are
+// in registers).  RecG reads links through the scalar cache, which is
 // valid because K1 wrote them in an earlier kernel launch.
 __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
                                           int full_sweep, int lds_bytes, int rec_mode, char *lds)
@@ -1160,9 +1251,8 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
         return;
     const DevModel &m = models[w.model];
     const bool therm = w.sweeps > w.thermalization;
-    uint64_t *rbuf = (uint64_t *)lds;
     K2Stream rs;
-    rs.init(rbuf, w.key, w.ctr);
+    rs.init(w.key, w.ctr);
     const bool small_codes = ((int64_t)m.n_types << m.dt_lgp) <= 256;
     const bool u16_codes = ((int64_t)m.n_types << m.dt_lgp) <= 65536;
     int fmt = 0;
@@ -1178,9 +1268,8 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
     double total = 1.0;
     int64_t scat = 0;
     int err = 0;
-    // records at a fixed offset after the Philox words, so the chase addresses
-    // them with the offset folded into the LDS instruction
-    char *q = lds + 8 * K2Stream::kNW;
+    // records at the start of the walker's LDS
+    char *q = lds;
     if (fmt == 2) {
         lds_u16 *lec = (lds_u16 *)(uint16_t *)q;
         const bool ok = k2_stage_codes(w, m, lec);
@@ -1533,7 +1622,7 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
         if (((int64_t)m.n_types << m.dt_lgp) > 256)
             c.small_codes = false;
     }
-    c.lds_k2 = 8 * K2Stream::kNW + 4096;
+    c.lds_k2 = 4096;
     if (c.lds_limit > 65536) {
         (void)hipFuncSetAttribute((const void *)wave_worm_kernel,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_limit);
