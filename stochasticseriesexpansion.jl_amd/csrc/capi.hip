@@ -148,7 +148,8 @@ static int validate_model(const sse_model_desc &m, int k)
     for (int t = 0; t < m.n_types; ++t) {
         const sse_vertex_data_desc &vd = m.types[t];
         if (vd.n_vertices <= 0 || vd.n_vertices > 512 || vd.dims[0] < 1 || vd.dims[1] < 1 ||
-            vd.dims[0] > 255 || vd.dims[1] > 255 || vd.max_worm_count < 1) {
+            vd.dims[0] > 255 || vd.dims[1] > 255 || vd.max_worm_count < 1 ||
+            vd.max_worm_count > 4096) {
             snprintf(buf, sizeof buf, "model %d type %d: unsupported table shape", k, t);
             return fail(SSE_ERR_INVALID, buf);
         }
@@ -281,10 +282,10 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
         ++lgp;
     const size_t P = (size_t)1 << lgp;
     std::vector<DirectEntry> dt((size_t)m.n_types * P * dmw * 4);
-    for (auto &x : dt) {  // invalid transition: every word falls through
+    for (auto &x : dt) {  // invalid transition: every word takes the exact path, which fails
         for (int j = 0; j < 4; ++j) {
             x.K[j] = 0;
-            x.out[j] = 0xFFFFFFFFu;
+            x.out[j] = kChaseSlow;
         }
         x.count = 0;
         x.trans = 0xFFFFFFFFu;
@@ -305,24 +306,28 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
                         x.count = (uint32_t)cnt;
                         x.trans = (uint32_t)(ebase + off) | ((uint32_t)cnt << 20);
                         x.pad[0] = x.pad[1] = 0;
-                        if (cnt > 4) {  // exits looked up in wtab: fall through to it
-                            for (int j = 0; j < 4; ++j) {
-                                x.K[j] = 0;
-                                x.out[j] = 0xFFFFFFFFu;
-                            }
-                        } else {
-                            for (int j = 0; j < 4; ++j) {
-                                const int jj = std::min(j, cnt - 1);
-                                const double c = vd.exit_cumprob[off + jj];
-                                // r < c  <=>  (word >> 11) < ceil(c * 2^53)
-                                x.K[j] = c <= 0.0 ? 0ull
-                                         : c >= 1.0 ? (1ull << 53)
-                                                    : (uint64_t)std::ceil(std::ldexp(c, 53));
-                                x.out[j] = j < cnt ? chase_out((uint32_t)t << lgp |
-                                                                   (uint32_t)vd.exit_target[off + j],
-                                                               vd.exit_leg[off + j], vd.exit_worm[off + j])
-                                                   : 0xFFFFFFFFu;
-                            }
+                        // r < c  <=>  (word >> 11) < ceil(c * 2^53)
+                        auto thr = [](double c) {
+                            return c <= 0.0 ? 0ull
+                                   : c >= 1.0 ? (1ull << 53)
+                                              : (uint64_t)std::ceil(std::ldexp(c, 53));
+                        };
+                        auto word = [&](int j) {
+                            return chase_out((uint32_t)t << lgp | (uint32_t)vd.exit_target[off + j],
+                                             vd.exit_leg[off + j], vd.exit_worm[off + j]);
+                        };
+                        // slots past the last exit: K = 2^53 (always taken), slow marker
+                        for (int j = 0; j < 4; ++j) {
+                            x.K[j] = j < cnt ? thr(vd.exit_cumprob[off + j]) : (1ull << 53);
+                            x.out[j] = j < cnt ? word(j) : kChaseSlow;
+                        }
+                        if (cnt > 4) {  // first three inline, the rest from the exit list
+                            x.K[3] = 1ull << 53;
+                            x.out[3] = kChaseSlow;
+                        } else if (cnt == 4) {
+                            x.pad[0] = x.out[3];
+                            if (x.K[3] < (1ull << 53))  // the fourth exit needs its compare
+                                x.out[3] = kChaseSlow;
                         }
                         for (int dg = 0; dg < 2; ++dg)
                             dt[((((size_t)t << lgp) + (2 * v + dg)) * dmw + wv) * 4 + l] = x;

@@ -78,13 +78,23 @@ struct DevModel {
 // outs are 0xFFFFFFFF, so m >= K[count-1] reaches the fall-through test.  An
 // invalid transition (count 0) or one with more than 4 exits has all K = 0:
 // every word falls through and the count decides.
+// The chase compares m with K[0..2] only and takes out[3] otherwise, so
+// out[j] holds kChaseSlow (send the step to the exact path) wherever that
+// would be wrong: slots past the last exit (K = 2^53 there), the fourth exit
+// when its threshold is below 2^53 (its word is then kept in pad[0]), and
+// out[3] of an entry with more than four exits (the first three are inline).
 struct __align__(16) DirectEntry {
     uint64_t K[4];     // integer thresholds ceil(cum[j] * 2^53)
-    uint32_t out[4];   // chase_out(target ecode, exit leg, exit worm)
+    uint32_t out[4];   // chase_out(target ecode, exit leg, exit worm) or kChaseSlow
     uint32_t count;    // number of exits
     uint32_t trans;    // wtab trans word (exit index | count << 20)
-    uint32_t pad[2];
+    uint32_t pad[2];   // pad[0]: the fourth exit's word when count == 4
 };
+
+// Marker exit word: worm-1 has bit 13 set, which no real exit word has (the
+// model check keeps every worm count below 8192).
+#define kChaseSlow 0xFFFFFFFFu
+#define kChaseSlowBit 0x2000u
 
 // exit word of the chase: worm-1 in bits 0..13, exit leg in 14..15, target
 // ecode (type << dt_lgp | vcode) in 16..31

@@ -896,31 +896,51 @@ struct RecH {
     __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
 };
 
-// Exit selection of one scatter on the scalar unit: m = word >> 11 against the
-// entry's thresholds by 64-bit borrow chains (s_sub/s_subb leave the borrow,
-// m < K, in SCC).  Returns out[j] for the first j with m < K[j]; ft = 1 when
-// m >= K[3] (fall-through, or an entry whose exits live in wtab).
-__device__ __forceinline__ uint32_t chase_select(uint32_t mlo, uint32_t mhi, ConstEntry *E, uint32_t &ft)
+// Record index of the leg id held shifted in lid14 (lid << 14).  For the
+// 8-byte LDS records it is produced on the vector unit (v_bfe), so the record's
+// LDS address takes no scalar instruction and no scalar-to-vector move: the
+// scalar unit is what the chase runs out of when several walkers share a
+// compute unit.
+template <class R>
+__device__ __forceinline__ uint32_t rec_index(typename R::Key lid14)
 {
+    return (uint32_t)(lid14 >> 16);
+}
+template <>
+__device__ __forceinline__ uint32_t rec_index<Rec8>(uint32_t lid14)
+{
+    uint32_t r;
+    asm("v_bfe_u32 %0, %1, 16, 16" : "=v"(r) : "s"(lid14));
+    return r;
+}
+
+// Exit selection of one scatter on the scalar unit: m = word >> 11 against the
+// entry's first three thresholds by 64-bit borrow chains (s_sub/s_subb leave
+// the borrow, m < K, in SCC), from the last to the first: out[j] for the first
+// j < 3 with m < K[j], else out[3].  The table puts the chase's slow marker
+// (kChaseSlow) wherever the answer is not simply out[3] (a fourth exit whose
+// threshold is below 2^53, more than four exits, a fall-through), so the
+// caller tests one bit of the result.
+__device__ __forceinline__ uint32_t chase_select(uint32_t mlo, uint32_t mhi, ConstEntry *E)
+{
+    // K[3] is loaded too (unused by the selection) so that the thresholds
+    // arrive in one 32-byte scalar load
     const uint64_t K0 = E->K[0], K1 = E->K[1], K2 = E->K[2], K3 = E->K[3];
     const uint32_t o0 = E->out[0], o1 = E->out[1], o2 = E->out[2], o3 = E->out[3];
     uint32_t eo, t;
-    asm("s_sub_u32 %1, %3, %5\n\t"
-        "s_subb_u32 %1, %4, %6\n\t"
-        "s_cselect_b32 %0, %13, %14\n\t"
-        "s_sub_u32 %1, %3, %7\n\t"
-        "s_subb_u32 %1, %4, %8\n\t"
-        "s_cselect_b32 %0, %0, %15\n\t"
-        "s_sub_u32 %1, %3, %9\n\t"
-        "s_subb_u32 %1, %4, %10\n\t"
-        "s_cselect_b32 %0, %0, %16\n\t"
-        "s_sub_u32 %1, %3, %11\n\t"
-        "s_subb_u32 %1, %4, %12\n\t"
-        "s_cselect_b32 %2, 0, 1"
-        : "=&s"(eo), "=&s"(t), "=s"(ft)
+    asm("s_sub_u32 %1, %2, %8\n\t"
+        "s_subb_u32 %1, %3, %9\n\t"
+        "s_cselect_b32 %0, %12, %13\n\t"
+        "s_sub_u32 %1, %2, %6\n\t"
+        "s_subb_u32 %1, %3, %7\n\t"
+        "s_cselect_b32 %0, %11, %0\n\t"
+        "s_sub_u32 %1, %2, %4\n\t"
+        "s_subb_u32 %1, %3, %5\n\t"
+        "s_cselect_b32 %0, %10, %0"
+        : "=&s"(eo), "=&s"(t)
         : "s"(uni(mlo)), "s"(uni(mhi)), "s"((uint32_t)K0), "s"((uint32_t)(K0 >> 32)), "s"((uint32_t)K1),
-          "s"((uint32_t)(K1 >> 32)), "s"((uint32_t)K2), "s"((uint32_t)(K2 >> 32)), "s"((uint32_t)K3),
-          "s"((uint32_t)(K3 >> 32)), "s"(o0), "s"(o1), "s"(o2), "s"(o3)
+          "s"((uint32_t)(K1 >> 32)), "s"((uint32_t)K2), "s"((uint32_t)(K2 >> 32)), "s"(o0), "s"(o1),
+          "s"(o2), "s"(o3), "s"((uint32_t)K3), "s"((uint32_t)(K3 >> 32))
         : "scc");
     return eo;
 }
@@ -986,7 +1006,7 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
                     rs.ensure(1);
             }
             const uint32_t off = uni(rs.off);
-            const uint32_t r = (uint32_t)(lid14 >> 16);
+            const uint32_t r = rec_index<R>(lid14);
             const typename R::Word x = rec.load(r);
             // rand() = mm * 2^-53: the mantissa of this step's word
             const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)rs.mlo, (int)off);
@@ -996,21 +1016,34 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
             const uint32_t legbits = (uint32_t)(lid14 >> 8) & 0xC0u;  // leg_in << 6
             const uint32_t eoff = kOneWorm ? (e << 8) | legbits : e * estride + (wfm << 8) + legbits;
             ConstEntry *E = (ConstEntry *)(CT + eoff);
-            uint32_t ft;
-            uint32_t eo = chase_select(mlo, mhi, E, ft);
-            if (ft) {
-                // more than four exits: walk the exit list (vertex_data.jl:113-125);
-                // a fall-through or an invalid transition is an error
-                const uint32_t cnt = uni(E->count);
+            uint32_t eo = chase_select(mlo, mhi, E);
+            if (eo & kChaseSlowBit) {
+                // the entry's address is formed here, not hoisted into the fast path
+                uint32_t eoff_s = eoff;
+                asm volatile("" : "+s"(eoff_s));
+                ConstEntry *ES = (ConstEntry *)(CT + eoff_s);
+                // the exact scatter (vertex_data.jl:113-125): the entry's own
+                // thresholds for up to four exits, the exit list for more; a
+                // fall-through or an invalid transition is an error
+                const uint32_t cnt = uni(ES->count);
                 // on failure: rewrite the unchanged code and leave the loop through
                 // the exit test (lid and the exit word are made to match key A)
                 const uint32_t efail = (e << 16) | (l0 << 14) | (dim0 - wf0 - 1u);
                 eo = efail;
                 if (cnt <= 4u) {
-                    err = SSE_ERR_SCATTER;
-                    lid14 = (Key)lid0 << 14;
+                    const uint64_t mm = ((uint64_t)mhi << 32) | mlo;
+                    uint32_t j = 0;
+                    for (; j < cnt; ++j)
+                        if (mm < uni64(ES->K[j]))
+                            break;
+                    if (j < cnt) {
+                        eo = uni(j < 3u ? ES->out[j] : ES->pad[0]);
+                    } else {
+                        err = SSE_ERR_SCATTER;
+                        lid14 = (Key)lid0 << 14;
+                    }
                 } else {
-                    const uint32_t trw = uni(E->trans);
+                    const uint32_t trw = uni(ES->trans);
                     const WormTab tab = worm_tab((const char *)m.wtab, m);
                     const double xr = (double)(((uint64_t)mhi << 32) | mlo) * 0x1.0p-53;
                     uint32_t ex = trw & 0xFFFFFu;
@@ -1035,10 +1068,11 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
             // closing tests (sse.jl:290-297) folded into one loop exit: the worm
             // ends leaving through (r0, l0, dim0 - wf0), or entering (r0, l0, wf0)
             const uint32_t nx = R::link(x, (eo >> 14) & 3u);
-            const uint32_t wfn = eo & 0x3FFFu;
-            const Key kA = (lid14 & ~(Key)0xC000u) | (eo & 0xFFFFu);
+            // one worm type: every exit word has worm-1 = 0
+            const uint32_t wfn = kOneWorm ? 0u : eo & 0x3FFFu;
+            const Key kA = (lid14 & ~(Key)0xC000u) | (eo & (kOneWorm ? 0xC000u : 0xFFFFu));
             const Key nx14 = (Key)nx << 14;
-            const Key kB = nx14 | wfn;
+            const Key kB = kOneWorm ? nx14 : nx14 | wfn;
             // one exit: d == 0 when either closing test holds (a failed scatter
             // above forced the exit test to hold)
             Key d = min(kA ^ keyA, kB ^ keyB);
