@@ -313,8 +313,9 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
                                               : (uint64_t)std::ceil(std::ldexp(c, 53));
                         };
                         auto word = [&](int j) {
-                            return chase_out((uint32_t)t << lgp | (uint32_t)vd.exit_target[off + j],
-                                             vd.exit_leg[off + j], vd.exit_worm[off + j]);
+                            const uint32_t ec = (uint32_t)t << lgp | (uint32_t)vd.exit_target[off + j];
+                            return dmw == 1 ? chase_out1(ec, vd.exit_leg[off + j])
+                                            : chase_out(ec, vd.exit_leg[off + j], vd.exit_worm[off + j]);
                         };
                         // slots past the last exit: K = 2^53 (always taken), slow marker
                         for (int j = 0; j < 4; ++j) {

@@ -102,6 +102,13 @@ __host__ __device__ inline uint32_t chase_out(uint32_t ecode, uint32_t leg, uint
 {
     return (worm - 1u) | (leg << 14) | (ecode << 16);
 }
+// exit word of a model with one worm type: worm-1 is always 0, so bits 0..5
+// carry 14 * exit leg instead, the shift that takes the exit leg's link out of
+// an 8-byte record
+__host__ __device__ inline uint32_t chase_out1(uint32_t ecode, uint32_t leg)
+{
+    return (14u * leg) | (leg << 14) | (ecode << 16);
+}
 __host__ __device__ inline uint32_t ecode_of_vt(uint32_t vt, uint32_t lgp)
 {
     return ((vt >> 10) << lgp) | (vt & 1023u);

@@ -23,6 +23,7 @@
 // K3 wave_measure_kernel: measure (sse.jl:70-87,305-376) with the estimator
 //   running sums as wave prefix sums.
 #pragma once
+#include <type_traits>
 #include <vector>
 
 #include "sse_common.h"
@@ -815,6 +816,14 @@ struct Rec8 {
         const uint64_t v = (uint64_t)x.lo | ((uint64_t)x.hi << 32);
         return (uint32_t)(v >> (14u * leg)) & 0x3FFFu;
     }
+    __device__ __forceinline__ static void set_link0(Word &x, uint32_t v) { x.lo = (x.lo & ~0x3FFFu) | v; }
+    // link of the exit leg when the exit word carries 14 * leg in its low six
+    // bits (one-worm tables): one 64-bit shift by the word itself
+    __device__ __forceinline__ static uint32_t link_sh(const Word &x, uint32_t eo)
+    {
+        const uint64_t v = (uint64_t)x.lo | ((uint64_t)x.hi << 32);
+        return (uint32_t)(v >> (eo & 63u)) & 0x3FFFu;
+    }
     // every lane writes the same byte (no exec-mask switch in the chase loop)
     __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const
     {
@@ -843,6 +852,7 @@ struct Rec10 {
         const uint64_t v = (uint64_t)x.lo | ((uint64_t)x.hi << 32);
         return (uint32_t)(v >> (16u * leg)) & 0xFFFFu;
     }
+    __device__ __forceinline__ static void set_link0(Word &x, uint32_t v) { x.lo = (x.lo & ~0xFFFFu) | v; }
     __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const { ec[r] = (uint16_t)e; }
     __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
 };
@@ -866,6 +876,7 @@ struct RecG {
     {
         return leg == 0 ? x.l.x : leg == 1 ? x.l.y : leg == 2 ? x.l.z : x.l.w;
     }
+    __device__ __forceinline__ static void set_link0(Word &x, uint32_t v) { x.l.x = v; }
     // every lane stores, so each lane's own later load is ordered after it
     __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const { ec[r] = e; }
     __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
@@ -892,27 +903,28 @@ struct RecH {
     {
         return leg == 0 ? x.l.x : leg == 1 ? x.l.y : leg == 2 ? x.l.z : x.l.w;
     }
+    __device__ __forceinline__ static void set_link0(Word &x, uint32_t v) { x.l.x = v; }
     __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const { ec[r] = (uint16_t)e; }
     __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
 };
 
-// Record index of the leg id held shifted in lid14 (lid << 14).  For the
-// 8-byte LDS records it is produced on the vector unit (v_bfe), so the record's
-// LDS address takes no scalar instruction and no scalar-to-vector move: the
-// scalar unit is what the chase runs out of when several walkers share a
-// compute unit.
-template <class R>
-__device__ __forceinline__ uint32_t rec_index(typename R::Key lid14)
-{
-    return (uint32_t)(lid14 >> 16);
-}
-template <>
-__device__ __forceinline__ uint32_t rec_index<Rec8>(uint32_t lid14)
-{
-    uint32_t r;
-    asm("v_bfe_u32 %0, %1, 16, 16" : "=v"(r) : "s"(lid14));
-    return r;
-}
+// Record index of the leg id held shifted in lidS (lid << S).  For the 8-byte
+// LDS records it is produced on the vector unit (v_bfe), so the record's LDS
+// address takes no scalar instruction and no scalar-to-vector move: the scalar
+// unit is what the chase runs out of when several walkers share a compute unit.
+template <class R, uint32_t S>
+struct RecIndex {
+    __device__ __forceinline__ static uint32_t of(typename R::Key lidS) { return (uint32_t)(lidS >> (S + 2)); }
+};
+template <uint32_t S>
+struct RecIndex<Rec8, S> {
+    __device__ __forceinline__ static uint32_t of(uint32_t lidS)
+    {
+        uint32_t r;
+        asm("v_bfe_u32 %0, %1, %2, 16" : "=v"(r) : "s"(lidS), "i"(S + 2));
+        return r;
+    }
+};
 
 // Exit selection of one scatter on the scalar unit: m = word >> 11 against the
 // entry's first three thresholds by 64-bit borrow chains (s_sub/s_subb leave
@@ -987,13 +999,19 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
         if (wf0 - 1u >= dmw)
             return SSE_ERR_STATE;
         const uint32_t lid0 = 4u * r0 + l0;
-        // closing keys (sse.jl:290-297): exit (r0, l0, dim0 - wf0) or entry (r0, l0, wf0)
+        // closing tests (sse.jl:290-297), both on the key of the next leg: the
+        // worm leaves through (r0, l0) with worm dim0 - wf0 exactly when the hop
+        // lands on the leg linked to (r0, l0) (the links are an involution
+        // without fixed points, so at most one test holds), and it re-enters
+        // (r0, l0) with worm wf0.  Key = lid << S | worm-1; with one worm type
+        // the worm field is always 0 and S = 6 puts the leg at bits 6..7, the
+        // entry's leg offset in the transition table.
+        constexpr uint32_t S = kOneWorm ? 6u : 14u;
         typedef typename R::Key Key;
-        const Key keyA = ((Key)lid0 << 14) | (dim0 - wf0 - 1u);
-        const Key keyB = ((Key)lid0 << 14) | (wf0 - 1u);
-        // current leg id kept shifted (lid << 14): the record index, the entry's leg
-        // and both closing keys come from it with one operation each
-        Key lid14 = (Key)lid0 << 14;
+        const uint32_t lnk0 = uni(R::link(rec.load(r0), l0));
+        const Key keyA = ((Key)lnk0 << S) | (kOneWorm ? 0u : dim0 - wf0 - 1u);
+        const Key keyB = ((Key)lid0 << S) | (kOneWorm ? 0u : wf0 - 1u);
+        Key lidS = (Key)lid0 << S;
         uint32_t wfm = wf0 - 1u;
         // one Philox word per step: the step count is the stream advance
         const uint64_t pos0 = rs.bbase + rs.off;
@@ -1006,14 +1024,15 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
                     rs.ensure(1);
             }
             const uint32_t off = uni(rs.off);
-            const uint32_t r = rec_index<R>(lid14);
-            const typename R::Word x = rec.load(r);
+            const uint32_t r = RecIndex<R, S>::of(lidS);
+            typename R::Word x = rec.load(r);
             // rand() = mm * 2^-53: the mantissa of this step's word
             const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)rs.mlo, (int)off);
             const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)rs.mhi, (int)off);
             rs.off = off + 1u;
             const uint32_t e = R::ecode(x);
-            const uint32_t legbits = (uint32_t)(lid14 >> 8) & 0xC0u;  // leg_in << 6
+            const uint32_t legbits = kOneWorm ? (uint32_t)lidS & 0xC0u          // leg_in << 6
+                                              : (uint32_t)(lidS >> 8) & 0xC0u;
             const uint32_t eoff = kOneWorm ? (e << 8) | legbits : e * estride + (wfm << 8) + legbits;
             ConstEntry *E = (ConstEntry *)(CT + eoff);
             uint32_t eo = chase_select(mlo, mhi, E);
@@ -1024,23 +1043,25 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
                 ConstEntry *ES = (ConstEntry *)(CT + eoff_s);
                 // the exact scatter (vertex_data.jl:113-125): the entry's own
                 // thresholds for up to four exits, the exit list for more; a
-                // fall-through or an invalid transition is an error
+                // fall-through or an invalid transition is an error.  A failure
+                // leaves the loop through the exit test (the code is rewritten
+                // unchanged, exit leg 0 of the record copy made the leg linked
+                // to (r0, l0), the worm field that test's), so the fast path
+                // keeps a single loop exit.
                 const uint32_t cnt = uni(ES->count);
-                // on failure: rewrite the unchanged code and leave the loop through
-                // the exit test (lid and the exit word are made to match key A)
-                const uint32_t efail = (e << 16) | (l0 << 14) | (dim0 - wf0 - 1u);
-                eo = efail;
+                const uint32_t efail = (e << 16) | (kOneWorm ? 0u : dim0 - wf0 - 1u);
                 if (cnt <= 4u) {
                     const uint64_t mm = ((uint64_t)mhi << 32) | mlo;
                     uint32_t j = 0;
                     for (; j < cnt; ++j)
                         if (mm < uni64(ES->K[j]))
                             break;
-                    if (j < cnt) {
-                        eo = uni(j < 3u ? ES->out[j] : ES->pad[0]);
-                    } else {
+                    if (j >= cnt) {
                         err = SSE_ERR_SCATTER;
-                        lid14 = (Key)lid0 << 14;
+                        eo = efail;
+                        R::set_link0(x, lnk0);
+                    } else {
+                        eo = uni(j < 3u ? ES->out[j] : ES->pad[0]);
                     }
                 } else {
                     const uint32_t trw = uni(ES->trans);
@@ -1048,43 +1069,40 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
                     const double xr = (double)(((uint64_t)mhi << 32) | mlo) * 0x1.0p-53;
                     uint32_t ex = trw & 0xFFFFFu;
                     const uint32_t eend = ex + (trw >> 20);
-                    for (;;) {
-                        if (ex >= eend) {
-                            err = SSE_ERR_SCATTER;
-                            lid14 = (Key)lid0 << 14;
-                            break;
-                        }
+                    for (; ex < eend; ++ex) {
                         const uint64_t cb = uni64((uint64_t)__double_as_longlong(tab.cum[ex]));
-                        if (xr < __longlong_as_double((long long)cb)) {
-                            const uint32_t vc = uni(tab.eout[ex]);
-                            eo = chase_out((e >> lgp << lgp) | (vc & 1023u), (vc >> 10) & 3u, vc >> 12);
+                        if (xr < __longlong_as_double((long long)cb))
                             break;
-                        }
-                        ++ex;
+                    }
+                    if (ex >= eend) {
+                        err = SSE_ERR_SCATTER;
+                        eo = efail;
+                        R::set_link0(x, lnk0);
+                    } else {
+                        const uint32_t vc = uni(tab.eout[ex]);
+                        const uint32_t ecn = (e >> lgp << lgp) | (vc & 1023u), lg = (vc >> 10) & 3u;
+                        eo = kOneWorm ? chase_out1(ecn, lg) : chase_out(ecn, lg, vc >> 12);
                     }
                 }
             }
             rec.put_ecode(r, eo >> 16);
-            // closing tests (sse.jl:290-297) folded into one loop exit: the worm
-            // ends leaving through (r0, l0, dim0 - wf0), or entering (r0, l0, wf0)
-            const uint32_t nx = R::link(x, (eo >> 14) & 3u);
-            // one worm type: every exit word has worm-1 = 0
-            const uint32_t wfn = kOneWorm ? 0u : eo & 0x3FFFu;
-            const Key kA = (lid14 & ~(Key)0xC000u) | (eo & (kOneWorm ? 0xC000u : 0xFFFFu));
-            const Key nx14 = (Key)nx << 14;
-            const Key kB = kOneWorm ? nx14 : nx14 | wfn;
-            // one exit: d == 0 when either closing test holds (a failed scatter
-            // above forced the exit test to hold)
-            Key d = min(kA ^ keyA, kB ^ keyB);
-            if (R::kChecked && (nx >> 2) >= n && kA != keyA) {
+            uint32_t nx;
+            if constexpr (kOneWorm && std::is_same<R, Rec8>::value)
+                nx = Rec8::link_sh(x, eo);
+            else
+                nx = R::link(x, (eo >> 14) & 3u);
+            if (R::kChecked && (nx >> 2) >= n) {
                 err = SSE_ERR_STATE;
-                d = 0;
-            }
-            if (d == 0) {
-                endB = kA != keyA;  // closed on re-entry (the exit test has priority)
                 break;
             }
-            lid14 = nx14;
+            const uint32_t wfn = kOneWorm ? 0u : eo & 0x3FFFu;
+            const Key nxS = (Key)nx << S;
+            const Key kB = kOneWorm ? nxS : nxS | wfn;
+            if (min(kB ^ keyA, kB ^ keyB) == 0) {
+                endB = kB != keyA;  // closed on re-entry
+                break;
+            }
+            lidS = nxS;
             wfm = wfn;
         }
         if (err)
@@ -1345,7 +1363,9 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
             err = SSE_ERR_STATE;
         } else {
             RecG rec{rec_vt(w), (const_u4 *)(const void *)rec_link(w)};
-            err = worm_chase<RecG, false>(w, m, rs, rec, gmask, gbase, total, scat);
+            // one-worm tables carry 14 * leg in the worm field: the one-worm chase
+            err = m.dt_mw == 1 ? worm_chase<RecG, true>(w, m, rs, rec, gmask, gbase, total, scat)
+                               : worm_chase<RecG, false>(w, m, rs, rec, gmask, gbase, total, scat);
             __syncthreads();
             if (!err)
                 worm_finish(w, m, rs, rec, gmask, gbase);
