@@ -1015,9 +1015,12 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
         uint32_t wfm = wf0 - 1u;
         // one Philox word per step: the step count is the stream advance
         const uint64_t pos0 = rs.bbase + rs.off;
-        uint32_t endB = 0, err = 0;
-        for (;;) {  // sse.jl:274-300
-            if (rs.off >= K2Stream::kNW) {
+        uint32_t err = 0;
+        Key kB = 0;
+        do {  // sse.jl:274-300
+            // the rare paths (refill, exact scatter, worm end) are marked
+            // unlikely so the step runs straight through with one taken branch
+            if (__builtin_expect(rs.off >= K2Stream::kNW, 0)) {
                 if (rs.off == K2Stream::kNW)
                     rs.refill_next();
                 else
@@ -1036,7 +1039,7 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
             const uint32_t eoff = kOneWorm ? (e << 8) | legbits : e * estride + (wfm << 8) + legbits;
             ConstEntry *E = (ConstEntry *)(CT + eoff);
             uint32_t eo = chase_select(mlo, mhi, E);
-            if (eo & kChaseSlowBit) {
+            if (__builtin_expect((eo & kChaseSlowBit) != 0, 0)) {
                 // the entry's address is formed here, not hoisted into the fast path
                 uint32_t eoff_s = eoff;
                 asm volatile("" : "+s"(eoff_s));
@@ -1097,14 +1100,11 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
             }
             const uint32_t wfn = kOneWorm ? 0u : eo & 0x3FFFu;
             const Key nxS = (Key)nx << S;
-            const Key kB = kOneWorm ? nxS : nxS | wfn;
-            if (min(kB ^ keyA, kB ^ keyB) == 0) {
-                endB = kB != keyA;  // closed on re-entry
-                break;
-            }
+            kB = kOneWorm ? nxS : nxS | wfn;
             lidS = nxS;
             wfm = wfn;
-        }
+        } while (__builtin_expect(min(kB ^ keyA, kB ^ keyB) != 0, 1));
+        const uint32_t endB = kB != keyA;  // closed on re-entry
         if (err)
             return (int)err;
         const uint32_t steps = (uint32_t)(rs.bbase + rs.off - pos0);
