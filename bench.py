@@ -252,12 +252,18 @@ def extra_serving(sse, seconds=30.0, sweep_scale=1e-3, slots=1024):
     from importlib import import_module
     sv = import_module("sse_amd.serving")
     jobs = sv.poisson_jobs(int(200 * seconds), rate=200.0, seed=7, sweep_scale=sweep_scale)
-    loop = sv.ServingLoop(n_slots=slots, max_L=48, sweeps_per_step=5)
-    loop.batch.set_option(sse.capi.SSE_OPT_TIMING, 0)
-    done, stats = loop.run(jobs, time_limit=seconds * 3)
+    # size classes by estimated operator count, each its own batch, host
+    # thread and stream (small jobs do not wait behind large walkers)
+    loop = sv.SizeClassServing(classes=((2000, slots * 3 // 16, 10, 48),
+                                        (8000, slots * 6 // 16, 4, 48),
+                                        (30000, slots * 5 // 16, 2, 48),
+                                        (float("inf"), slots * 2 // 16, 1, 48)))
+    done, stats = loop.run(jobs, time_limit=seconds * 3,
+                           log=lambda m: print(f"serving: {m}", file=sys.stderr, flush=True))
     loop.close()
     stats["workload"] = (f"serving: Poisson 200 jobs/s seed 7, L 8..48, beta 1..32, sweeps "
-                         f"log-uniform [1e3,1e5] x {sweep_scale:g}, {slots} slots")
+                         f"log-uniform [1e3,1e5] x {sweep_scale:g}, {slots} slots in 4 size "
+                         f"classes (n <= 2000 / 8000 / 30000 / more), models prebuilt")
     stats["jobs_submitted"] = len(jobs)
     return stats
 

@@ -100,6 +100,21 @@ struct sse_batch {
     std::vector<uint8_t> pt_member;
 };
 
+// Copies and fills of the host-side entry points go through the batch's own
+// non-blocking stream and wait for it.  The legacy stream is never used, so a
+// batch driven by one host thread never waits on, or invalidates, another
+// thread's batch (one that is capturing a graph at that moment, say).
+static hipError_t bmemcpy(sse_batch *b, void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, b->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(b->stream);
+}
+static hipError_t bmemset(sse_batch *b, void *dst, int value, size_t bytes)
+{
+    hipError_t e = hipMemsetAsync(dst, value, bytes, b->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(b->stream);
+}
+
 template <class T>
 static int dev_alloc(sse_batch *b, T **p, size_t count)
 {
@@ -390,7 +405,7 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
         x.ewrm = (const uint8_t *)(d + voff[t].ew);
     }
     memcpy(blob.data() + o_types, types.data(), sizeof(DevVD) * m.n_types);
-    HIPCHECK(hipMemcpy(d, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    HIPCHECK(bmemcpy(b, d, blob.data(), blob.size(), hipMemcpyHostToDevice));
     dm.n_sites = m.n_sites;
     dm.n_bonds = m.n_bonds;
     dm.n_types = m.n_types;
@@ -451,14 +466,14 @@ static int sync_pool_to_device(sse_batch *b)
     p.status = 0;
     p.max_n = 0;
     p.max_L = 0;
-    HIPCHECK(hipMemcpy(b->d_pool, &p, sizeof p, hipMemcpyHostToDevice));
+    HIPCHECK(bmemcpy(b, b->d_pool, &p, sizeof p, hipMemcpyHostToDevice));
     return SSE_OK;
 }
 
 static int sync_pool_from_device(sse_batch *b)
 {
     DevPool p;
-    HIPCHECK(hipMemcpy(&p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));
+    HIPCHECK(bmemcpy(b, &p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));
     b->pool_host_next = p.next > p.size ? p.size : p.next;
     if (p.overflow)
         return fail(SSE_ERR_OOM, "slab pool exhausted; raise SSE_POOL_MB");
@@ -469,7 +484,7 @@ static int check_walker_errors(sse_batch *b)
 {
     DevPool p;
     HIPCHECK(hipStreamSynchronize(b->stream));
-    HIPCHECK(hipMemcpy(&p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));
+    HIPCHECK(bmemcpy(b, &p, b->d_pool, sizeof p, hipMemcpyDeviceToHost));
     if (p.overflow)
         return fail(SSE_ERR_OOM, "slab pool exhausted; raise SSE_POOL_MB");
     if (b->mode == SSE_MODE_WAVE && (p.max_n || p.max_L)) {
@@ -478,7 +493,7 @@ static int check_walker_errors(sse_batch *b)
         b->seen_max_n = p.max_n;
         b->seen_max_L = p.max_L;
         const unsigned long long zero[2] = {0, 0};
-        HIPCHECK(hipMemcpy(&b->d_pool->max_n, zero, sizeof zero, hipMemcpyHostToDevice));
+        HIPCHECK(bmemcpy(b, &b->d_pool->max_n, zero, sizeof zero, hipMemcpyHostToDevice));
         wave_resize_lds(b->wave, b->seen_max_n, b->seen_max_L);
     }
     if (p.status) {
@@ -558,7 +573,7 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
     }
     if (!rc)
         rc = dev_alloc(b, &b->d_models, n_models);
-    if (!rc && hipMemcpy(b->d_models, b->h_models.data(), sizeof(DevModel) * n_models,
+    if (!rc && bmemcpy(b, b->d_models, b->h_models.data(), sizeof(DevModel) * n_models,
                          hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(SSE_ERR_HIP, "model upload failed");
     // initial cutoffs and slab capacities
@@ -656,16 +671,16 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
             x.remaining = -1;
         }
     }
-    if (!rc && hipMemcpy(b->d_walkers, b->h_walkers.data(), sizeof(DevWalker) * n_walkers,
+    if (!rc && bmemcpy(b, b->d_walkers, b->h_walkers.data(), sizeof(DevWalker) * n_walkers,
                          hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(SSE_ERR_HIP, "walker upload failed");
     if (!rc)
         rc = sync_pool_to_device(b);
     if (!rc) {
-        (void)hipMemset(b->d_sum, 0, sizeof(double) * b->total_width);
-        (void)hipMemset(b->d_sumsq, 0, sizeof(double) * b->total_width);
-        (void)hipMemset(b->d_count, 0, sizeof(int64_t) * n_walkers);
-        (void)hipMemset(b->d_done, 0, sizeof(int32_t) * n_walkers);
+        (void)bmemset(b, b->d_sum, 0, sizeof(double) * b->total_width);
+        (void)bmemset(b, b->d_sumsq, 0, sizeof(double) * b->total_width);
+        (void)bmemset(b, b->d_count, 0, sizeof(int64_t) * n_walkers);
+        (void)bmemset(b, b->d_done, 0, sizeof(int32_t) * n_walkers);
         rc = wave_configure(b->wave, b->h_models, b->max_sites);
     }
     if (rc) {
@@ -1111,12 +1126,12 @@ int sse_read_accumulators(sse_batch *b, double *sums, double *sumsq, int64_t *co
         return fail(SSE_ERR_INVALID, "null batch");
     HIPCHECK(hipStreamSynchronize(b->stream));
     if (sums)
-        HIPCHECK(hipMemcpy(sums, b->d_sum, sizeof(double) * b->total_width, hipMemcpyDeviceToHost));
+        HIPCHECK(bmemcpy(b, sums, b->d_sum, sizeof(double) * b->total_width, hipMemcpyDeviceToHost));
     if (sumsq)
-        HIPCHECK(hipMemcpy(sumsq, b->d_sumsq, sizeof(double) * b->total_width,
+        HIPCHECK(bmemcpy(b, sumsq, b->d_sumsq, sizeof(double) * b->total_width,
                            hipMemcpyDeviceToHost));
     if (counts)
-        HIPCHECK(hipMemcpy(counts, b->d_count, sizeof(int64_t) * b->n_walkers,
+        HIPCHECK(bmemcpy(b, counts, b->d_count, sizeof(int64_t) * b->n_walkers,
                            hipMemcpyDeviceToHost));
     return SSE_OK;
 }
@@ -1142,7 +1157,7 @@ int sse_get_walker(sse_batch *b, int32_t walker, sse_walker_scalars *s, uint32_t
         return fail(SSE_ERR_INVALID, "sse_get_walker: bad walker");
     HIPCHECK(hipStreamSynchronize(b->stream));
     DevWalker w;
-    HIPCHECK(hipMemcpy(&w, b->d_walkers + walker, sizeof w, hipMemcpyDeviceToHost));
+    HIPCHECK(bmemcpy(b, &w, b->d_walkers + walker, sizeof w, hipMemcpyDeviceToHost));
     if (s) {
         s->cutoff = w.L;
         s->num_operators = w.n;
@@ -1156,10 +1171,10 @@ int sse_get_walker(sse_batch *b, int32_t walker, sse_walker_scalars *s, uint32_t
     }
     if (ops && ops_capacity > 0 && w.L > 0) {
         int64_t k = w.L < ops_capacity ? w.L : ops_capacity;
-        HIPCHECK(hipMemcpy(ops, w.ops, sizeof(uint32_t) * k, hipMemcpyDeviceToHost));
+        HIPCHECK(bmemcpy(b, ops, w.ops, sizeof(uint32_t) * k, hipMemcpyDeviceToHost));
     }
     if (spins)
-        HIPCHECK(hipMemcpy(spins, w.spins, w.n_sites, hipMemcpyDeviceToHost));
+        HIPCHECK(bmemcpy(b, spins, w.spins, w.n_sites, hipMemcpyDeviceToHost));
     return SSE_OK;
 }
 
@@ -1170,7 +1185,7 @@ int sse_set_walker(sse_batch *b, int32_t walker, const sse_walker_scalars *s, co
         return fail(SSE_ERR_INVALID, "sse_set_walker: bad arguments");
     HIPCHECK(hipStreamSynchronize(b->stream));
     DevWalker w;
-    HIPCHECK(hipMemcpy(&w, b->d_walkers + walker, sizeof w, hipMemcpyDeviceToHost));
+    HIPCHECK(bmemcpy(b, &w, b->d_walkers + walker, sizeof w, hipMemcpyDeviceToHost));
     const DevModel &m = b->h_models[w.model];
     int64_t n = 0;
     for (int64_t p = 0; p < s->cutoff; ++p) {
@@ -1209,9 +1224,9 @@ int sse_set_walker(sse_batch *b, int32_t walker, const sse_walker_scalars *s, co
     w.pending = 0;
     w.retire = 0;
     if (w.L > 0)
-        HIPCHECK(hipMemcpy(w.ops, ops, sizeof(uint32_t) * w.L, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(w.spins, spins, w.n_sites, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(b->d_walkers + walker, &w, sizeof w, hipMemcpyHostToDevice));
+        HIPCHECK(bmemcpy(b, w.ops, ops, sizeof(uint32_t) * w.L, hipMemcpyHostToDevice));
+    HIPCHECK(bmemcpy(b, w.spins, spins, w.n_sites, hipMemcpyHostToDevice));
+    HIPCHECK(bmemcpy(b, b->d_walkers + walker, &w, sizeof w, hipMemcpyHostToDevice));
     b->inited = true;
     return SSE_OK;
 }
@@ -1223,10 +1238,10 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links, int64_t li
         return fail(SSE_ERR_INVALID, "sse_get_vertex_list: bad walker");
     HIPCHECK(hipStreamSynchronize(b->stream));
     DevWalker w;
-    HIPCHECK(hipMemcpy(&w, b->d_walkers + walker, sizeof w, hipMemcpyDeviceToHost));
+    HIPCHECK(bmemcpy(b, &w, b->d_walkers + walker, sizeof w, hipMemcpyDeviceToHost));
     if (b->mode == SSE_MODE_LANE) {
         if (links && links_capacity >= 4 * w.L && w.L > 0)
-            HIPCHECK(hipMemcpy(links, w.links, sizeof(int32_t) * 4 * w.L, hipMemcpyDeviceToHost));
+            HIPCHECK(bmemcpy(b, links, w.links, sizeof(int32_t) * 4 * w.L, hipMemcpyDeviceToHost));
     }
     std::vector<int64_t> pos;  // wave path: operator rank -> slot
     if (b->mode != SSE_MODE_LANE) {
@@ -1236,10 +1251,10 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links, int64_t li
         std::vector<uint64_t> mask(nch > 0 ? nch : 1);
         std::vector<uint32_t> lk(4 * (w.n > 0 ? w.n : 1));
         if (nch > 0)
-            HIPCHECK(hipMemcpy(mask.data(), (const uint64_t *)(vt + 5 * w.cap), 8 * nch,
+            HIPCHECK(bmemcpy(b, mask.data(), (const uint64_t *)(vt + 5 * w.cap), 8 * nch,
                                hipMemcpyDeviceToHost));
         if (w.n > 0)
-            HIPCHECK(hipMemcpy(lk.data(), vt + w.cap, 16 * w.n, hipMemcpyDeviceToHost));
+            HIPCHECK(bmemcpy(b, lk.data(), vt + w.cap, 16 * w.n, hipMemcpyDeviceToHost));
         for (int64_t c = 0; c < nch; ++c)
             for (int i = 0; i < 64; ++i)
                 if ((mask[c] >> i) & 1ull)
@@ -1258,8 +1273,8 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links, int64_t li
         }
         if (first || last) {
             std::vector<int32_t> f(w.n_sites), l(w.n_sites);
-            HIPCHECK(hipMemcpy(f.data(), w.first, 4 * w.n_sites, hipMemcpyDeviceToHost));
-            HIPCHECK(hipMemcpy(l.data(), w.last, 4 * w.n_sites, hipMemcpyDeviceToHost));
+            HIPCHECK(bmemcpy(b, f.data(), w.first, 4 * w.n_sites, hipMemcpyDeviceToHost));
+            HIPCHECK(bmemcpy(b, l.data(), w.last, 4 * w.n_sites, hipMemcpyDeviceToHost));
             for (int i = 0; i < w.n_sites; ++i) {
                 if (first)
                     first[i] = dense(f[i] < 0 ? -1 : f[i]);
@@ -1270,9 +1285,9 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links, int64_t li
         return SSE_OK;
     }
     if (first)
-        HIPCHECK(hipMemcpy(first, w.first, sizeof(int32_t) * w.n_sites, hipMemcpyDeviceToHost));
+        HIPCHECK(bmemcpy(b, first, w.first, sizeof(int32_t) * w.n_sites, hipMemcpyDeviceToHost));
     if (last)
-        HIPCHECK(hipMemcpy(last, w.last, sizeof(int32_t) * w.n_sites, hipMemcpyDeviceToHost));
+        HIPCHECK(bmemcpy(b, last, w.last, sizeof(int32_t) * w.n_sites, hipMemcpyDeviceToHost));
     return SSE_OK;
 }
 
@@ -1285,7 +1300,7 @@ static int compact_pool(sse_batch *b)
 {
     HIPCHECK(hipStreamSynchronize(b->stream));
     std::vector<DevWalker> h(b->n_walkers);
-    HIPCHECK(hipMemcpy(h.data(), b->d_walkers, sizeof(DevWalker) * b->n_walkers,
+    HIPCHECK(bmemcpy(b, h.data(), b->d_walkers, sizeof(DevWalker) * b->n_walkers,
                        hipMemcpyDeviceToHost));
     char *np = nullptr;
     if (hipMalloc((void **)&np, b->pool_size) != hipSuccess)
@@ -1320,7 +1335,7 @@ static int compact_pool(sse_batch *b)
     b->pool_base = np;
     b->allocs.push_back(np);
     b->pool_host_next = next;
-    HIPCHECK(hipMemcpy(b->d_walkers, h.data(), sizeof(DevWalker) * b->n_walkers,
+    HIPCHECK(bmemcpy(b, b->d_walkers, h.data(), sizeof(DevWalker) * b->n_walkers,
                        hipMemcpyHostToDevice));
     return sync_pool_to_device(b);
 }
@@ -1348,7 +1363,7 @@ int sse_add_model(sse_batch *b, const sse_model_desc *model, int32_t *index)
     rc = dev_alloc(b, &nd, b->h_models.size());
     if (rc)
         return rc;
-    HIPCHECK(hipMemcpy(nd, b->h_models.data(), sizeof(DevModel) * b->h_models.size(),
+    HIPCHECK(bmemcpy(b, nd, b->h_models.data(), sizeof(DevModel) * b->h_models.size(),
                        hipMemcpyHostToDevice));
     b->d_models = nd;
     *index = b->n_models++;
@@ -1373,7 +1388,7 @@ int sse_slot_start(sse_batch *b, int32_t slot, const sse_walker_desc *wd, int64_
     if (rc)
         return rc;
     DevWalker x;
-    HIPCHECK(hipMemcpy(&x, b->d_walkers + slot, sizeof x, hipMemcpyDeviceToHost));
+    HIPCHECK(bmemcpy(b, &x, b->d_walkers + slot, sizeof x, hipMemcpyDeviceToHost));
     int64_t L0 = wd->init_opstring_cutoff >= 0 ? wd->init_opstring_cutoff
                                                : (int64_t)std::nearbyint((double)m.n_sites * wd->T);
     int64_t cap = slab_capacity(L0);
@@ -1383,7 +1398,7 @@ int sse_slot_start(sse_batch *b, int32_t slot, const sse_walker_desc *wd, int64_
             x.active = 0;
             x.pending = 0;
             x.remaining = 0;  // release the old slab during compaction
-            HIPCHECK(hipMemcpy(b->d_walkers + slot, &x, sizeof x, hipMemcpyHostToDevice));
+            HIPCHECK(bmemcpy(b, b->d_walkers + slot, &x, sizeof x, hipMemcpyHostToDevice));
             rc = compact_pool(b);
             if (rc)
                 return rc;
@@ -1426,11 +1441,11 @@ int sse_slot_start(sse_batch *b, int32_t slot, const sse_walker_desc *wd, int64_
     x.pending = 1;
     x.retire = 0;
     x.remaining = n_sweeps;
-    HIPCHECK(hipMemcpy(b->d_walkers + slot, &x, sizeof x, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemset(b->d_sum + x.meas_offset, 0, sizeof(double) * b->widths[slot]));
-    HIPCHECK(hipMemset(b->d_sumsq + x.meas_offset, 0, sizeof(double) * b->widths[slot]));
-    HIPCHECK(hipMemset(b->d_count + slot, 0, sizeof(int64_t)));
-    HIPCHECK(hipMemset(b->d_done + slot, 0, sizeof(int32_t)));
+    HIPCHECK(bmemcpy(b, b->d_walkers + slot, &x, sizeof x, hipMemcpyHostToDevice));
+    HIPCHECK(bmemset(b, b->d_sum + x.meas_offset, 0, sizeof(double) * b->widths[slot]));
+    HIPCHECK(bmemset(b, b->d_sumsq + x.meas_offset, 0, sizeof(double) * b->widths[slot]));
+    HIPCHECK(bmemset(b, b->d_count + slot, 0, sizeof(int64_t)));
+    HIPCHECK(bmemset(b, b->d_done + slot, 0, sizeof(int32_t)));
     b->any_pending = true;
     if (n_sweeps > 0)
         b->any_budget = true;
@@ -1444,12 +1459,12 @@ int sse_slot_stop(sse_batch *b, int32_t slot)
         return fail(SSE_ERR_INVALID, "sse_slot_stop: bad slot");
     HIPCHECK(hipStreamSynchronize(b->stream));
     DevWalker x;
-    HIPCHECK(hipMemcpy(&x, b->d_walkers + slot, sizeof x, hipMemcpyDeviceToHost));
+    HIPCHECK(bmemcpy(b, &x, b->d_walkers + slot, sizeof x, hipMemcpyDeviceToHost));
     x.active = 0;
     x.pending = 0;
     x.retire = 0;
     x.remaining = 0;
-    HIPCHECK(hipMemcpy(b->d_walkers + slot, &x, sizeof x, hipMemcpyHostToDevice));
+    HIPCHECK(bmemcpy(b, b->d_walkers + slot, &x, sizeof x, hipMemcpyHostToDevice));
     b->inited = true;
     return SSE_OK;
 }
@@ -1460,7 +1475,7 @@ int sse_slot_poll(sse_batch *b, int32_t *done_slots, int32_t capacity, int32_t *
         return fail(SSE_ERR_INVALID, "sse_slot_poll: bad arguments");
     HIPCHECK(hipStreamSynchronize(b->stream));
     std::vector<int32_t> d(b->n_walkers);
-    HIPCHECK(hipMemcpy(d.data(), b->d_done, sizeof(int32_t) * b->n_walkers, hipMemcpyDeviceToHost));
+    HIPCHECK(bmemcpy(b, d.data(), b->d_done, sizeof(int32_t) * b->n_walkers, hipMemcpyDeviceToHost));
     int32_t k = 0;
     for (int i = 0; i < b->n_walkers; ++i)
         if (d[i]) {
@@ -1469,7 +1484,7 @@ int sse_slot_poll(sse_batch *b, int32_t *done_slots, int32_t capacity, int32_t *
             ++k;
         }
     if (k)
-        HIPCHECK(hipMemset(b->d_done, 0, sizeof(int32_t) * b->n_walkers));
+        HIPCHECK(bmemset(b, b->d_done, 0, sizeof(int32_t) * b->n_walkers));
     *n_done = k;
     return SSE_OK;
 }
@@ -1575,7 +1590,7 @@ int sse_work_counters(sse_batch *b, int64_t *L_sum, int64_t *n_sum, int64_t *W_s
         return fail(SSE_ERR_INVALID, "null batch");
     HIPCHECK(hipStreamSynchronize(b->stream));
     std::vector<DevWalker> h(b->n_walkers);
-    HIPCHECK(hipMemcpy(h.data(), b->d_walkers, sizeof(DevWalker) * b->n_walkers,
+    HIPCHECK(bmemcpy(b, h.data(), b->d_walkers, sizeof(DevWalker) * b->n_walkers,
                        hipMemcpyDeviceToHost));
     int64_t a = 0, c = 0, d = 0, e = 0;
     for (auto &w : h) {
@@ -1640,7 +1655,7 @@ int sse_pt_set_ladders(sse_batch *b, const int32_t *slot_walkers, int32_t n_ladd
     if (total > (size_t)b->n_walkers)
         return fail(SSE_ERR_INVALID, "sse_pt_set_ladders: more ladder slots than walkers");
     std::vector<DevWalker> h(b->n_walkers);
-    HIPCHECK(hipMemcpy(h.data(), b->d_walkers, sizeof(DevWalker) * b->n_walkers,
+    HIPCHECK(bmemcpy(b, h.data(), b->d_walkers, sizeof(DevWalker) * b->n_walkers,
                        hipMemcpyDeviceToHost));
     std::vector<uint8_t> member(b->n_walkers, 0);
     for (int32_t l = 0; l < n_ladders; ++l) {
@@ -1676,9 +1691,9 @@ int sse_pt_set_ladders(sse_batch *b, const int32_t *slot_walkers, int32_t n_ladd
             return rc;
         b->pt_cap = total;
     }
-    HIPCHECK(hipMemcpy(b->d_pt_slot, slot_walkers, sizeof(int32_t) * total, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemset(b->d_pt_acc, 0, sizeof(int64_t) * total));
-    HIPCHECK(hipMemset(b->d_pt_prop, 0, sizeof(int64_t) * total));
+    HIPCHECK(bmemcpy(b, b->d_pt_slot, slot_walkers, sizeof(int32_t) * total, hipMemcpyHostToDevice));
+    HIPCHECK(bmemset(b, b->d_pt_acc, 0, sizeof(int64_t) * total));
+    HIPCHECK(bmemset(b, b->d_pt_prop, 0, sizeof(int64_t) * total));
     b->pt_ladders = n_ladders;
     b->pt_len = ladder_len;
     b->pt_interval = interval;
@@ -1717,13 +1732,13 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted, i
     const size_t total = (size_t)b->pt_ladders * (size_t)b->pt_len;
     if (total) {
         if (slot_walkers)
-            HIPCHECK(hipMemcpy(slot_walkers, b->d_pt_slot, sizeof(int32_t) * total,
+            HIPCHECK(bmemcpy(b, slot_walkers, b->d_pt_slot, sizeof(int32_t) * total,
                                hipMemcpyDeviceToHost));
         if (accepted)
-            HIPCHECK(hipMemcpy(accepted, b->d_pt_acc, sizeof(int64_t) * total,
+            HIPCHECK(bmemcpy(b, accepted, b->d_pt_acc, sizeof(int64_t) * total,
                                hipMemcpyDeviceToHost));
         if (proposed)
-            HIPCHECK(hipMemcpy(proposed, b->d_pt_prop, sizeof(int64_t) * total,
+            HIPCHECK(bmemcpy(b, proposed, b->d_pt_prop, sizeof(int64_t) * total,
                                hipMemcpyDeviceToHost));
     }
     if (rounds)
@@ -1736,7 +1751,7 @@ static int read_walker(sse_batch *b, int32_t walker, DevWalker *w)
     if (!b || walker < 0 || walker >= b->n_walkers)
         return fail(SSE_ERR_INVALID, "bad walker");
     HIPCHECK(hipStreamSynchronize(b->stream));
-    HIPCHECK(hipMemcpy(w, b->d_walkers + walker, sizeof *w, hipMemcpyDeviceToHost));
+    HIPCHECK(bmemcpy(b, w, b->d_walkers + walker, sizeof *w, hipMemcpyDeviceToHost));
     return SSE_OK;
 }
 
@@ -1761,7 +1776,7 @@ int sse_pt_change_parameter(sse_batch *b, int32_t walker, double new_T)
     if (rc)
         return rc;
     w.T = new_T;
-    HIPCHECK(hipMemcpy(b->d_walkers + walker, &w, sizeof w, hipMemcpyHostToDevice));
+    HIPCHECK(bmemcpy(b, b->d_walkers + walker, &w, sizeof w, hipMemcpyHostToDevice));
     return SSE_OK;
 }
 

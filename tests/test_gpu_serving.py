@@ -72,3 +72,31 @@ def test_serving_stream_small(sse_pkg):
     assert stats["p99_time_to_result_s"] >= stats["p50_time_to_result_s"] > 0
     for j in done:
         assert j.result[2] == j.sweeps
+
+
+def test_size_classes_equal_alone(sse_pkg):
+    """Size-class serving (one batch, host thread and stream per class):
+    every job's result is bit-identical to running it alone."""
+    from importlib import import_module
+    sv = import_module("sse_amd.serving")
+    print("serving: size classes", flush=True)
+    jobs = _jobs(sv, 8, sweeps=20)
+    routes = set()
+    loop = sv.SizeClassServing(classes=((200, 3, 3, 8), (float("inf"), 3, 2, 8)))
+    for j in jobs:
+        routes.add(loop.route(j))
+    assert routes == {0, 1}, "the jobs should exercise both classes"
+    done, stats = loop.run(jobs, time_limit=120)
+    loop.close()
+    assert stats["jobs_done"] == 8
+    assert sum(c["jobs_done"] for c in stats["classes"]) == 8
+    by_id = {j.job_id: j for j in done}
+    for job in jobs:
+        alone = sv.ServingLoop(n_slots=1, max_L=8, sweeps_per_step=4)
+        d, _ = alone.run([sv.Job(job.job_id, 0.0, job.model_key, job.T, job.sweeps,
+                                 job.thermalization, job.seed)])
+        alone.close()
+        s, s2, cnt = d[0].result
+        ms, ms2, mcnt = by_id[job.job_id].result
+        assert mcnt == cnt == job.sweeps
+        assert np.array_equal(ms, s) and np.array_equal(ms2, s2), f"job {job.job_id} differs"
