@@ -1496,33 +1496,58 @@ __device__ __forceinline__ void k3_pass(const DevWalker &w, const DevModel &m, i
     int64_t cnt = 0;
     const gmem<uint32_t> *ops = as_global((const uint32_t *)w.ops);
     const gmem<uint64_t> *brec = as_global((const uint64_t *)m.bond_rec);  // int2 pairs
-    // the next chunk's operators are loaded while this chunk is processed
-    uint32_t op_next = (int64_t)lane < L ? ops[lane] : 0u;
-    for (int64_t base = 0; base < L; base += 64) {
-        const int64_t p = base + lane;
-        const uint32_t op = op_next;
-        op_next = p + 64 < L ? ops[p + 64] : 0u;
-        const uint32_t vc = op_vcode(op);
-        int2 br = make_int2(0, 0);
-        const DevVD *vd = m.types;
+    // Software pipeline over the 64-slot chunks: the operator is loaded three
+    // chunks ahead, its bond record two ahead, the vertex's sign and leg states
+    // one ahead, so a chunk waits only for its estimator table values.  The
+    // vertex tables of up to four bond types are addressed from registers.
+    const int nt = m.n_types;
+    const int8_t *sgt[4];
+    const uint8_t *lst[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        sgt[t] = t < nt ? m.types[t].signs : nullptr;
+        lst[t] = t < nt ? m.types[t].leg_states : nullptr;
+    }
+    auto op_at = [&](int64_t q) -> uint32_t { return q < L ? ops[q] : 0u; };
+    auto bond_of = [&](uint32_t op) -> uint64_t { return op != 0u ? brec[op_bond(op)] : 0ull; };
+    // sign of a vertex and, for an off-diagonal one, its four leg states (one u32)
+    auto vertex_of = [&](uint32_t op, uint64_t bv, int8_t &sg, uint32_t &legs) {
+        sg = 0;
+        legs = 0u;
         if (op != 0u) {
-            const uint64_t bv = brec[op_bond(op)];
-            br = make_int2((int)(uint32_t)bv, (int)(uint32_t)(bv >> 32));
-            vd = &m.types[br.x >> 20];
+            const int type = (int)((uint32_t)bv >> 20);
+            const uint32_t v = op_vcode(op) >> 1;
+            const int8_t *sp = nt <= 4 ? (type == 0 ? sgt[0] : type == 1 ? sgt[1] : type == 2 ? sgt[2] : sgt[3])
+                                       : m.types[type].signs;
+            const uint8_t *lp = nt <= 4 ? (type == 0 ? lst[0] : type == 1 ? lst[1] : type == 2 ? lst[2] : lst[3])
+                                        : m.types[type].leg_states;
+            sg = as_global(sp)[v];
+            if (!(op_vcode(op) & 1u))  // off-diagonal: the leg states matter
+                legs = *(const gmem<uint32_t> *)as_global((const uint32_t *)(lp + 4 * v));
         }
+    };
+    // prime the pipeline
+    uint32_t op0 = op_at(lane), op1 = op_at(64 + lane), op2 = op_at(128 + lane);
+    uint64_t bv0 = bond_of(op0), bv1 = bond_of(op1);
+    int8_t sg0;
+    uint32_t lg0;
+    vertex_of(op0, bv0, sg0, lg0);
+    for (int64_t base = 0; base < L; base += 64) {
+        // issue the later stages' loads for the chunks ahead
+        const uint32_t op3 = op_at(base + 192 + lane);
+        const uint64_t bv2 = bond_of(op2);
+        int8_t sg1;
+        uint32_t lg1;
+        vertex_of(op1, bv1, sg1, lg1);
+        // this chunk
+        const uint32_t op = op0;
+        const uint32_t vc = op_vcode(op);
         if (want_sign)
-            neg += popc(__ballot(op != 0u && as_global(vd->signs)[vc >> 1] < 0));
+            neg += popc(__ballot(op != 0u && sg0 < 0));
         cnt += popc(__ballot(op != 0u));
         const bool offd = op != 0u && !(vc & 1u);
-        uint32_t l0 = 0, l1 = 0, l2 = 0, l3 = 0;
-        if (offd) {
-            const gmem<uint8_t> *ls = as_global(vd->leg_states) + 4 * (vc >> 1);
-            l0 = ls[0];
-            l1 = ls[1];
-            l2 = ls[2];
-            l3 = ls[3];
-        }
-        const int sa = br.x & 0xFFFFF, sb = br.y;
+        const uint32_t l0 = lg0 & 255u, l1 = (lg0 >> 8) & 255u, l2 = (lg0 >> 16) & 255u, l3 = lg0 >> 24;
+        const int sa = (int)((uint32_t)bv0 & 0xFFFFFu), sb = (int)(uint32_t)(bv0 >> 32);
 #pragma unroll
         for (int k = 0; k < K3_MAXE; ++k) {
             if (k < ne) {
@@ -1544,6 +1569,14 @@ __device__ __forceinline__ void k3_pass(const DevWalker &w, const DevModel &m, i
                 run[k] += __shfl(inc, 63);
             }
         }
+        // rotate the pipeline
+        op0 = op1;
+        op1 = op2;
+        op2 = op3;
+        bv0 = bv1;
+        bv1 = bv2;
+        sg0 = sg1;
+        lg0 = lg1;
     }
     if (want_sign)
         sign = (neg & 1) ? -1.0 : 1.0;
