@@ -451,3 +451,48 @@ def test_walker_groups_many_walkers(sse_pkg):
         assert c1 == c2 and np.array_equal(s1, s2) and np.array_equal(q1, q2)
     a.close()
     b.close()
+
+
+def test_full_size_c2_properties(sse_pkg, oracle_mod):
+    """The benchmark's full workload (16x16 Heisenberg, beta=8, 4096 walkers in
+    two stream groups, 150 sweeps with the fused measurement after 50
+    thermalization sweeps), checked through size-independent properties:
+    sampled walkers bit-exact against the oracle, operator counts equal to the
+    non-identity operators of every walker, every cutoff on the growth
+    sequence of its initial cutoff, worldline consistency of a sample, and
+    one measurement per walker per thermalized sweep."""
+    model = square_heisenberg(sse_pkg, 16)
+    flat = model.flat()
+    nw, sweeps, therm = 4096, 150, 50
+    walkers = [sse_pkg.walker_desc(T=0.125, seed=s, thermalization=therm) for s in range(nw)]
+    _say(f"full size: {nw} walkers, {sweeps} sweeps")
+    batch = sse_pkg.Batch([flat], walkers)
+    batch.init()
+    batch.run(sweeps, measure=True)
+    acc = batch.accumulators()
+    assert all(a[2] == sweeps - therm for a in acc)
+    rng = np.random.default_rng(11)
+    sample = sorted(set(rng.integers(0, nw, 4).tolist()) | {0, nw - 1})
+    for k in sample:
+        o = oracle_mod.OracleWalker(sse_pkg.capi, flat, walkers[k])
+        o.init()
+        rows = o.run_series(sweeps)
+        gs, gops, gspins = batch.get_walker(k)
+        os_, oops, ospins = o.state()
+        assert np.array_equal(gops, oops) and np.array_equal(gspins, ospins), f"walker {k}"
+        for key in KEYS:
+            assert gs[key] == os_[key], f"walker {k}: {key}"
+        assert np.allclose(acc[k][0], rows.sum(axis=0), rtol=1e-11, atol=1e-9), f"walker {k}"
+    L0 = int(round(flat.n_sites * 0.125))
+    growth = {L0}
+    x = L0
+    for _ in range(64):
+        x = (3 * x) // 2 + 100
+        growth.add(x)
+    for k in range(nw):
+        sc, ops, spins = batch.get_walker(k, with_spins=(k % 64 == 0))
+        assert sc["num_operators"] == int((ops != 0).sum()), f"walker {k}"
+        assert sc["cutoff"] in growth, f"walker {k}: cutoff {sc['cutoff']}"
+        if k % 64 == 0:
+            assert isconsistent(flat, ops, spins), f"walker {k}"
+    batch.close()
