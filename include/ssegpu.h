@@ -290,15 +290,23 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
  *                Results are identical for any value.
  * SSE_OPT_K2_LDS (default 0): LDS bytes per walker of the worm kernel; 0 =
  *                sized from the batch's largest operator count (then widened to
- *                the largest size with the same walkers per CU).  Walkers that
- *                do not fit take a slower record layout; results are identical. */
+ *                the largest size with the same walkers per CU), and with
+ *                stream groups from each group's own largest count.  Walkers
+ *                that do not fit take a slower record layout; results are
+ *                identical.
+ * SSE_OPT_ORDER (default 1): launch schedule.  1 = walkers in order of their
+ *                expected operator count (bonds / T), largest first, so the
+ *                longest chains start first and each stream group holds walkers
+ *                of similar size (and gets a worm-kernel LDS size to match);
+ *                0 = walker index order.  Results are identical.             */
 enum sse_option {
     SSE_OPT_GRAPH = 1,
     SSE_OPT_TIMING = 2,
     SSE_OPT_WORM_RECORDS = 3,
     SSE_OPT_PERSISTENT = 4,
     SSE_OPT_STREAMS = 5,
-    SSE_OPT_K2_LDS = 6
+    SSE_OPT_K2_LDS = 6,
+    SSE_OPT_ORDER = 7
 };
 int sse_set_option(sse_batch *b, int32_t option, int64_t value);
 

@@ -24,6 +24,7 @@ SSE_OPT_WORM_RECORDS = 3
 SSE_OPT_PERSISTENT = 4
 SSE_OPT_STREAMS = 5
 SSE_OPT_K2_LDS = 6
+SSE_OPT_ORDER = 7
 
 STATUS = {0: "SSE_OK", 1: "SSE_ERR_INVALID", 2: "SSE_ERR_HIP", 3: "SSE_ERR_OOM",
           4: "SSE_ERR_SCATTER", 5: "SSE_ERR_NO_DEVICE", 6: "SSE_ERR_STATE"}

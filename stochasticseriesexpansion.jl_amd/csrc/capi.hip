@@ -6,6 +6,7 @@
 // spin/first/last arrays, measurement buffers and one HIP stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -74,6 +75,7 @@ struct sse_batch {
     int g_sweeps = 0, g_measure = -1;
     size_t g_lds = 0;
     int g_rec_mode = 0;
+    bool g_order = true;
     const DevModel *g_models = nullptr;
     double phase_ms[3] = {0, 0, 0};
     int64_t phase_launches = 0;
@@ -96,6 +98,12 @@ struct sse_batch {
     int64_t pt_round = 0;
     size_t pt_cap = 0;
     int32_t *d_pt_slot = nullptr;
+    // schedule: walker indices by expected operator count, largest first (the
+    // longest chains start first; stream groups then hold walkers of similar
+    // size, each group's worm kernel sized for its own largest walker)
+    int32_t *d_order = nullptr;
+    bool use_order = true;
+    int n_big = 0;  // leading walkers of the schedule too large for 8-byte LDS records
     int64_t *d_pt_acc = nullptr, *d_pt_prop = nullptr;
     std::vector<uint8_t> pt_member;
 };
@@ -489,12 +497,18 @@ static int check_walker_errors(sse_batch *b)
         return fail(SSE_ERR_OOM, "slab pool exhausted; raise SSE_POOL_MB");
     if (b->mode == SSE_MODE_WAVE && (p.max_n || p.max_L)) {
         // size the worm kernel's LDS from the operator counts of this call's
-        // sweeps (the device maxima restart at zero for the next call)
+        // sweeps (the device maxima restart at zero for the next call): one
+        // size for single-stream launches, one per stream group
         b->seen_max_n = p.max_n;
         b->seen_max_L = p.max_L;
-        const unsigned long long zero[2] = {0, 0};
+        const unsigned long long zero[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        static_assert(sizeof zero == sizeof p.max_n + sizeof p.max_L + sizeof p.group_max_n,
+                      "pool maxima layout");
         HIPCHECK(bmemcpy(b, &b->d_pool->max_n, zero, sizeof zero, hipMemcpyHostToDevice));
         wave_resize_lds(b->wave, b->seen_max_n, b->seen_max_L);
+        for (int g = 0; g < 7; ++g)
+            if (p.group_max_n[g])
+                b->wave.lds_k2_group[g] = wave_k2_lds_for(b->wave, p.group_max_n[g], p.max_L);
     }
     if (p.status) {
         if (p.status & (1 << SSE_ERR_SCATTER))
@@ -677,6 +691,28 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
     if (!rc)
         rc = sync_pool_to_device(b);
     if (!rc) {
+        // schedule: expected operator count n ~ beta * bonds, largest first
+        // (stable, so equal walkers keep their index order)
+        std::vector<int32_t> order(n_walkers);
+        std::vector<double> key(n_walkers);
+        for (int w = 0; w < n_walkers; ++w) {
+            order[w] = w;
+            key[w] = (double)b->h_models[walkers[w].model].n_bonds / walkers[w].T;
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int32_t a, int32_t c) { return key[a] > key[c]; });
+        // n ~ 0.7 beta * bonds for the magnets here; beyond ~3600 operators a
+        // walker outgrows the 8-byte records, and with stream groups these
+        // walkers get a group (and a worm-kernel LDS size) of their own
+        b->n_big = 0;
+        while (b->n_big < n_walkers && 0.7 * key[order[b->n_big]] > 3600.0)
+            ++b->n_big;
+        rc = dev_alloc(b, &b->d_order, (size_t)n_walkers);
+        if (!rc && bmemcpy(b, b->d_order, order.data(), sizeof(int32_t) * n_walkers,
+                           hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(SSE_ERR_HIP, "schedule upload failed");
+    }
+    if (!rc) {
         (void)bmemset(b, b->d_sum, 0, sizeof(double) * b->total_width);
         (void)bmemset(b, b->d_sumsq, 0, sizeof(double) * b->total_width);
         (void)bmemset(b, b->d_count, 0, sizeof(int64_t) * n_walkers);
@@ -739,7 +775,7 @@ static int launch_sweep(sse_batch *b, int phases, hipEvent_t *ev = nullptr)
         }
     } else {
         int rc = wave_launch_sweep(b->wave, b->stream, b->d_walkers, b->d_models, b->d_pool, 0, n,
-                                   phases, ev);
+                                   phases, ev, b->use_order ? b->d_order : nullptr, -1);
         if (rc)
             return rc;
     }
@@ -792,7 +828,7 @@ static int graph_run(sse_batch *b, int sweeps, int measure)
 {
     const size_t lds = b->wave.lds_k2;
     if (!b->gexec || b->g_sweeps != sweeps || b->g_measure != measure || b->g_lds != lds ||
-        b->g_rec_mode != b->wave.rec_mode ||
+        b->g_rec_mode != b->wave.rec_mode || b->g_order != b->use_order ||
         b->g_models != b->d_models) {
         if (b->gexec) {
             (void)hipGraphExecDestroy(b->gexec);
@@ -821,6 +857,7 @@ static int graph_run(sse_batch *b, int sweeps, int measure)
         b->g_measure = measure;
         b->g_lds = lds;
         b->g_rec_mode = b->wave.rec_mode;
+        b->g_order = b->use_order;
         b->g_models = b->d_models;
     }
     HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
@@ -877,15 +914,25 @@ static int groups_run(sse_batch *b, int n_sweeps, int measure)
     HIPCHECK(hipEventRecord(b->gfork, b->stream));
     for (int g = 1; g < G; ++g)
         HIPCHECK(hipStreamWaitEvent(b->gstreams[g - 1], b->gfork, 0));
+    // group bounds in schedule positions: equal shares, or, when the schedule
+    // leads with large walkers, those in group 0 and the rest shared equally
+    std::vector<int> bound(G + 1);
+    const int big = b->use_order && G > 1 && b->n_big > 0 && b->n_big < n ? b->n_big : 0;
+    for (int g = 0; g <= G; ++g)
+        bound[g] = big ? (g == 0 ? 0 : big + (int)((int64_t)(n - big) * (g - 1) / (G - 1)))
+                       : (int)((int64_t)n * g / G);
     for (int k = 0; k < n_sweeps; ++k) {
         for (int g = 0; g < G; ++g) {
             hipStream_t gs = g == 0 ? b->stream : b->gstreams[g - 1];
-            const int w0 = (int)((int64_t)n * g / G), w1 = (int)((int64_t)n * (g + 1) / G);
+            const int w0 = bound[g], w1 = bound[g + 1];
+            if (w0 >= w1)
+                continue;
+            const int32_t *ord = b->use_order ? b->d_order : nullptr;
             int rc = wave_launch_sweep(b->wave, gs, b->d_walkers, b->d_models, b->d_pool, w0, w1, 7,
-                                       nullptr);
+                                       nullptr, ord, g);
             if (!rc && measure)
                 rc = wave_launch_measure(b->wave, gs, b->d_walkers, b->d_models, b->d_out, b->d_sum,
-                                         b->d_sumsq, b->d_count, w0, w1, 1, b->d_done);
+                                         b->d_sumsq, b->d_count, w0, w1, 1, b->d_done, ord);
             if (rc)
                 return fail(rc, "group sweep launch failed");
         }
@@ -1078,7 +1125,8 @@ static int launch_measure(sse_batch *b, int mode)
                            b->d_scratch, b->max_sites, n, mode, b->d_done);
     } else {
         int rc = wave_launch_measure(b->wave, b->stream, b->d_walkers, b->d_models, b->d_out,
-                                     b->d_sum, b->d_sumsq, b->d_count, 0, n, mode, b->d_done);
+                                     b->d_sum, b->d_sumsq, b->d_count, 0, n, mode, b->d_done,
+                                     b->use_order ? b->d_order : nullptr);
         if (rc)
             return rc;
     }
@@ -1526,13 +1574,25 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         HIPCHECK(hipStreamSynchronize(b->stream));
         b->wave.lds_k2_fixed = (size_t)value & ~(size_t)15;
         wave_resize_lds(b->wave, b->seen_max_n, b->seen_max_L);
+        for (size_t &g : b->wave.lds_k2_group)
+            g = 0;
         return SSE_OK;
     }
+    case SSE_OPT_ORDER:
+        if (value != 0 && value != 1)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_ORDER takes 0 (walker index order) or 1");
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        b->use_order = value != 0;
+        for (size_t &g : b->wave.lds_k2_group)  // the groups hold other walkers now
+            g = 0;
+        return SSE_OK;
     case SSE_OPT_STREAMS:
         if (value < 1 || value > 64)
             return fail(SSE_ERR_INVALID, "SSE_OPT_STREAMS takes 1..64 walker groups");
         HIPCHECK(hipStreamSynchronize(b->stream));
         b->n_groups = (int)value;
+        for (size_t &g : b->wave.lds_k2_group)  // the groups hold other walkers now
+            g = 0;
         return SSE_OK;
     case SSE_OPT_WORM_RECORDS:
         if (value < 0 || value > 3)

@@ -174,6 +174,7 @@ struct DevPool {
     int32_t status;           // bit (1 << code) per walker error code seen
     unsigned long long max_n; // largest operator count after a diagonal update
     unsigned long long max_L; // largest cutoff
+    unsigned long long group_max_n[8];  // the same per walker group (stream group)
 };
 
 __device__ __forceinline__ void flag_error(DevPool *pool, int code)

@@ -38,6 +38,7 @@ struct WaveConfig {
     bool small_codes = true;   // every model has <= 256 chase codes (Rec8 records)
     int rec_mode = 0;          // worm-kernel record layout: 0 auto, 1 no Rec8, 2 global
     size_t lds_k2_fixed = 0;   // worm-kernel LDS per walker set by the caller (0 = sized here)
+    size_t lds_k2_group[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per stream group (0 = lds_k2)
 };
 
 __device__ __forceinline__ uint64_t below_mask(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
@@ -395,7 +396,7 @@ __device__ void k1_serial_chunk(DevWalker &w, const DevModel &m, WaveStream &rs,
 // sweeps are skipped).
 template <class Tab>
 __device__ __forceinline__ void k1_walker_t(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
-                                            int phases, int warm, char *lds, Tab tab)
+                                            int phases, int warm, char *lds, Tab tab, int grp)
 {
     const int lane = threadIdx.x;
     DevWalker w = W[wi];
@@ -726,6 +727,7 @@ __device__ __forceinline__ void k1_walker_t(DevWalker *W, const DevModel *models
         o.links = w.links;
         o.ctr = rs.ctr();
         atomicMax(&pool->max_n, (unsigned long long)n0);
+        atomicMax(&pool->group_max_n[grp & 7], (unsigned long long)n0);
         atomicMax(&pool->max_L, (unsigned long long)L);
     }
 }
@@ -733,30 +735,39 @@ __device__ __forceinline__ void k1_walker_t(DevWalker *W, const DevModel *models
 // K1 of walker wi; the model's lookups come from LDS when the launch's LDS
 // (lds_bytes) holds them
 __device__ __forceinline__ void k1_walker(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
-                                          int phases, int warm, char *lds, int lds_bytes)
+                                          int phases, int warm, char *lds, int lds_bytes, int grp)
 {
     const DevModel &m = models[W[wi].model];
     const size_t base = k1_lds_bytes(m.n_sites);
     if (k1_lds_staged(m.n_sites, m.n_bonds, m.k1tab_bytes) <= (size_t)lds_bytes) {
         K1Staged tab{(const int2 *)(lds + base),
                      (const uint8_t *)(lds + base + ((8 * (size_t)m.n_bonds + 15) & ~(size_t)15))};
-        k1_walker_t(W, models, pool, wi, phases, warm, lds, tab);
+        k1_walker_t(W, models, pool, wi, phases, warm, lds, tab, grp);
     } else {
         K1Global tab{&m};
-        k1_walker_t(W, models, pool, wi, phases, warm, lds, tab);
+        k1_walker_t(W, models, pool, wi, phases, warm, lds, tab, grp);
     }
 }
 
 // kernels of the per-sweep path take a walker range [w_begin, w_end), one
 // workgroup (one wave) per walker
+// Walker of workgroup b of a launch over positions [w_begin, w_end): with a
+// schedule (order: walker indices, largest expected operator count first) the
+// position indexes it, else it is the walker index itself.
+__device__ __forceinline__ int walker_at(const int32_t *order, int pos)
+{
+    return order ? order[pos] : pos;
+}
+
 __global__ __launch_bounds__(64) void wave_diag_vlist_kernel(DevWalker *W, const DevModel *models,
                                                              DevPool *pool, int w_begin, int w_end,
-                                                             int phases, int warm, int lds_bytes)
+                                                             int phases, int warm, int lds_bytes,
+                                                             const int32_t *order, int grp)
 {
     extern __shared__ __align__(16) char lds[];
-    const int wi = w_begin + (int)blockIdx.x;
-    if (wi < w_end)
-        k1_walker(W, models, pool, wi, phases, warm, lds, lds_bytes);
+    const int pos = w_begin + (int)blockIdx.x;
+    if (pos < w_end)
+        k1_walker(W, models, pool, walker_at(order, pos), phases, warm, lds, lds_bytes, grp);
 }
 
 // ------------------------------------------------------------------ K2
@@ -1407,12 +1418,13 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
 // RecH, then RecG); 1 skips Rec8; 2 forces RecG; 3 forces RecH.
 __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevModel *models,
                                                        DevPool *pool, int w_begin, int w_end,
-                                                       int full_sweep, int lds_bytes, int rec_mode)
+                                                       int full_sweep, int lds_bytes, int rec_mode,
+                                                       const int32_t *order)
 {
     extern __shared__ __align__(16) char lds[];
-    const int wi = w_begin + (int)blockIdx.x;
-    if (wi < w_end)
-        k2_walker(W, models, pool, wi, full_sweep, lds_bytes, rec_mode, lds);
+    const int pos = w_begin + (int)blockIdx.x;
+    if (pos < w_end)
+        k2_walker(W, models, pool, walker_at(order, pos), full_sweep, lds_bytes, rec_mode, lds);
 }
 
 // ------------------------------------------------------------------ init
@@ -1665,11 +1677,11 @@ __device__ __forceinline__ void k3_walker(DevWalker *W, const DevModel *models, 
 __global__ __launch_bounds__(64) void wave_measure_kernel(DevWalker *W, const DevModel *models,
                                                           double *out, double *sum, double *sumsq,
                                                           int64_t *count, int w_begin, int w_end,
-                                                          int mode, int32_t *done)
+                                                          int mode, int32_t *done, const int32_t *order)
 {
-    const int wi = w_begin + (int)blockIdx.x;
-    if (wi < w_end)
-        k3_walker(W, models, out, sum, sumsq, count, wi, mode, done);
+    const int pos = w_begin + (int)blockIdx.x;
+    if (pos < w_end)
+        k3_walker(W, models, out, sum, sumsq, count, walker_at(order, pos), mode, done);
 }
 
 // pending walkers become active once initialised
@@ -1719,17 +1731,16 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
     return SSE_OK;
 }
 
-// Size the worm kernel's LDS for the largest walker of the last call (max n,
+// Worm-kernel LDS per walker for the largest walker of the last call (max n,
 // max L) with a little headroom; walkers that do not fit take a slower record
-// layout for that sweep.
-// The allocation is then widened to the largest size that keeps the same
-// number of walkers per CU (more headroom at no cost in residency).
-static inline void wave_resize_lds(WaveConfig &c, unsigned long long max_n, unsigned long long max_L)
+// layout for that sweep.  The allocation is then widened to the largest size
+// that keeps the same number of walkers per CU (more headroom at no cost in
+// residency).
+static inline size_t wave_k2_lds_for(const WaveConfig &c, unsigned long long max_n,
+                                     unsigned long long max_L)
 {
-    if (c.lds_k2_fixed) {
-        c.lds_k2 = c.lds_k2_fixed;
-        return;
-    }
+    if (c.lds_k2_fixed)
+        return c.lds_k2_fixed;
     const int64_t nh = (int64_t)(max_n + 16);
     size_t need = k2_lds_need(nh, (int64_t)max_L, nh <= 4096 && c.small_codes && c.rec_mode == 0 ? 8
                                                   : c.rec_mode == 3                           ? 2
@@ -1749,7 +1760,12 @@ static inline void wave_resize_lds(WaveConfig &c, unsigned long long max_n, unsi
     size_t wide = ((per_cu / k) & ~(size_t)511) - 256;
     if (wide < need)
         wide = need;
-    c.lds_k2 = wide;
+    return wide;
+}
+
+static inline void wave_resize_lds(WaveConfig &c, unsigned long long max_n, unsigned long long max_L)
+{
+    c.lds_k2 = wave_k2_lds_for(c, max_n, max_L);
 }
 
 static inline int wave_launch_init(WaveConfig &c, hipStream_t s, DevWalker *W, const DevModel *M,
@@ -1758,26 +1774,31 @@ static inline int wave_launch_init(WaveConfig &c, hipStream_t s, DevWalker *W, c
     hipLaunchKernelGGL(wave_init_kernel, dim3(n), dim3(64), 0, s, W, M, n);
     for (int r = 0; r < max_warmup; ++r)
         hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, 0, n, 1, r,
-                           (int)c.lds_k1);
+                           (int)c.lds_k1, (const int32_t *)nullptr, 0);
     hipLaunchKernelGGL(activate_pending_kernel, dim3((n + 63) / 64), dim3(64), 0, s, W, n);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
 
-// one sweep (phase mask) of walkers [w0, w1) on stream s
+// one sweep (phase mask) of the walkers at schedule positions [w0, w1) on
+// stream s; grp is their stream group (0..6: its own worm-kernel LDS size
+// once sized; -1: one stream for all walkers)
 static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, const DevModel *M,
-                                    DevPool *pool, int w0, int w1, int phases, hipEvent_t *ev)
+                                    DevPool *pool, int w0, int w1, int phases, hipEvent_t *ev,
+                                    const int32_t *order = nullptr, int grp = -1)
 {
     const int n = w1 - w0;
     const int k1 = phases & (1 | 2 | 8);
+    const int g = grp >= 0 && grp < 7 ? grp : 7;  // slot 7 collects the rest
+    const size_t lds2 = g < 7 && c.lds_k2_group[g] ? c.lds_k2_group[g] : c.lds_k2;
     if (k1 & 3) {
         hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, w0, w1, k1,
-                           -1, (int)c.lds_k1);
+                           -1, (int)c.lds_k1, order, g);
     }
     if (ev)
         (void)hipEventRecord(ev[1], s);
     if (phases & 4)
-        hipLaunchKernelGGL(wave_worm_kernel, dim3(n), dim3(64), c.lds_k2, s, W, M, pool, w0, w1,
-                           (phases & 7) == 7 ? 1 : 0, (int)c.lds_k2, c.rec_mode);
+        hipLaunchKernelGGL(wave_worm_kernel, dim3(n), dim3(64), lds2, s, W, M, pool, w0, w1,
+                           (phases & 7) == 7 ? 1 : 0, (int)lds2, c.rec_mode, order);
     if (ev)
         (void)hipEventRecord(ev[2], s);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
@@ -1785,10 +1806,11 @@ static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, 
 
 static inline int wave_launch_measure(WaveConfig &, hipStream_t s, DevWalker *W, const DevModel *M,
                                       double *out, double *sum, double *sumsq, int64_t *count,
-                                      int w0, int w1, int mode, int32_t *done)
+                                      int w0, int w1, int mode, int32_t *done,
+                                      const int32_t *order = nullptr)
 {
     hipLaunchKernelGGL(wave_measure_kernel, dim3(w1 - w0), dim3(64), 0, s, W, M, out, sum, sumsq, count,
-                       w0, w1, mode, done);
+                       w0, w1, mode, done, order);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
 

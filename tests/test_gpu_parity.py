@@ -496,3 +496,37 @@ def test_full_size_c2_properties(sse_pkg, oracle_mod):
         if k % 64 == 0:
             assert isconsistent(flat, ops, spins), f"walker {k}"
     batch.close()
+
+
+def test_schedule_order_and_group_lds(sse_pkg):
+    """A heterogeneous batch swept with the size-ordered schedule (stream
+    groups of similar walkers, each group's worm kernel sized for its own
+    largest walker) and with walker index order ends in identical walkers
+    and accumulators."""
+    rng = np.random.default_rng(5)
+    flats, walkers = [], []
+    for k in range(96):
+        L = int(rng.choice([4, 8, 12, 16]))
+        model = square_heisenberg(sse_pkg, L, d=float(rng.uniform(-0.5, 0.5)), hz=0.1)
+        flats.append(model.flat())
+        beta = float(np.exp(rng.uniform(np.log(0.5), np.log(16.0))))
+        walkers.append(sse_pkg.walker_desc(model=k, T=1.0 / beta, seed=900 + k, thermalization=10))
+    results = []
+    for order in (1, 0):
+        _say(f"schedule order={order}")
+        batch = sse_pkg.Batch(flats, walkers)
+        batch.set_option(sse_pkg.capi.SSE_OPT_ORDER, order)
+        batch.set_option(sse_pkg.capi.SSE_OPT_STREAMS, 3)
+        batch.init()
+        for _ in range(4):
+            batch.run(15, measure=True)
+        states = [batch.get_walker(k) for k in range(len(walkers))]
+        results.append((states, batch.accumulators()))
+        batch.close()
+    (s1, a1), (s0, a0) = results
+    for k in range(len(walkers)):
+        g1, g0 = s1[k], s0[k]
+        assert np.array_equal(g1[1], g0[1]) and np.array_equal(g1[2], g0[2]), f"walker {k}"
+        for key in KEYS:
+            assert g1[0][key] == g0[0][key], f"walker {k}: {key}"
+        assert np.array_equal(a1[k][0], a0[k][0]) and a1[k][2] == a0[k][2], f"walker {k} sums"
