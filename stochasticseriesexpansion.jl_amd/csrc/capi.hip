@@ -37,6 +37,13 @@ int fail(int code, const std::string &msg)
             return fail(SSE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));   \
     } while (0)
 
+// What sse_set_walker checks a restored state against, per model: the vertex
+// count of each bond's type and each site's local dimension.
+struct HostModelCheck {
+    std::vector<int16_t> bond_nv;
+    std::vector<uint8_t> site_dims;
+};
+
 }  // namespace
 
 struct sse_batch {
@@ -48,6 +55,7 @@ struct sse_batch {
     std::vector<int32_t> widths, offsets, nsites;
     std::vector<DevModel> h_models;
     std::vector<int> model_max_dim;
+    std::vector<HostModelCheck> h_check;  // [model], in upload order
     DevModel *d_models = nullptr;
     std::vector<void *> allocs;
     DevWalker *d_walkers = nullptr;
@@ -207,6 +215,12 @@ static int validate_model(const sse_model_desc &m, int k)
 // Upload one model: every array into one device buffer, pointers fixed up.
 static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
 {
+    HostModelCheck hc;
+    hc.site_dims.assign(m.site_dims, m.site_dims + m.n_sites);
+    hc.bond_nv.resize(m.n_bonds);
+    for (int k = 0; k < m.n_bonds; ++k)
+        hc.bond_nv[k] = (int16_t)m.types[m.bond_type[k]].n_vertices;
+    b->h_check.push_back(std::move(hc));
     std::vector<char> blob;
     auto put = [&](const void *src, size_t bytes) {
         size_t off = (blob.size() + 15) & ~(size_t)15;
@@ -1235,17 +1249,25 @@ int sse_set_walker(sse_batch *b, int32_t walker, const sse_walker_scalars *s, co
     DevWalker w;
     HIPCHECK(bmemcpy(b, &w, b->d_walkers + walker, sizeof w, hipMemcpyDeviceToHost));
     const DevModel &m = b->h_models[w.model];
+    const HostModelCheck &hc = b->h_check[w.model];
     int64_t n = 0;
     for (int64_t p = 0; p < s->cutoff; ++p) {
         uint32_t op = ops[p];
         if (op) {
             ++n;
-            if ((op >> SSE_OP_BOND_SHIFT) >= (uint32_t)m.n_bonds)
+            const uint32_t bond = op >> SSE_OP_BOND_SHIFT;
+            if (bond >= (uint32_t)m.n_bonds)
                 return fail(SSE_ERR_INVALID, "sse_set_walker: operator bond out of range");
+            if (((op >> 2) & 511u) >= (uint32_t)hc.bond_nv[bond])
+                return fail(SSE_ERR_INVALID, "sse_set_walker: vertex index out of range for the "
+                                             "operator's bond type");
         }
     }
     if (n != s->num_operators)
         return fail(SSE_ERR_INVALID, "sse_set_walker: num_operators does not match ops");
+    for (int32_t i = 0; i < w.n_sites; ++i)
+        if (spins[i] < 1 || spins[i] > hc.site_dims[i])
+            return fail(SSE_ERR_INVALID, "sse_set_walker: local state outside 1..dim");
     int rc = sync_pool_from_device(b);
     if (rc)
         return rc;
