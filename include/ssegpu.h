@@ -314,6 +314,30 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value);
 uint64_t sse_opcode_to_reference(uint32_t op);
 uint32_t sse_opcode_from_reference(uint64_t op);
 
+/* Transition tables of one bond type: the abstract-loop linear programs of
+ * construct_transitions (src/vertex_data.jl:213-416, solved there by HiGHS
+ * through JuMP at :267-350), here by a dense two-phase simplex on the host;
+ * no device is used.  Same step, variable and class orders, normalisation,
+ * clamping, pruning (prob > tolerance / #steps) and cumulative sums.
+ *   n_sites, dims[n_sites]          bond sites and their local dimensions
+ *   leg_states[2*n_sites][n_vertices]  1-based, as VertexData.leg_states
+ *   weights[n_vertices]             |matrix element| of each vertex
+ * Out: trans_offset / trans_length [n_vertices][max_worm][2*n_sites]
+ * (max_worm = max(dims) - 1): 1-based offset into the exit lists (0 = no
+ * entry) and exit count - 1; up to exit_capacity exits in cumprobs,
+ * targets (vertex codes diag | idx << 1, 1-based idx), out_leg, out_worm.
+ * *n_exits receives the exit count (SSE_ERR_INVALID when it exceeds
+ * exit_capacity); *flags: bit 0 a probability was clamped by more than
+ * tolerance, bit 1 an exit list misses 1 by more than tolerance, bit 2 some
+ * LP optimum may not be unique (another solver may return another valid
+ * table).  SSE_ERR_INVALID also when an LP fails or its solution leaves
+ * [0, weight] by more than lp_tolerance.                                   */
+int sse_vertex_transitions(int32_t n_sites, const int32_t *dims, int32_t n_vertices,
+                           const uint8_t *leg_states, const double *weights, double tolerance,
+                           double lp_tolerance, int32_t *trans_offset, int32_t *trans_length,
+                           int64_t exit_capacity, double *cumprobs, int32_t *targets,
+                           int32_t *out_leg, int32_t *out_worm, int64_t *n_exits, int32_t *flags);
+
 const char *sse_last_error(void);
 int32_t sse_abi_version(void);
 

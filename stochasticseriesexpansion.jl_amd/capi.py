@@ -212,6 +212,10 @@ def load_library(path=None):
     lib.sse_pt_exchange.argtypes = [vp]
     lib.sse_pt_get_ladders.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    i32p, f64p, i64p = C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_int64)
+    lib.sse_vertex_transitions.argtypes = [C.c_int32, i32p, C.c_int32, C.POINTER(C.c_uint8),
+                                           f64p, C.c_double, C.c_double, i32p, i32p, C.c_int64,
+                                           f64p, i32p, i32p, i32p, i64p, i32p]
     lib.sse_last_error.restype = C.c_char_p
     lib.sse_abi_version.restype = C.c_int32
     if path is None:
@@ -227,7 +231,35 @@ EXPORTED_SYMBOLS = (
     "sse_opcode_from_reference", "sse_last_error", "sse_abi_version", "sse_add_model",
     "sse_slot_start", "sse_slot_stop", "sse_slot_poll", "sse_pool_usage", "sse_set_option",
     "sse_call_time", "sse_lds_bytes", "sse_pt_log_weight_ratio", "sse_pt_change_parameter",
-    "sse_pt_get_parameter", "sse_pt_set_ladders", "sse_pt_exchange", "sse_pt_get_ladders")
+    "sse_pt_get_parameter", "sse_pt_set_ladders", "sse_pt_exchange", "sse_pt_get_ladders",
+    "sse_vertex_transitions")
+
+
+def vertex_transitions(dims, leg_states, weights, tolerance, lp_tolerance):
+    """Native transition tables (sse_vertex_transitions).  Returns
+    (offset, length, cumprobs, targets, out_leg, out_worm, flags) with
+    offset/length shaped [n_vertices][max_worm][2 * n_sites]."""
+    lib = load_library()
+    dims = np.ascontiguousarray(dims, dtype=np.int32)
+    ls = np.ascontiguousarray(leg_states, dtype=np.uint8)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    ns, nv = len(dims), len(w)
+    mw = max(1, int(dims.max()) - 1)
+    legs = 2 * ns
+    n_steps = sum(max(0, int(d) - 1) for d in dims) * 2
+    cap = max(1, nv * n_steps ** 3)  # <= nv * n_steps LPs, each <= n_steps^2 exits
+    off = np.zeros((nv, mw, legs), dtype=np.int32)
+    ln = np.zeros((nv, mw, legs), dtype=np.int32)
+    cum = np.zeros(cap)
+    tg, ol, ow = (np.zeros(cap, dtype=np.int32) for _ in range(3))
+    n_exits, flags = C.c_int64(), C.c_int32()
+    check(lib, lib.sse_vertex_transitions(
+        ns, _ptr(dims, C.c_int32), nv, _ptr(ls, C.c_uint8), _ptr(w, C.c_double), tolerance,
+        lp_tolerance, _ptr(off, C.c_int32), _ptr(ln, C.c_int32), cap, _ptr(cum, C.c_double),
+        _ptr(tg, C.c_int32), _ptr(ol, C.c_int32), _ptr(ow, C.c_int32), C.byref(n_exits),
+        C.byref(flags)))
+    k = n_exits.value
+    return off, ln, cum[:k], tg[:k], ol[:k], ow[:k], flags.value
 
 
 class SSEError(RuntimeError):

@@ -16,6 +16,7 @@
 
 #include "../../include/ssegpu.h"
 #include "sse_common.h"
+#include "vertex_lp.h"
 #include "sse_lane.h"
 #include "sse_wave.h"
 #include "sse_pt.h"
@@ -557,6 +558,61 @@ uint32_t sse_opcode_from_reference(uint64_t op)
     uint64_t bond = (op >> 26) - 1;
     uint64_t vc = (vref & 1) | (((vref >> 1) - 1) << 1);
     return (uint32_t)(1u | (vc << 1) | (bond << SSE_OP_BOND_SHIFT));
+}
+
+static int max_dim_of(const int32_t *dims, int32_t n)
+{
+    int m = 1;
+    for (int32_t i = 0; i < n; ++i)
+        m = dims[i] > m ? dims[i] : m;
+    return m;
+}
+
+int sse_vertex_transitions(int32_t n_sites, const int32_t *dims, int32_t n_vertices,
+                           const uint8_t *leg_states, const double *weights, double tolerance,
+                           double lp_tolerance, int32_t *trans_offset, int32_t *trans_length,
+                           int64_t exit_capacity, double *cumprobs, int32_t *targets,
+                           int32_t *out_leg, int32_t *out_worm, int64_t *n_exits, int32_t *flags)
+{
+    if (!dims || !leg_states || !weights || !trans_offset || !trans_length || !n_exits ||
+        !flags || n_sites < 1 || n_sites > 8 || n_vertices < 0 || exit_capacity < 0 ||
+        (exit_capacity > 0 && (!cumprobs || !targets || !out_leg || !out_worm)))
+        return fail(SSE_ERR_INVALID, "sse_vertex_transitions: invalid arguments");
+    ssevd::Transitions t;
+    int rc;
+    try {
+        rc = ssevd::build_transitions(n_sites, dims, n_vertices, leg_states, weights, tolerance,
+                                      lp_tolerance, t);
+    } catch (...) {
+        return fail(SSE_ERR_OOM, "sse_vertex_transitions: out of host memory");
+    }
+    static const char *const what[] = {
+        "", "sse_vertex_transitions: bad dimensions or vertex count",
+        "sse_vertex_transitions: a transition LP has no optimum",
+        "sse_vertex_transitions: LP solution below its bound",
+        "sse_vertex_transitions: LP solution above its bound",
+        "sse_vertex_transitions: transition to an invalid vertex"};
+    if (rc)
+        return fail(SSE_ERR_INVALID, what[rc]);
+    *n_exits = (int64_t)t.cumprobs.size();
+    *flags = (t.inaccurate ? 1 : 0) | (t.norm_warning ? 2 : 0) | (t.alt_optima ? 4 : 0);
+    if ((int64_t)t.cumprobs.size() > exit_capacity)
+        return fail(SSE_ERR_INVALID, "sse_vertex_transitions: exit_capacity too small");
+    const size_t nt = (size_t)n_vertices * (size_t)(max_dim_of(dims, n_sites) - 1) * 2 * n_sites;
+    if (t.offset.empty()) {  // no entries: the caller's table is all zeros
+        memset(trans_offset, 0, sizeof(int32_t) * nt);
+        memset(trans_length, 0, sizeof(int32_t) * nt);
+    } else {
+        memcpy(trans_offset, t.offset.data(), sizeof(int32_t) * t.offset.size());
+        memcpy(trans_length, t.length.data(), sizeof(int32_t) * t.length.size());
+    }
+    if (!t.cumprobs.empty()) {
+        memcpy(cumprobs, t.cumprobs.data(), sizeof(double) * t.cumprobs.size());
+        memcpy(targets, t.targets.data(), sizeof(int32_t) * t.targets.size());
+        memcpy(out_leg, t.out_leg.data(), sizeof(int32_t) * t.out_leg.size());
+        memcpy(out_worm, t.out_worm.data(), sizeof(int32_t) * t.out_worm.size());
+    }
+    return SSE_OK;
 }
 
 int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_walker_desc *walkers,

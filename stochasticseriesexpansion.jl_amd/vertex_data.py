@@ -4,10 +4,11 @@ Host-side restatement of src/vertex_data.jl (StochasticSeriesExpansion.jl).
 Conventions follow the reference: vertex indices and states are 1-based,
 a vertex code is ``diag | idx << 1`` (opercode.jl:18-21) and ``None`` is the
 invalid vertex (opercode.jl:16).  The transition LP (vertex_data.jl:213-416)
-is solved with the HiGHS dual simplex shipped in scipy.  When an LP optimum
-is degenerate HiGHS may return a different optimal vertex than the
-reference's HiGHS.jl build; every returned table is checked for detailed
-balance and normalisation by the tests instead.
+is solved by the library's C++ builder (sse_vertex_transitions: a dense
+two-phase simplex) or, with lp_solver="highs", by the HiGHS dual simplex
+shipped in scipy.  When an LP optimum is degenerate either may return a
+different optimal vertex than the reference's HiGHS.jl build; every returned
+table is checked for detailed balance and normalisation by the tests instead.
 """
 from __future__ import annotations
 
@@ -114,9 +115,31 @@ def _solve_lp(cost, A, b, lp_tolerance):
     return res.x
 
 
+def construct_transitions_native(weights, leg_states, max_worm_count, dims, tolerance,
+                                 lp_tolerance):
+    """construct_transitions by the library's C++ builder (sse_vertex_transitions:
+    the same LPs, orders and rules, solved by a dense simplex on the host).
+    Also returns the builder's flags (bit 2: some LP optimum may not be
+    unique)."""
+    from .capi import vertex_transitions
+    off, ln, cum, tg, ol, ow, flags = vertex_transitions(dims, leg_states, weights, tolerance,
+                                                         lp_tolerance)
+    transitions = {}
+    nv, mw, legs = off.shape
+    for v, worm, leg in zip(*np.nonzero(off)):
+        transitions[(int(leg) + 1, int(worm) + 1, int(v) + 1)] = (int(off[v, worm, leg]),
+                                                                  int(ln[v, worm, leg]))
+    if flags & 2:
+        warnings.warn("normalization error in a transition exit list")
+    if flags & 1:
+        warnings.warn("had to truncate some probabilities in a possibly inaccurate way!")
+    return (transitions, [float(x) for x in cum], [int(t) for t in tg],
+            [(int(a), int(b)) for a, b in zip(ol, ow)], flags)
+
+
 def construct_transitions(weights, leg_states, max_worm_count, dims, tolerance, lp_tolerance):
     """Transition tables from one LP per class of vertices connected by a
-    single worm step (vertex_data.jl:213-416).
+    single worm step (vertex_data.jl:213-416), each LP solved by scipy's HiGHS.
 
     Returns (transitions, cumprobs, targets, step_outs) where transitions maps
     (leg_in, worm_in, vertex) -> (offset, length) with a 1-based offset and
@@ -246,7 +269,7 @@ class VertexData:
     leg_states (2*NSites x n_vertices, 1-based states)."""
 
     def __init__(self, dims, bond_hamiltonian, energy_offset_factor=0.25, tolerance=1e-7,
-                 lp_tolerance=1e-10):
+                 lp_tolerance=1e-10, lp_solver="native"):
         H = np.asarray(bond_hamiltonian, dtype=np.float64)
         dims = tuple(int(d) for d in dims)
         total_dim = int(np.prod(dims))
@@ -258,9 +281,17 @@ class VertexData:
         (self.diagonal_vertices, self.weights, self.leg_states, self.signs) = \
             construct_vertices(dims, H, self.energy_offset, tolerance)
         assert self.leg_states.shape == (2 * len(dims), len(self.weights))
-        (self.transitions, self.transition_cumprobs, self.transition_targets,
-         self.transition_step_outs) = construct_transitions(
-            self.weights, self.leg_states, self.max_worm_count, dims, tolerance, lp_tolerance)
+        args = (self.weights, self.leg_states, self.max_worm_count, dims, tolerance, lp_tolerance)
+        self.lp_solver = lp_solver
+        self.lp_flags = 0
+        if lp_solver == "native":
+            (self.transitions, self.transition_cumprobs, self.transition_targets,
+             self.transition_step_outs, self.lp_flags) = construct_transitions_native(*args)
+        elif lp_solver == "highs":
+            (self.transitions, self.transition_cumprobs, self.transition_targets,
+             self.transition_step_outs) = construct_transitions(*args)
+        else:
+            raise ValueError(f"lp_solver {lp_solver!r}: 'native' or 'highs'")
 
     # accessors (vertex_data.jl:91-104)
     def vertex_count(self):
