@@ -6,7 +6,9 @@ summed over all tables) agrees to 1e-9.  The native tables satisfy detailed
 balance and normalisation; where the builder reports every optimum unique,
 its tables equal HiGHS's entry for entry.  Models: the reference's test jobs
 (spin-1/2 square with field, mixed spin-1/2 / spin-1 honeycomb with Dz and
-Dx, dimer-basis bilayer), the dimer ladder, and random Hamiltonians."""
+Dx, dimer-basis bilayer), the dimer ladder, and random Hamiltonians (also of
+a three-site bond: the tables are generic in the number of bond sites, the
+device kernels are not)."""
 import itertools
 
 import numpy as np
@@ -37,7 +39,7 @@ def _bond_hamiltonians(sse_pkg):
     heis = np.kron(sz, sz) + 0.5 * (np.kron(sp, sp.T) + np.kron(sp.T, sp))
     captured.append(((2, 2), heis, {"energy_offset_factor": 0.0}))
     rng = np.random.default_rng(17)
-    for dims in ((2, 2), (2, 3), (3, 3), (2, 4), (4, 4)):
+    for dims in ((2, 2), (2, 3), (3, 3), (2, 4), (2, 2, 2), (4, 4)):
         D = int(np.prod(dims))
         H = rng.random((D, D))
         captured.append((dims, H + H.T, {}))
@@ -106,7 +108,7 @@ def test_every_step_has_a_transition(sse_pkg, hamiltonians):
     """Each (vertex, step) whose inverse step reaches a vertex has an entry."""
     u = sse_pkg.util
     VD = sse_pkg.vertex_data.VertexData
-    for dims, H, kw in hamiltonians[-5:]:
+    for dims, H, kw in hamiltonians[-6:]:
         vd = VD(dims, H, **kw)
         ns = len(dims)
         for v in range(1, vd.vertex_count() + 1):
