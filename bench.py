@@ -201,7 +201,6 @@ def run_batch(sse, flats, wds, thermalization, steps, what):
 def extra_heterogeneous(sse, walkers=2048, thermalization=500, steps=20, seed=3):
     """2048 walkers, square L in {8,12,16,24,32}, beta log-uniform [0.5,16],
     XXZ with Jz ~ U[0.5,1.5], Jxy=1, h=0.1; one model per walker."""
-    from concurrent.futures import ProcessPoolExecutor
     rng = np.random.default_rng(seed)
     sizes = rng.choice([8, 12, 16, 24, 32], size=walkers)
     betas = np.exp(rng.uniform(np.log(0.5), np.log(16.0), size=walkers))
@@ -211,8 +210,9 @@ def extra_heterogeneous(sse, walkers=2048, thermalization=500, steps=20, seed=3)
                                 "measure": ["magnetization", "staggered_magnetization"]})
             for L in (8, 12, 16, 24, 32)}
     base_flat = {L: m.flat() for L, m in base.items()}
-    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        types = list(ex.map(_xxz_types, [(float(j), 0.1) for j in jzs], chunksize=32))
+    t0 = time.perf_counter()
+    types = [_xxz_types((float(j), 0.1)) for j in jzs]  # native transition LPs
+    log(f"heterogeneous: {walkers} per-walker tables built in {time.perf_counter() - t0:.1f} s")
     from sse_amd.sse_data import FlatModel, SSEData
     bonds = {L: [sse.SSEBond(b.type, (b.i, b.j)) for b in m.lattice.bonds] for L, m in base.items()}
     flats, wds = [], []
@@ -233,11 +233,11 @@ def extra_heterogeneous(sse, walkers=2048, thermalization=500, steps=20, seed=3)
 def extra_ladder(sse, walkers=1024, thermalization=500, steps=20, seed=4):
     """1024 walkers of a 64-rung ladder in the dimer basis (local dim 4),
     J_rung=1, J_leg ~ U[0.2,1.0], beta=10."""
-    from concurrent.futures import ProcessPoolExecutor
     rng = np.random.default_rng(seed)
     jls = rng.uniform(0.2, 1.0, size=walkers)
-    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        flats = list(ex.map(_ladder_flat, [float(j) for j in jls], chunksize=16))
+    t0 = time.perf_counter()
+    flats = [_ladder_flat(float(j)) for j in jls]  # native transition LPs
+    log(f"ladder: {walkers} per-walker models built in {time.perf_counter() - t0:.1f} s")
     wds = [sse.walker_desc(model=k, T=0.1, seed=200000 + k, thermalization=thermalization)
            for k in range(walkers)]
     r = run_batch(sse, flats, wds, thermalization, steps, "ladder")
