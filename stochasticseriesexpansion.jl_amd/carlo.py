@@ -200,7 +200,8 @@ def calc_rebin_count(sample_count, min_bin_count=10):
 
 
 def _nan_to_none(x):
-    if isinstance(x, float) and math.isnan(x):
+    """JSON has no NaN or infinity: such values are written as null."""
+    if isinstance(x, float) and not math.isfinite(x):
         return None
     return x
 

@@ -212,8 +212,10 @@ class Group:
             arr = np.array(value, dtype=np.float64)
             scalar = True
         else:
-            arr = np.ascontiguousarray(value)
-            scalar = False
+            arr = np.asarray(value)
+            scalar = arr.ndim == 0  # (ascontiguousarray would make it 1-d)
+            if not scalar:
+                arr = np.ascontiguousarray(arr)
         if name in self:
             _ok(lib.H5Ldelete(self.hid, name.encode(), H5P_DEFAULT), f"replace {name}")
         if scalar:

@@ -50,6 +50,7 @@ def test_h5_roundtrip_and_layout(sse_pkg, carlo, tmp_path):
             g.write("operators", codes, compound_member="code")
             g.write("state", np.array([1, 2, 2, 1], dtype=np.uint8))
             g.write("num_worms", 3.5)  # replaces
+            g.write("zero_d", np.array(4, dtype=np.uint64))
     with h5.File(path, "r") as f:
         assert "simulation" in f and "nothing" not in f
         with f.group("simulation") as g:
@@ -61,6 +62,8 @@ def test_h5_roundtrip_and_layout(sse_pkg, carlo, tmp_path):
             assert g.dataset_info("num_worms") == (h5.H5T_FLOAT, 8, None, ())
             assert g.dataset_info("operators") == (h5.H5T_COMPOUND, 8, "code", (4,))
             assert g.dataset_info("state") == (h5.H5T_INTEGER, 1, None, (4,))
+            assert g.dataset_info("zero_d") == (h5.H5T_INTEGER, 8, None, ())
+            assert g.read("zero_d") == 4
             with pytest.raises(KeyError):
                 g.read("missing")
 
