@@ -179,7 +179,7 @@ inline int build_transitions(int ns, const int32_t *dims, int nv, const uint8_t 
     const int legs = 2 * ns;
     int max_worm = 0;
     for (int s = 0; s < ns; ++s) {
-        if (dims[s] < 1)
+        if (dims[s] < 1 || dims[s] > 255)  // leg states are bytes
             return kVdBadInput;
         max_worm = dims[s] - 1 > max_worm ? dims[s] - 1 : max_worm;
     }
