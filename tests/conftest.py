@@ -15,6 +15,10 @@ sse = __graft_entry__.load_package()
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
     config.addinivalue_line("markers", "slow: long-running statistical test")
+    # temporary files of the tests (checkpoint and results files) stay inside
+    # the repository tree, in an ignored folder that pytest clears per run
+    if not config.option.basetemp:
+        config.option.basetemp = os.path.join(ROOT, ".pytest_tmp")
 
 
 @pytest.fixture(scope="session")
