@@ -173,7 +173,9 @@ def bin_walkers(batch, binsize, n_bins):
     `binsize` consecutive per-sweep measurements (accumulated in sweep order
     on the device, divided by the bin length, as Carlo's accumulator forms a
     bin).  Every walker must be past thermalization; a walker that measured
-    fewer sweeps in a bin raises ValueError."""
+    fewer sweeps in a bin raises ValueError.  The batch's accumulators are
+    reset before the first bin and after every bin (they hold one bin at a
+    time, so each bin's sum is formed in sweep order from zero)."""
     if binsize < 1 or n_bins < 1:
         raise ValueError("binsize and n_bins must be positive")
     out = [np.zeros((n_bins, batch.flats[w.model].measure_width)) for w in batch.walkers]
