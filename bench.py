@@ -7,9 +7,11 @@ beta=8, Philox seeds 0..4095, random initial spins, cutoff grown by the
 reference rule, 2000 thermalization sweeps (untimed setup).  One step = one
 sweep (diagonal update + vertex list + worm update) of every walker followed
 by the measurement (energy, uniform and staggered magnetization family),
-accumulated on the device.  The timed steps run in calls of --chunk sweeps;
-the walkers are split into --streams groups, each swept on its own HIP stream
-so one group's slow worm-update tail overlaps the others' sweeps.
+accumulated on the device.  The timed steps run as one call (or in calls of
+--chunk sweeps); the walkers are split into --streams groups, each swept on
+its own HIP stream so one group's slow worm-update tail overlaps the others'
+sweeps.  Four groups match the four hardware queues a process gets by
+default; more streams than queues share queues and serialize.
 
 Reported beside it, in the same run:
   * the plain GPU form: the same walkers (thermalized states) run one walker
@@ -273,9 +275,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--chunk", type=int, default=20,
-                    help="sweeps per timed call")
-    ap.add_argument("--streams", type=int, default=2,
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="sweeps per timed call (0: all timed steps in one call)")
+    ap.add_argument("--streams", type=int, default=4,
                     help="walker groups, each swept on its own HIP stream")
     ap.add_argument("--walkers", type=int, default=4096)
     ap.add_argument("--lattice", type=int, default=16)
@@ -313,14 +315,14 @@ def main():
         sc, _, _ = batch.get_walker(0, with_ops=False, with_spins=False)
         log(f"thermalization {done}/{args.thermalization} sweeps ({time.perf_counter() - t0:.1f} s)"
             f" walker0 L={sc['cutoff']} n={sc['num_operators']} worms={sc['num_worms']:.2f}")
+    # warm-up and timed region with the walkers split over `streams` stream
+    # groups; the timed region is `steps` sweeps (+ fused measurement) of every
+    # walker in calls of `chunk` sweeps
+    batch.set_option(sse.capi.SSE_OPT_STREAMS, args.streams)
     if args.warmup:
         batch.run(args.warmup, measure=True)
     batch.reset_accumulators()
-
-    # timed region: `steps` sweeps (+ fused measurement) of every walker, in
-    # calls of `chunk` sweeps, walkers split over `streams` stream groups
-    batch.set_option(sse.capi.SSE_OPT_STREAMS, args.streams)
-    chunk = max(1, min(args.chunk, args.steps))
+    chunk = max(1, min(args.chunk if args.chunk > 0 else args.steps, args.steps))
     call_ms, dev_ms = [], 0.0
     done = 0
     while done < args.steps:

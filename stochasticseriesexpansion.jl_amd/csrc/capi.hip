@@ -89,8 +89,9 @@ struct sse_batch {
     double phase_ms[3] = {0, 0, 0};
     int64_t phase_launches = 0;
     // wave mode: walkers split into groups, each swept on its own stream, so
-    // one group's sweeps overlap the slow tail of another's
-    int n_groups = 2;
+    // one group's sweeps overlap the slow tail of another's; one group per
+    // hardware queue of the default HIP configuration
+    int n_groups = 4;
     unsigned long long seen_max_n = 0, seen_max_L = 0;  // operator counts behind the LDS size
     std::vector<hipStream_t> gstreams;
     std::vector<hipEvent_t> gjoin;
