@@ -282,16 +282,16 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
  *                take (walker, sweep) tasks in order).  Disabled in this build:
  *                setting it to 1 returns SSE_ERR_INVALID (its first GPU run
  *                faulted and the cause is not yet found).
- * SSE_OPT_STREAMS (default 4): wave mode splits the walkers into this many
+ * SSE_OPT_STREAMS (default 0): wave mode splits the walkers into this many
  *                groups, each swept on its own HIP stream (forked from and
  *                joined into the batch stream), so the slow tail of one
- *                group's worm update overlaps the other groups' sweeps.  The
- *                default matches the 4 hardware queues a process gets by
- *                default; streams beyond the queues share them and
- *                serialize (use fewer groups when other streams of the
- *                process run kernels at the same time).  1 = one stream
- *                (graphs and per-kernel timing apply only then).  Results
- *                are identical for any value.
+ *                group's worm update overlaps the other groups' sweeps.
+ *                0 = automatic: one group per 512 walkers, at least 2 and at
+ *                most 4 (the 4 hardware queues a process gets by default;
+ *                streams beyond the queues share them and serialize, and
+ *                smaller groups leave CUs idle).  1 = one stream (graphs and
+ *                per-kernel timing apply only then).  Results are identical
+ *                for any value.
  * SSE_OPT_K2_LDS (default 0): LDS bytes per walker of the worm kernel; 0 =
  *                sized from the batch's largest operator count (then widened to
  *                the largest size with the same walkers per CU), and with

@@ -89,9 +89,9 @@ struct sse_batch {
     double phase_ms[3] = {0, 0, 0};
     int64_t phase_launches = 0;
     // wave mode: walkers split into groups, each swept on its own stream, so
-    // one group's sweeps overlap the slow tail of another's; one group per
-    // hardware queue of the default HIP configuration
-    int n_groups = 4;
+    // one group's sweeps overlap the slow tail of another's (0: automatic,
+    // see group_count)
+    int n_groups = 0;
     unsigned long long seen_max_n = 0, seen_max_L = 0;  // operator counts behind the LDS size
     std::vector<hipStream_t> gstreams;
     std::vector<hipEvent_t> gjoin;
@@ -965,10 +965,21 @@ static int call_timer(sse_batch *b, int stage, int64_t sweeps)
 // everything else (copies, syncs, the next call) stays ordered on it.  Sweeps
 // are issued round-robin over the groups; a group whose slowest walker is
 // still in its worm update leaves the rest of the device to the others.
+// Walker groups of a wave-mode sweep: the SSE_OPT_STREAMS value, or
+// automatically one group per 512 walkers, at least 2 and at most 4 (the
+// hardware queues of the default HIP configuration; smaller groups would
+// leave CUs idle while they drain).
+static int group_count(const sse_batch *b)
+{
+    const int n = b->n_walkers;
+    const int g = b->n_groups > 0 ? b->n_groups : std::min(4, std::max(2, n / 512));
+    return std::max(1, std::min(g, n));
+}
+
 static int groups_run(sse_batch *b, int n_sweeps, int measure)
 {
     const int n = b->n_walkers;
-    const int G = std::max(1, std::min(b->n_groups, n));
+    const int G = group_count(b);
     // group 0 runs on the batch stream itself: G groups use G streams (the
     // device maps streams onto a few hardware queues; streams sharing one
     // queue would serialize behind each other)
@@ -1115,7 +1126,7 @@ static int sweep_call(sse_batch *b, int32_t n_sweeps, int32_t measure)
         if (rc)
             return rc;
     }
-    const bool grouped = b->mode == SSE_MODE_WAVE && b->n_groups > 1 && b->n_walkers > 1;
+    const bool grouped = b->mode == SSE_MODE_WAVE && group_count(b) > 1;
     const bool timed = !grouped && b->use_timing && n_sweeps <= 256;
     const bool pt = b->pt_ladders > 0 && b->pt_interval > 0;
     int rc0 = call_timer(b, 0, 0);
@@ -1666,8 +1677,8 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
             g = 0;
         return SSE_OK;
     case SSE_OPT_STREAMS:
-        if (value < 1 || value > 64)
-            return fail(SSE_ERR_INVALID, "SSE_OPT_STREAMS takes 1..64 walker groups");
+        if (value < 0 || value > 64)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_STREAMS takes 0 (automatic) or 1..64 walker groups");
         HIPCHECK(hipStreamSynchronize(b->stream));
         b->n_groups = (int)value;
         for (size_t &g : b->wave.lds_k2_group)  // the groups hold other walkers now
