@@ -10,8 +10,10 @@ by the measurement (energy, uniform and staggered magnetization family),
 accumulated on the device.  The timed steps run as one call (or in calls of
 --chunk sweeps); the walkers are split into --streams groups, each swept on
 its own HIP stream so one group's slow worm-update tail overlaps the others'
-sweeps.  Four groups match the four hardware queues a process gets by
-default; more streams than queues share queues and serialize.
+sweeps.  The process asks for --hw-queues (8) hardware queues
+(GPU_MAX_HW_QUEUES) and by default the library makes one group per 512
+walkers up to one per queue (8 for 4096 walkers); more streams than queues
+would share queues and serialize.
 
 Reported beside it, in the same run:
   * the plain GPU form: the same walkers (thermalized states) run one walker
@@ -277,8 +279,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--chunk", type=int, default=0,
                     help="sweeps per timed call (0: all timed steps in one call)")
-    ap.add_argument("--streams", type=int, default=4,
-                    help="walker groups, each swept on its own HIP stream")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="walker groups, each swept on its own HIP stream (0: the "
+                         "library's choice, one per 512 walkers up to one per hardware queue)")
+    ap.add_argument("--hw-queues", type=int, default=8,
+                    help="GPU_MAX_HW_QUEUES for this process (HIP's default is 4): set "
+                         "before the HIP runtime starts, so each walker group's stream "
+                         "gets a hardware queue of its own")
     ap.add_argument("--walkers", type=int, default=4096)
     ap.add_argument("--lattice", type=int, default=16)
     ap.add_argument("--beta", type=float, default=8.0)
@@ -299,6 +306,11 @@ def main():
               f"chains, so N GPUs run N independent batches.", file=sys.stderr)
         sys.exit(2)
 
+    if not 1 <= args.hw_queues <= 32:
+        print("--hw-queues takes 1..32", file=sys.stderr)
+        sys.exit(2)
+    # read by the HIP runtime when it starts (the library's first call below)
+    os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     import __graft_entry__
     sse = __graft_entry__.load_package()
     flat, wds = build_workload(sse, args.lattice, args.beta, args.walkers, args.thermalization)
@@ -319,6 +331,8 @@ def main():
     # groups; the timed region is `steps` sweeps (+ fused measurement) of every
     # walker in calls of `chunk` sweeps
     batch.set_option(sse.capi.SSE_OPT_STREAMS, args.streams)
+    queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4)
+    groups = args.streams or min(queues, max(2, args.walkers // 512))  # the library's rule
     if args.warmup:
         batch.run(args.warmup, measure=True)
     batch.reset_accumulators()
@@ -399,7 +413,8 @@ def main():
         "data": "synthetic (Philox-seeded random initial spins, thermalized chains)",
         "config": {"workload": f"batched Heisenberg {args.lattice}x{args.lattice} beta={args.beta:g}",
                    "walkers": args.walkers, "thermalization_sweeps": args.thermalization,
-                   "parallelism": f"one wavefront per walker, {args.streams} stream groups"},
+                   "parallelism": f"one wavefront per walker, {groups} stream groups on "
+                                  f"{os.environ.get('GPU_MAX_HW_QUEUES')} hardware queues"},
         "sweeps_per_s": wsweeps / total_s,
         "hbm_gbs_algorithmic": B / total_s / 1e9,
         "hbm_roofline_frac": B / total_s / 1e9 / HBM_PEAK_GBS,

@@ -287,9 +287,9 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
  *                joined into the batch stream), so the slow tail of one
  *                group's worm update overlaps the other groups' sweeps.
  *                0 = automatic: one group per 512 walkers, at least 2 and at
- *                most 4 (the 4 hardware queues a process gets by default;
- *                streams beyond the queues share them and serialize, and
- *                smaller groups leave CUs idle).  1 = one stream (graphs and
+ *                most one per hardware queue (GPU_MAX_HW_QUEUES, 4 when
+ *                unset; streams beyond the queues share them and serialize,
+ *                and smaller groups leave CUs idle).  1 = one stream (graphs and
  *                per-kernel timing apply only then).  Results are identical
  *                for any value.
  * SSE_OPT_K2_LDS (default 0): LDS bytes per walker of the worm kernel; 0 =

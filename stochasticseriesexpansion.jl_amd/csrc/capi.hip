@@ -965,14 +965,25 @@ static int call_timer(sse_batch *b, int stage, int64_t sweeps)
 // everything else (copies, syncs, the next call) stays ordered on it.  Sweeps
 // are issued round-robin over the groups; a group whose slowest walker is
 // still in its worm update leaves the rest of the device to the others.
+// Hardware queues the HIP runtime gives this process: GPU_MAX_HW_QUEUES, 4
+// when unset (HIP's default).  Streams beyond it share queues and serialize.
+static int hw_queues()
+{
+    static const int q = [] {
+        const char *e = getenv("GPU_MAX_HW_QUEUES");
+        const int v = e ? atoi(e) : 0;
+        return v > 0 ? v : 4;
+    }();
+    return q;
+}
+
 // Walker groups of a wave-mode sweep: the SSE_OPT_STREAMS value, or
-// automatically one group per 512 walkers, at least 2 and at most 4 (the
-// hardware queues of the default HIP configuration; smaller groups would
-// leave CUs idle while they drain).
+// automatically one group per 512 walkers, at least 2 and at most one per
+// hardware queue (smaller groups would leave CUs idle while they drain).
 static int group_count(const sse_batch *b)
 {
     const int n = b->n_walkers;
-    const int g = b->n_groups > 0 ? b->n_groups : std::min(4, std::max(2, n / 512));
+    const int g = b->n_groups > 0 ? b->n_groups : std::min(hw_queues(), std::max(2, n / 512));
     return std::max(1, std::min(g, n));
 }
 
