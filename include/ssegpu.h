@@ -174,6 +174,13 @@ int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure);
 int sse_read_accumulators(sse_batch *b, double *sums, double *sumsq,
                           int64_t *counts);
 int sse_reset_accumulators(sse_batch *b);
+/* WormLengthFraction = (sum of worm lengths) / n, which the reference
+ * measures inside the worm update of every thermalized sweep with n != 0
+ * (src/sse.jl:200-202): per walker ([n_walkers] each) its sum, sum of
+ * squares and sample count over the measured sweeps of sse_run since the
+ * last sse_reset_accumulators (slots follow the temperatures under
+ * parallel tempering, like the other sums).                                */
+int sse_read_worm_fraction(sse_batch *b, double *sums, double *sumsq, int64_t *counts);
 
 /* Walker state (checkpoint / validation).  ops: [cutoff] device operator
  * codes or NULL; spins: [n_sites] 1-based states or NULL.                   */

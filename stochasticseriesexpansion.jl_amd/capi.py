@@ -181,6 +181,8 @@ def load_library(path=None):
     lib.sse_read_accumulators.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                           C.POINTER(C.c_int64)]
     lib.sse_reset_accumulators.argtypes = [vp]
+    lib.sse_read_worm_fraction.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                           C.POINTER(C.c_int64)]
     lib.sse_get_walker.argtypes = [vp, C.c_int32, C.POINTER(WalkerScalars),
                                    C.POINTER(C.c_uint32), C.c_int64, C.POINTER(C.c_uint8)]
     lib.sse_set_walker.argtypes = [vp, C.c_int32, C.POINTER(WalkerScalars),
@@ -232,7 +234,7 @@ EXPORTED_SYMBOLS = (
     "sse_slot_start", "sse_slot_stop", "sse_slot_poll", "sse_pool_usage", "sse_set_option",
     "sse_call_time", "sse_lds_bytes", "sse_pt_log_weight_ratio", "sse_pt_change_parameter",
     "sse_pt_get_parameter", "sse_pt_set_ladders", "sse_pt_exchange", "sse_pt_get_ladders",
-    "sse_vertex_transitions")
+    "sse_vertex_transitions", "sse_read_worm_fraction")
 
 
 def vertex_transitions(dims, leg_states, weights, tolerance, lp_tolerance):
@@ -345,6 +347,17 @@ class Batch:
             out.append((s[k:k + cw], s2[k:k + cw], int(cnt[w])))
             k += wd
         return out
+
+    def worm_fractions(self):
+        """Per walker (sum, sum of squares, count) of WormLengthFraction over
+        the measured sweeps since the last reset (sse.jl:200-202)."""
+        n = len(self.walkers)
+        s, s2 = np.zeros(n), np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int64)
+        check(self.lib, self.lib.sse_read_worm_fraction(self.handle, _ptr(s, C.c_double),
+                                                        _ptr(s2, C.c_double),
+                                                        _ptr(cnt, C.c_int64)))
+        return [(float(s[k]), float(s2[k]), int(cnt[k])) for k in range(n)]
 
     def reset_accumulators(self):
         check(self.lib, self.lib.sse_reset_accumulators(self.handle))

@@ -18,7 +18,8 @@ Checkpoints
 
 Results
   bin_walkers       internal bins: means over `binsize` consecutive measured
-                    sweeps of every walker (Carlo's accumulator bins)
+                    sweeps of every walker (Carlo's accumulator bins), with
+                    WormLengthFraction as the last column
   merge_observable  Carlo's merge of a task's internal bins: rebin count,
                     rebin length, mean, error, autocorrelation time
   jackknife         bias-corrected jackknife over the rebinned means
@@ -167,26 +168,42 @@ def load_run(path, batch, walker, check_key=True):
 
 
 # ---------------------------------------------------------------- binning
+WORM_LENGTH_FRACTION = "WormLengthFraction"
+
+
+def binned_names(flat):
+    """Column names of bin_walkers' bins: the measured observables, then
+    WormLengthFraction."""
+    return list(flat.observable_names()) + [WORM_LENGTH_FRACTION]
+
+
 def bin_walkers(batch, binsize, n_bins):
     """Run n_bins * binsize sweeps and return every walker's internal bins: a
-    list with one [bin][measure width] array per walker, each bin the mean of
-    `binsize` consecutive per-sweep measurements (accumulated in sweep order
-    on the device, divided by the bin length, as Carlo's accumulator forms a
-    bin).  Every walker must be past thermalization; a walker that measured
-    fewer sweeps in a bin raises ValueError.  The batch's accumulators are
-    reset before the first bin and after every bin (they hold one bin at a
-    time, so each bin's sum is formed in sweep order from zero)."""
+    list with one [bin][measure width + 1] array per walker (columns as
+    binned_names), each bin the mean of `binsize` consecutive per-sweep
+    measurements (accumulated in sweep order on the device, divided by the
+    bin length, as Carlo's accumulator forms a bin).  The last column is the
+    bin's mean WormLengthFraction, which the reference measures only in
+    sweeps with n != 0 (sse.jl:200-202): its mean over the samples the bin
+    has (NaN without any).  Every walker must be past thermalization; a walker
+    that measured fewer sweeps in a bin raises ValueError.  The batch's
+    accumulators are reset before the first bin and after every bin (they
+    hold one bin at a time, so each bin's sum is formed in sweep order from
+    zero)."""
     if binsize < 1 or n_bins < 1:
         raise ValueError("binsize and n_bins must be positive")
-    out = [np.zeros((n_bins, batch.flats[w.model].measure_width)) for w in batch.walkers]
+    out = [np.zeros((n_bins, batch.flats[w.model].measure_width + 1)) for w in batch.walkers]
     batch.reset_accumulators()
     for b in range(n_bins):
         batch.run(binsize, measure=True)
+        wlf = batch.worm_fractions()
         for k, (s, _, cnt) in enumerate(batch.accumulators()):
             if cnt != binsize:
                 raise ValueError(f"walker {k} measured {cnt} of {binsize} sweeps in bin {b}: "
                                  "thermalize the batch first")
-            out[k][b] = s / binsize
+            out[k][b, :-1] = s / binsize
+            ws, _, wc = wlf[k]
+            out[k][b, -1] = ws / wc if wc else float("nan")
         batch.reset_accumulators()
     return out
 
@@ -383,6 +400,6 @@ def run_tasks(tasks, runs_per_task=1, seed0=0, mode=0, batch_cls=None, progress=
                     for r in range(runs_per_task)]
             params = {key: v for key, v in tasks[t].items() if key not in ("model", "name")}
             entries[t] = {"task": tasks[t]["name"], "parameters": params,
-                          "results": evaluate_task(flat.observable_names(), runs, binsize,
+                          "results": evaluate_task(binned_names(flat), runs, binsize,
                                                    flat.normalization_site_count)}
     return entries

@@ -69,6 +69,7 @@ struct sse_batch {
     int32_t *d_first = nullptr, *d_last = nullptr;
     double *d_out = nullptr, *d_sum = nullptr, *d_sumsq = nullptr;
     int64_t *d_count = nullptr;
+    double *d_wlf = nullptr;  // [slot][3]: WormLengthFraction sum, sum of squares, count
     int32_t *d_done = nullptr;
     uint8_t *d_scratch = nullptr;
     bool any_pending = true;
@@ -713,6 +714,8 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
     if (!rc)
         rc = dev_alloc(b, &b->d_count, (size_t)n_walkers);
     if (!rc)
+        rc = dev_alloc(b, &b->d_wlf, 3 * (size_t)n_walkers);
+    if (!rc)
         rc = dev_alloc(b, &b->d_done, (size_t)n_walkers);
     if (!rc)
         rc = dev_alloc(b, &b->d_walkers, (size_t)n_walkers);
@@ -787,6 +790,7 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
         (void)bmemset(b, b->d_sum, 0, sizeof(double) * b->total_width);
         (void)bmemset(b, b->d_sumsq, 0, sizeof(double) * b->total_width);
         (void)bmemset(b, b->d_count, 0, sizeof(int64_t) * n_walkers);
+        (void)bmemset(b, b->d_wlf, 0, sizeof(double) * 3 * n_walkers);
         (void)bmemset(b, b->d_done, 0, sizeof(int32_t) * n_walkers);
         rc = wave_configure(b->wave, b->h_models, b->max_sites);
     }
@@ -1025,7 +1029,8 @@ static int groups_run(sse_batch *b, int n_sweeps, int measure)
                                        nullptr, ord, g);
             if (!rc && measure)
                 rc = wave_launch_measure(b->wave, gs, b->d_walkers, b->d_models, b->d_out, b->d_sum,
-                                         b->d_sumsq, b->d_count, w0, w1, 1, b->d_done, ord);
+                                         b->d_sumsq, b->d_count, b->d_wlf, w0, w1, 1, b->d_done,
+                                         ord);
             if (rc)
                 return fail(rc, "group sweep launch failed");
         }
@@ -1215,11 +1220,11 @@ static int launch_measure(sse_batch *b, int mode)
     if (b->mode == SSE_MODE_LANE) {
         hipLaunchKernelGGL(lane_measure_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream,
                            b->d_walkers, b->d_models, b->d_out, b->d_sum, b->d_sumsq, b->d_count,
-                           b->d_scratch, b->max_sites, n, mode, b->d_done);
+                           b->d_wlf, b->d_scratch, b->max_sites, n, mode, b->d_done);
     } else {
         int rc = wave_launch_measure(b->wave, b->stream, b->d_walkers, b->d_models, b->d_out,
-                                     b->d_sum, b->d_sumsq, b->d_count, 0, n, mode, b->d_done,
-                                     b->use_order ? b->d_order : nullptr);
+                                     b->d_sum, b->d_sumsq, b->d_count, b->d_wlf, 0, n, mode,
+                                     b->d_done, b->use_order ? b->d_order : nullptr);
         if (rc)
             return rc;
     }
@@ -1277,6 +1282,21 @@ int sse_read_accumulators(sse_batch *b, double *sums, double *sumsq, int64_t *co
     return SSE_OK;
 }
 
+int sse_read_worm_fraction(sse_batch *b, double *sums, double *sumsq, int64_t *counts)
+{
+    if (!b || !sums || !sumsq || !counts)
+        return fail(SSE_ERR_INVALID, "sse_read_worm_fraction: bad arguments");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    std::vector<double> h(3 * (size_t)b->n_walkers);
+    HIPCHECK(bmemcpy(b, h.data(), b->d_wlf, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int w = 0; w < b->n_walkers; ++w) {
+        sums[w] = h[3 * (size_t)w];
+        sumsq[w] = h[3 * (size_t)w + 1];
+        counts[w] = (int64_t)h[3 * (size_t)w + 2];
+    }
+    return SSE_OK;
+}
+
 int sse_reset_accumulators(sse_batch *b)
 {
     if (!b)
@@ -1284,6 +1304,7 @@ int sse_reset_accumulators(sse_batch *b)
     HIPCHECK(hipMemsetAsync(b->d_sum, 0, sizeof(double) * b->total_width, b->stream));
     HIPCHECK(hipMemsetAsync(b->d_sumsq, 0, sizeof(double) * b->total_width, b->stream));
     HIPCHECK(hipMemsetAsync(b->d_count, 0, sizeof(int64_t) * b->n_walkers, b->stream));
+    HIPCHECK(hipMemsetAsync(b->d_wlf, 0, sizeof(double) * 3 * b->n_walkers, b->stream));
     hipLaunchKernelGGL(reset_work_counters_kernel, dim3((b->n_walkers + 63) / 64), dim3(64), 0,
                        b->stream, b->d_walkers, b->n_walkers);
     HIPCHECK(hipGetLastError());
@@ -1594,6 +1615,7 @@ int sse_slot_start(sse_batch *b, int32_t slot, const sse_walker_desc *wd, int64_
     HIPCHECK(bmemset(b, b->d_sum + x.meas_offset, 0, sizeof(double) * b->widths[slot]));
     HIPCHECK(bmemset(b, b->d_sumsq + x.meas_offset, 0, sizeof(double) * b->widths[slot]));
     HIPCHECK(bmemset(b, b->d_count + slot, 0, sizeof(int64_t)));
+    HIPCHECK(bmemset(b, b->d_wlf + 3 * (size_t)slot, 0, 3 * sizeof(double)));
     HIPCHECK(bmemset(b, b->d_done + slot, 0, sizeof(int32_t)));
     b->any_pending = true;
     if (n_sweeps > 0)

@@ -368,6 +368,21 @@ __device__ __forceinline__ void worm_controller(DevWalker &w, double total, bool
     }
 }
 
+// WormLengthFraction (sse.jl:200-202) of the last sweep into the slot's
+// accumulator {sum, sum of squares, count}: the reference measures it in
+// every thermalized sweep with n != 0 (last_wlf is NaN otherwise).  The slot
+// follows the temperature under parallel tempering, like the other sums.
+__device__ __forceinline__ void worm_fraction_accumulate(const DevWalker &w, double *wlf)
+{
+    const double v = w.last_wlf;
+    if (v == v) {
+        double *q = wlf + 3 * (size_t)w.acc_slot;
+        q[0] += v;
+        q[1] += v * v;
+        q[2] += 1.0;
+    }
+}
+
 // Growth rule of the cutoff (sse.jl:138-145): n >= L/2 -> L = floor(1.5 L + 100).
 __device__ __forceinline__ int64_t grown_cutoff(int64_t L) { return (3 * L) / 2 + 100; }
 

@@ -352,8 +352,9 @@ __device__ __forceinline__ void accumulate(double *sum, double *sumsq, const dou
 // mode 1: measure thermalized walkers (Carlo measures after a sweep once
 //         sweeps > thermalization) and accumulate into sums
 __global__ void lane_measure_kernel(DevWalker *W, const DevModel *models, double *out, double *sum,
-                                    double *sumsq, int64_t *count, uint8_t *scratch_spins,
-                                    int max_sites, int n_walkers, int mode, int32_t *done)
+                                    double *sumsq, int64_t *count, double *wlf,
+                                    uint8_t *scratch_spins, int max_sites, int n_walkers, int mode,
+                                    int32_t *done)
 {
     int wi = blockIdx.x * blockDim.x + threadIdx.x;
     if (wi >= n_walkers)
@@ -368,6 +369,7 @@ __global__ void lane_measure_kernel(DevWalker *W, const DevModel *models, double
         if (mode == 1) {
             accumulate(sum + w.meas_offset, sumsq + w.meas_offset, o, w.meas_width);
             count[w.acc_slot] += 1;
+            worm_fraction_accumulate(w, wlf);
         }
     }
     if (mode == 1 && w.retire) {

@@ -1625,7 +1625,8 @@ __device__ __forceinline__ void k3_pass(const DevWalker &w, const DevModel &m, i
 // (fused sweep+measure loop).  Estimators are taken K3_MAXE per pass over the
 // operator string; the first pass also computes the sign.
 __device__ __forceinline__ void k3_walker(DevWalker *W, const DevModel *models, double *out, double *sum,
-                                          double *sumsq, int64_t *count, int wi, int mode, int32_t *done)
+                                          double *sumsq, int64_t *count, double *wlf, int wi, int mode,
+                                          int32_t *done)
 {
     const int lane = threadIdx.x;
     const DevWalker w = W[wi];
@@ -1665,6 +1666,7 @@ __device__ __forceinline__ void k3_walker(DevWalker *W, const DevModel *models, 
         }
         if (lane == 0) {
             count[w.acc_slot] += 1;
+            worm_fraction_accumulate(w, wlf);
             if (w.retire) {
                 W[wi].retire = 0;
                 W[wi].active = 0;
@@ -1676,12 +1678,13 @@ __device__ __forceinline__ void k3_walker(DevWalker *W, const DevModel *models, 
 
 __global__ __launch_bounds__(64) void wave_measure_kernel(DevWalker *W, const DevModel *models,
                                                           double *out, double *sum, double *sumsq,
-                                                          int64_t *count, int w_begin, int w_end,
-                                                          int mode, int32_t *done, const int32_t *order)
+                                                          int64_t *count, double *wlf, int w_begin,
+                                                          int w_end, int mode, int32_t *done,
+                                                          const int32_t *order)
 {
     const int pos = w_begin + (int)blockIdx.x;
     if (pos < w_end)
-        k3_walker(W, models, out, sum, sumsq, count, walker_at(order, pos), mode, done);
+        k3_walker(W, models, out, sum, sumsq, count, wlf, walker_at(order, pos), mode, done);
 }
 
 // pending walkers become active once initialised
@@ -1806,11 +1809,11 @@ static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, 
 
 static inline int wave_launch_measure(WaveConfig &, hipStream_t s, DevWalker *W, const DevModel *M,
                                       double *out, double *sum, double *sumsq, int64_t *count,
-                                      int w0, int w1, int mode, int32_t *done,
+                                      double *wlf, int w0, int w1, int mode, int32_t *done,
                                       const int32_t *order = nullptr)
 {
     hipLaunchKernelGGL(wave_measure_kernel, dim3(w1 - w0), dim3(64), 0, s, W, M, out, sum, sumsq, count,
-                       w0, w1, mode, done, order);
+                       wlf, w0, w1, mode, done, order);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
 

@@ -122,33 +122,48 @@ def test_restored_state_outside_model_rejected(sse_pkg):
     b.close()
 
 
-def test_bins_equal_oracle_rows(sse_pkg, oracle_mod, carlo):
-    _say("carlo: device bins vs oracle")
+@pytest.mark.parametrize("mode", [0, 1])
+def test_bins_equal_oracle_rows(sse_pkg, oracle_mod, carlo, mode):
+    _say(f"carlo: device bins vs oracle mode={mode}")
     flat = square_heisenberg(sse_pkg, 6).flat()
     wd = [sse_pkg.walker_desc(T=0.4, seed=700 + s, thermalization=6) for s in range(4)]
-    b = sse_pkg.Batch([flat], wd)
+    b = sse_pkg.Batch([flat], wd, mode=mode)
     b.init()
     b.run(6, measure=False)
     bins = carlo.bin_walkers(b, 10, 5)
     b.close()
+    assert carlo.binned_names(flat)[-1] == "WormLengthFraction"
     for k, w in enumerate(wd):
         o = oracle_mod.OracleWalker(sse_pkg.capi, flat, w)
         o.init()
-        rows = o.run_series(6 + 50)
+        rows, wlf = [], []
+        for _ in range(6 + 50):
+            o.sweep(1)
+            sc = o.scalars()
+            if sc["sweeps"] > 6:
+                rows.append(o.measure())
+                wlf.append(sc["last_worm_length_fraction"])
+        rows = np.array(rows)
         assert len(rows) == 50
-        ref = np.zeros((5, rows.shape[1]))
+        ref = np.zeros((5, rows.shape[1] + 1))
         for i in range(5):
             acc = np.zeros(rows.shape[1])
             for r in rows[10 * i:10 * i + 10]:
                 acc = acc + r
-            ref[i] = acc / 10
-        assert np.allclose(bins[k], ref, rtol=1e-12, atol=1e-12), k
+            ref[i, :-1] = acc / 10
+            ws, wc = 0.0, 0
+            for v in wlf[10 * i:10 * i + 10]:
+                if v == v:  # measured in thermalized sweeps with n != 0
+                    ws, wc = ws + v, wc + 1
+            ref[i, -1] = ws / wc if wc else float("nan")
+        assert np.allclose(bins[k], ref, rtol=1e-12, atol=1e-12, equal_nan=True), k
+        assert np.isfinite(bins[k][:, -1]).all()
 
 
 def test_run_tasks_results_json_vs_ed(sse_pkg, carlo, tmp_path):
     """Two tasks of the reference's ED job (test/test_jobs.jl:4-29), 8 runs
     each, merged into results.json: Energy and SpecificHeat within 4 errors
-    of exact diagonalisation."""
+    of exact diagonalisation; WormLengthFraction merged like the others."""
     _say("carlo: run_tasks vs ED")
     Ts = [float(TS[2]), float(TS[4])]
     tasks = [{"name": f"task{i + 1:04d}", "model": job_magnet_square(sse_pkg, T), "T": T,
@@ -163,6 +178,7 @@ def test_run_tasks_results_json_vs_ed(sse_pkg, carlo, tmp_path):
         r = e["results"]
         assert r["OperatorCount"]["internal_bin_len"] == 100
         assert r["OperatorCount"]["rebin_count"] == carlo.calc_rebin_count(8 * 50)
+        assert r["WormLengthFraction"]["mean"] > 0 and r["WormLengthFraction"]["error"] > 0
         for name in ("Energy", "SpecificHeat"):
             mean, err = r[name]["mean"][0], r[name]["error"][0]
             z = abs(mean - exact[name][i]) / max(err, 1e-8)
