@@ -14,6 +14,21 @@
  * rand() Float64 = (word >> 11) * 2^-53; rand(1:s) = Lemire's nearly
  * divisionless method on 64-bit words (Lemire 2019).  In validation mode the
  * reference consumes the same words through its rng argument.
+ *
+ * What pins each part (tests/): Philox against the Random123 known-answer
+ * vectors and rocRAND's engine; the range sampler against its closed form
+ * and a chi-square test; so_tanh / so_log within 4 ulp of libm; the vertex
+ * list against the reference's golden fixture (test/test_vertex_list.jl);
+ * the chain (init, diagonal update, worm update, growth, controller) by
+ * worldline consistency after every sweep, checkpoint/resume equality and
+ * exact diagonalisation of the reference's three ED jobs (|z| <= 4); the
+ * measurement and estimators by the same ED comparison and the reference's
+ * published bani2v2o8 results; the parallel-tempering exchange by closed
+ * forms, an acceptance-rate check and ED per temperature.
+ * Parity unpinned: the draw-by-draw trace of the Julia reference under its
+ * own RNG (no Julia here; the PhiloxRNG shim contract is unverified), the
+ * FP64 bits of Julia's tanh in the controller, and Carlo's exchange
+ * schedule and its RNG use for parallel tempering.
  */
 #include "sse_oracle.h"
 
