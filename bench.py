@@ -259,15 +259,16 @@ def extra_serving(sse, seconds=30.0, sweep_scale=1e-3, slots=1024):
     # size classes by estimated operator count, each its own batch, host
     # thread and stream (small jobs do not wait behind large walkers)
     loop = sv.SizeClassServing(classes=((2000, slots * 3 // 16, 10, 48),
-                                        (8000, slots * 6 // 16, 4, 48),
+                                        (8000, slots * 5 // 16, 4, 48),
                                         (30000, slots * 5 // 16, 2, 48),
-                                        (float("inf"), slots * 2 // 16, 1, 48)))
+                                        (60000, slots * 2 // 16, 1, 48),
+                                        (float("inf"), slots * 1 // 16, 1, 48)))
     done, stats = loop.run(jobs, time_limit=seconds * 3,
                            log=lambda m: print(f"serving: {m}", file=sys.stderr, flush=True))
     loop.close()
     stats["workload"] = (f"serving: Poisson 200 jobs/s seed 7, L 8..48, beta 1..32, sweeps "
-                         f"log-uniform [1e3,1e5] x {sweep_scale:g}, {slots} slots in 4 size "
-                         f"classes (n <= 2000 / 8000 / 30000 / more), models prebuilt")
+                         f"log-uniform [1e3,1e5] x {sweep_scale:g}, {slots} slots in 5 size "
+                         f"classes (n <= 2000 / 8000 / 30000 / 60000 / more), models prebuilt")
     stats["jobs_submitted"] = len(jobs)
     return stats
 
