@@ -353,14 +353,22 @@ def write_results(path, entries):
         json.dump(out, f, indent=1, allow_nan=False)
 
 
-def run_tasks(tasks, runs_per_task=1, seed0=0, mode=0, batch_cls=None, progress=None):
+def run_tasks(tasks, runs_per_task=1, seed0=0, mode=0, batch_cls=None, progress=None,
+              pt_interval=0, pt_seed=1):
     """Run Carlo-style tasks on the GPU, every run of every task in one batch
     per (thermalization, sweeps, binsize) group.
 
     tasks: dicts with "name", "model" (a host model with .flat()), "T",
     "thermalization", "sweeps", "binsize" and any further parameters (kept
-    in the entry).  Run r of task t uses seed seed0 + t * runs_per_task + r.
-    Returns the results.json entries in task order."""
+    in the entry).  Tasks that pass the same model object share one device
+    model.  Run r of task t uses seed seed0 + t * runs_per_task + r.
+
+    pt_interval > 0 runs parallel tempering in T (the reference's hooks,
+    sse.jl:390-405): the tasks of a group must share one model object; run r
+    of every task forms ladder r (temperatures in increasing order) and an
+    exchange round follows every pt_interval-th sweep, thermalization
+    included.  A task's bins then belong to its temperature, whichever walker
+    holds it.  Returns the results.json entries in task order."""
     from .capi import Batch, walker_desc
     batch_cls = batch_cls or Batch
     groups = {}
@@ -371,16 +379,28 @@ def run_tasks(tasks, runs_per_task=1, seed0=0, mode=0, batch_cls=None, progress=
         groups.setdefault(key, []).append(t)
     entries = [None] * len(tasks)
     for (therm, sweeps, binsize), idx in groups.items():
-        flats = [tasks[t]["model"].flat() for t in idx]
+        flats, model_of = [], {}
+        for t in idx:
+            mid = id(tasks[t]["model"])
+            if mid not in model_of:
+                model_of[mid] = len(flats)
+                flats.append(tasks[t]["model"].flat())
+        if pt_interval and len(flats) != 1:
+            raise ValueError("parallel tempering: the tasks of a group must share one model object")
         walkers = []
-        for k, t in enumerate(idx):
+        for t in idx:
             for r in range(runs_per_task):
-                walkers.append(walker_desc(model=k, T=float(tasks[t]["T"]),
+                walkers.append(walker_desc(model=model_of[id(tasks[t]["model"])],
+                                           T=float(tasks[t]["T"]),
                                            seed=seed0 + t * runs_per_task + r,
                                            thermalization=therm))
         batch = batch_cls(flats, walkers, mode=mode)
         try:
             batch.init()
+            if pt_interval:
+                order = sorted(range(len(idx)), key=lambda k: float(tasks[idx[k]]["T"]))
+                batch.pt_set_ladders([[k * runs_per_task + r for k in order]
+                                      for r in range(runs_per_task)], pt_seed, pt_interval)
             if therm:
                 batch.run(therm, measure=False)
             n_bins = sweeps // binsize
@@ -395,10 +415,12 @@ def run_tasks(tasks, runs_per_task=1, seed0=0, mode=0, batch_cls=None, progress=
         finally:
             batch.close()
         for k, t in enumerate(idx):
-            flat = flats[k]
+            flat = flats[model_of[id(tasks[t]["model"])]]
             runs = [np.concatenate([c[k * runs_per_task + r] for c in chunks], axis=0)
                     for r in range(runs_per_task)]
             params = {key: v for key, v in tasks[t].items() if key not in ("model", "name")}
+            if pt_interval:
+                params["parallel_tempering_interval"] = int(pt_interval)
             entries[t] = {"task": tasks[t]["name"], "parameters": params,
                           "results": evaluate_task(binned_names(flat), runs, binsize,
                                                    flat.normalization_site_count)}

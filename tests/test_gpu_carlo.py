@@ -184,3 +184,29 @@ def test_run_tasks_results_json_vs_ed(sse_pkg, carlo, tmp_path):
             z = abs(mean - exact[name][i]) / max(err, 1e-8)
             _say(f"T={Ts[i]:.2f} {name}: {mean:.6f}+-{err:.2g} ED {exact[name][i]:.6f} z={z:.2f}")
             assert z <= 4
+
+
+def test_run_tasks_parallel_tempering_vs_ed(sse_pkg, carlo, tmp_path):
+    """Four temperatures of the reference's ED job as one parallel-tempering
+    group (8 ladders, an exchange round after every sweep): each task's
+    results, which follow its temperature across walkers, agree with exact
+    diagonalisation."""
+    _say("carlo: run_tasks with parallel tempering vs ED")
+    Ts = [float(t) for t in TS[1:5]]
+    model = job_magnet_square(sse_pkg, Ts[0])  # one model object: T is the walkers'
+    tasks = [{"name": f"task{i + 1:04d}", "model": model, "T": T, "thermalization": 1000,
+              "sweeps": 4000, "binsize": 100} for i, T in enumerate(Ts)]
+    entries = carlo.run_tasks(tasks, runs_per_task=8, seed0=777, pt_interval=1, pt_seed=5,
+                              progress=_say)
+    carlo.write_results(str(tmp_path / "results.json"), entries)
+    exact = ed.magnet_ed(job_magnet_square(sse_pkg, TS[0]), Ts, sse_pkg.spin_operators)
+    for i, e in enumerate(entries):
+        assert e["parameters"]["parallel_tempering_interval"] == 1
+        for name in ("Energy", "SpecificHeat"):
+            mean, err = e["results"][name]["mean"][0], e["results"][name]["error"][0]
+            z = abs(mean - exact[name][i]) / max(err, 1e-8)
+            _say(f"PT T={Ts[i]:.2f} {name}: {mean:.6f}+-{err:.2g} ED {exact[name][i]:.6f} z={z:.2f}")
+            assert z <= 4
+    with pytest.raises(ValueError):
+        carlo.run_tasks([dict(tasks[0]), dict(tasks[1], model=job_magnet_square(sse_pkg, Ts[1]))],
+                        pt_interval=1)
