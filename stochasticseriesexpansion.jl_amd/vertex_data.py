@@ -15,7 +15,6 @@ from __future__ import annotations
 import warnings
 
 import numpy as np
-from scipy.optimize import linprog
 
 from .util import join_idx, site_of_leg, split_idx, worm_action, worm_count, worm_inverse
 
@@ -107,6 +106,7 @@ def vertex_apply_change(leg_states, dims, vertex, step_in, step_out, index=None)
 
 
 def _solve_lp(cost, A, b, lp_tolerance):
+    from scipy.optimize import linprog  # only the lp_solver="highs" path needs scipy
     res = linprog(cost, A_eq=A, b_eq=b, bounds=[(0, None)] * len(cost), method="highs-ds",
                   options={"primal_feasibility_tolerance": lp_tolerance,
                            "dual_feasibility_tolerance": lp_tolerance})
