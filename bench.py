@@ -16,6 +16,8 @@ walkers up to one per queue (8 for 4096 walkers); more streams than queues
 would share queues and serialize.
 
 Reported beside it, in the same run:
+  * the serving stream (open-loop Poisson arrivals of mixed jobs, 20 s of
+    arrivals by default): p50/p99 time-to-result of the jobs done;
   * the plain GPU form: the same walkers (thermalized states) run one walker
     per launch with the lane kernel, and with the wave kernels;
   * the CPU baseline: the C restatement oracle (one walker per thread) on
@@ -266,7 +268,8 @@ def extra_serving(sse, seconds=30.0, sweep_scale=1e-3, slots=1024):
     done, stats = loop.run(jobs, time_limit=seconds * 3,
                            log=lambda m: print(f"serving: {m}", file=sys.stderr, flush=True))
     loop.close()
-    stats["workload"] = (f"serving: Poisson 200 jobs/s seed 7, L 8..48, beta 1..32, sweeps "
+    stats["workload"] = (f"serving: Poisson 200 jobs/s seed 7 for {seconds:g} s (results "
+                         f"counted for {3 * seconds:g} s), L 8..48, beta 1..32, sweeps "
                          f"log-uniform [1e3,1e5] x {sweep_scale:g}, {slots} slots in 5 size "
                          f"classes (n <= 2000 / 8000 / 30000 / 60000 / more), models prebuilt")
     stats["jobs_submitted"] = len(jobs)
@@ -299,6 +302,9 @@ def main():
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="end with three device-to-device copies of this many MiB "
                          "(reference kernel for HBM traffic counters)")
+    ap.add_argument("--serving-seconds", type=float, default=20.0,
+                    help="arrival window of the serving stream reported in the default line "
+                         "(p50/p99 time-to-result; 0 skips it, --extra runs 30 s)")
     ap.add_argument("--extra", action="store_true",
                     help="also run the heterogeneous, ladder and serving workloads")
     args = ap.parse_args()
@@ -434,6 +440,15 @@ def main():
         "cpu_baseline": cpu,
     }
     out.update(baselines)
+    if args.serving_seconds > 0 and not args.extra:
+        log(f"serving stream: {args.serving_seconds:g} s of arrivals")
+        try:
+            st = extra_serving(sse, seconds=args.serving_seconds)
+            out["serving"] = {k: st[k] for k in ("p50_time_to_result_s", "p99_time_to_result_s",
+                                                 "jobs_done", "jobs_submitted", "wall_s",
+                                                 "vertex_updates_per_s", "workload")}
+        except Exception as e:  # the headline line is reported either way
+            out["serving"] = {"error": f"{type(e).__name__}: {e}"}
     if args.extra:
         extras = {}
         for name, fn in (("heterogeneous", extra_heterogeneous), ("ladder", extra_ladder),
