@@ -173,6 +173,15 @@ int32_t sse_measure_width(const sse_batch *b, int32_t walker);
 int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure);
 int sse_read_accumulators(sse_batch *b, double *sums, double *sumsq,
                           int64_t *counts);
+/* One persistent, work-queue-driven launch in which walker w advances by its
+ * own sweeps[w] sweeps (each followed by the measurement once thermalized
+ * when measure != 0).  A walker's chain and results are those of the same
+ * number of Carlo.sweep! calls (src/sse.jl:62-87) however the budgets
+ * differ, so a driver can give every Carlo task of a batch its own sweep
+ * count, or balance uneven walkers by expected cost.  Wave mode; not with
+ * temperature ladders that exchange between sweeps.  No reference
+ * counterpart: the reference sweeps one MC per call.                        */
+int sse_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure);
 int sse_reset_accumulators(sse_batch *b);
 /* WormLengthFraction = (sum of worm lengths) / n, which the reference
  * measures inside the worm update of every thermalized sweep with n != 0
@@ -285,10 +294,13 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
  *                large walkers take the later layouts; 1 skips the 8-byte
  *                layout; 2 forces global memory; 3 forces the hybrid.  Results
  *                are identical; this is a tuning/testing knob.
- * SSE_OPT_PERSISTENT (default 0): persistent multi-sweep launches (workers
- *                take (walker, sweep) tasks in order).  Disabled in this build:
- *                setting it to 1 returns SSE_ERR_INVALID (its first GPU run
- *                faulted and the cause is not yet found).
+ * SSE_OPT_PERSISTENT (default 0): 1 = sse_run/sse_sweep issue one persistent,
+ *                work-queue-driven launch per call (wave mode): each
+ *                workgroup takes a walker from an atomic task counter and runs
+ *                all of that walker's sweeps of the call (diagonal update,
+ *                vertex list, worm update, measurement) before it takes the
+ *                next; walkers never wait for each other.  Results are
+ *                identical.
  * SSE_OPT_STREAMS (default 0): wave mode splits the walkers into this many
  *                groups, each swept on its own HIP stream (forked from and
  *                joined into the batch stream), so the slow tail of one
@@ -305,6 +317,14 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
  *                stream groups from each group's own largest count.  Walkers
  *                that do not fit take a slower record layout; results are
  *                identical.
+ * SSE_OPT_CHASE (default 0): worm traversal over the LDS record layouts.
+ *                0 = the scalar chase (one record read per step); 1 = record
+ *                lookahead: at each step four lanes read the records linked
+ *                to the current operator's four legs while the exit is
+ *                selected, so the next record is already in registers (more
+ *                instructions per step; slower where the step is bound by
+ *                instruction issue, as on the batched Heisenberg workload).
+ *                Results are identical.
  * SSE_OPT_ORDER (default 1): launch schedule.  1 = walkers in order of their
  *                expected operator count (bonds / T), largest first, so the
  *                longest chains start first and each stream group holds walkers
@@ -317,7 +337,8 @@ enum sse_option {
     SSE_OPT_PERSISTENT = 4,
     SSE_OPT_STREAMS = 5,
     SSE_OPT_K2_LDS = 6,
-    SSE_OPT_ORDER = 7
+    SSE_OPT_ORDER = 7,
+    SSE_OPT_CHASE = 8
 };
 int sse_set_option(sse_batch *b, int32_t option, int64_t value);
 

@@ -25,6 +25,7 @@ SSE_OPT_PERSISTENT = 4
 SSE_OPT_STREAMS = 5
 SSE_OPT_K2_LDS = 6
 SSE_OPT_ORDER = 7
+SSE_OPT_CHASE = 8
 
 STATUS = {0: "SSE_OK", 1: "SSE_ERR_INVALID", 2: "SSE_ERR_HIP", 3: "SSE_ERR_OOM",
           4: "SSE_ERR_SCATTER", 5: "SSE_ERR_NO_DEVICE", 6: "SSE_ERR_STATE"}
@@ -178,6 +179,7 @@ def load_library(path=None):
     lib.sse_measure_width.argtypes = [vp, C.c_int32]
     lib.sse_measure_width.restype = C.c_int32
     lib.sse_run.argtypes = [vp, C.c_int32, C.c_int32]
+    lib.sse_run_budgets.argtypes = [vp, C.POINTER(C.c_int64), C.c_int32]
     lib.sse_read_accumulators.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                           C.POINTER(C.c_int64)]
     lib.sse_reset_accumulators.argtypes = [vp]
@@ -234,7 +236,7 @@ EXPORTED_SYMBOLS = (
     "sse_slot_start", "sse_slot_stop", "sse_slot_poll", "sse_pool_usage", "sse_set_option",
     "sse_call_time", "sse_lds_bytes", "sse_pt_log_weight_ratio", "sse_pt_change_parameter",
     "sse_pt_get_parameter", "sse_pt_set_ladders", "sse_pt_exchange", "sse_pt_get_ladders",
-    "sse_vertex_transitions", "sse_read_worm_fraction")
+    "sse_vertex_transitions", "sse_read_worm_fraction", "sse_run_budgets")
 
 
 def vertex_transitions(dims, leg_states, weights, tolerance, lp_tolerance):
@@ -331,6 +333,14 @@ class Batch:
 
     def run(self, n_sweeps, measure=True):
         check(self.lib, self.lib.sse_run(self.handle, n_sweeps, int(bool(measure))))
+
+    def run_budgets(self, sweeps, measure=True):
+        """Walker w advances by sweeps[w] sweeps in one persistent launch."""
+        sw = np.ascontiguousarray(sweeps, dtype=np.int64)
+        if sw.shape != (len(self.walkers),):
+            raise ValueError("one sweep count per walker")
+        check(self.lib, self.lib.sse_run_budgets(self.handle, _ptr(sw, C.c_int64),
+                                                 int(bool(measure))))
 
     def accumulators(self):
         widths = self._slot_widths

@@ -20,6 +20,7 @@
 #include "sse_lane.h"
 #include "sse_wave.h"
 #include "sse_pt.h"
+#include "sse_persist.h"
 
 namespace {
 
@@ -37,6 +38,17 @@ int fail(int code, const std::string &msg)
         if (_e != hipSuccess)                                                              \
             return fail(SSE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));   \
     } while (0)
+
+// What a captured launch sequence (HIP graph) bakes in: a replay is valid
+// while every field is unchanged.  Compared bytewise (constructed zeroed).
+struct GraphKey {
+    int32_t kind;       // 1 one stream, 2 stream groups, 3 persistent launch
+    int32_t sweeps, measure, groups, rec_mode, chase, use_order, n_tasks;
+    size_t lds[9];      // worm-kernel LDS (one stream / per group), persistent LDS
+    const void *models, *budgets, *order;
+    GraphKey() { memset(this, 0, sizeof *this); }
+    bool same(const GraphKey &o) const { return memcmp(this, &o, sizeof *this) == 0; }
+};
 
 // What sse_set_walker checks a restored state against, per model: the vertex
 // count of each bond's type and each site's local dimension.
@@ -74,6 +86,10 @@ struct sse_batch {
     uint8_t *d_scratch = nullptr;
     bool any_pending = true;
     bool any_budget = false;
+    // warm-up rounds the pending walkers need: the largest
+    // diagonal_warmup_sweeps among the walkers created or restarted
+    // (sse_slot_start) since the last initialisation
+    int pending_warmup = 0;
     int max_sites = 0, total_width = 0;
     int64_t max_records = 0;
     bool inited = false;
@@ -82,11 +98,15 @@ struct sse_batch {
     // captured sweep(+measure) sequence, replayed by sse_run/sse_sweep
     bool use_graph = true, use_timing = true;
     hipGraphExec_t gexec = nullptr;
-    int g_sweeps = 0, g_measure = -1;
-    size_t g_lds = 0;
-    int g_rec_mode = 0;
-    bool g_order = true;
-    const DevModel *g_models = nullptr;
+    GraphKey gkey;
+    // persistent work-queue launch (SSE_OPT_PERSISTENT, sse_run_budgets):
+    // task counter, per-walker budgets and task order of a budgeted call
+    bool persistent = false;
+    int n_cu = 0;
+    int32_t *d_queue = nullptr, *d_budget = nullptr, *d_task_order = nullptr;
+    // expected operator count per sweep of each walker's current job (bonds /
+    // T of its model; a scheduling estimate kept by create and slot_start)
+    std::vector<double> work_key;
     double phase_ms[3] = {0, 0, 0};
     int64_t phase_launches = 0;
     // wave mode: walkers split into groups, each swept on its own stream, so
@@ -216,14 +236,14 @@ static int validate_model(const sse_model_desc &m, int k)
 }
 
 // Upload one model: every array into one device buffer, pointers fixed up.
-static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm)
+// The model's HostModelCheck goes into hc; the caller commits both hc and dm
+// only once every step of the upload has succeeded.
+static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, HostModelCheck &hc)
 {
-    HostModelCheck hc;
     hc.site_dims.assign(m.site_dims, m.site_dims + m.n_sites);
     hc.bond_nv.resize(m.n_bonds);
     for (int k = 0; k < m.n_bonds; ++k)
         hc.bond_nv[k] = (int16_t)m.types[m.bond_type[k]].n_vertices;
-    b->h_check.push_back(std::move(hc));
     std::vector<char> blob;
     auto put = [&](const void *src, size_t bytes) {
         size_t off = (blob.size() + 15) & ~(size_t)15;
@@ -651,10 +671,15 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
     b->n_models = n_models;
     b->n_walkers = n_walkers;
     b->wdesc.assign(walkers, walkers + n_walkers);
+    for (int w = 0; w < n_walkers; ++w)
+        b->pending_warmup = std::max(b->pending_warmup, (int)walkers[w].diagonal_warmup_sweeps);
     b->h_models.resize(n_models);
     int rc = SSE_OK;
     for (int k = 0; k < n_models && !rc; ++k) {
-        rc = upload_model(b, models[k], b->h_models[k]);
+        HostModelCheck hc;
+        rc = upload_model(b, models[k], b->h_models[k], hc);
+        if (!rc)
+            b->h_check.push_back(std::move(hc));
         b->max_sites = std::max(b->max_sites, (int)models[k].n_sites);
     }
     if (!rc)
@@ -719,6 +744,12 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
         rc = dev_alloc(b, &b->d_done, (size_t)n_walkers);
     if (!rc)
         rc = dev_alloc(b, &b->d_walkers, (size_t)n_walkers);
+    if (!rc)
+        rc = dev_alloc(b, &b->d_queue, 1);
+    if (!rc)
+        rc = dev_alloc(b, &b->d_budget, (size_t)n_walkers);
+    if (!rc)
+        rc = dev_alloc(b, &b->d_task_order, (size_t)n_walkers);
     if (!rc) {
         b->h_walkers.resize(n_walkers);
         for (int w = 0; w < n_walkers && !rc; ++w) {
@@ -768,7 +799,8 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
         // schedule: expected operator count n ~ beta * bonds, largest first
         // (stable, so equal walkers keep their index order)
         std::vector<int32_t> order(n_walkers);
-        std::vector<double> key(n_walkers);
+        std::vector<double> &key = b->work_key;
+        key.resize(n_walkers);
         for (int w = 0; w < n_walkers; ++w) {
             order[w] = w;
             key[w] = (double)b->h_models[walkers[w].model].n_bonds / walkers[w].T;
@@ -793,6 +825,14 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
         (void)bmemset(b, b->d_wlf, 0, sizeof(double) * 3 * n_walkers);
         (void)bmemset(b, b->d_done, 0, sizeof(int32_t) * n_walkers);
         rc = wave_configure(b->wave, b->h_models, b->max_sites);
+        if (!rc && b->wave.lds_limit > 65536)
+            (void)hipFuncSetAttribute((const void *)wave_persist_kernel,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)b->wave.lds_limit);
+        if (hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, b->device) !=
+                hipSuccess ||
+            b->n_cu <= 0)
+            b->n_cu = 256;
     }
     if (rc) {
         std::string keep = g_err;
@@ -897,29 +937,29 @@ static void clear_phase_times(sse_batch *b)
 
 static int launch_measure(sse_batch *b, int mode);
 
-// Capture `sweeps` x (sweep [+ fused measure]) into a HIP graph (no memset
-// nodes), recaptured whenever the kernel parameters it bakes in change.
-static int graph_run(sse_batch *b, int sweeps, int measure)
+// Replay the captured launch sequence of key k, capturing issue() into a
+// HIP graph first when the cached one was made for another key.  No memset
+// nodes: accumulators and counters are cleared by kernels.  Launches issued
+// on other streams during the capture join it through events (the stream
+// groups' fork and join), so a multi-stream pipeline becomes one graph.
+}  // extern "C"
+template <class F>
+static int graph_replay(sse_batch *b, const GraphKey &k, F &&issue)
 {
-    const size_t lds = b->wave.lds_k2;
-    if (!b->gexec || b->g_sweeps != sweeps || b->g_measure != measure || b->g_lds != lds ||
-        b->g_rec_mode != b->wave.rec_mode || b->g_order != b->use_order ||
-        b->g_models != b->d_models) {
+    if (!b->gexec || !b->gkey.same(k)) {
         if (b->gexec) {
             (void)hipGraphExecDestroy(b->gexec);
             b->gexec = nullptr;
         }
         hipGraph_t g = nullptr;
         HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        int rc = SSE_OK;
-        for (int k = 0; k < sweeps && !rc; ++k) {
-            rc = launch_sweep(b, 7);
-            if (!rc && measure)
-                rc = launch_measure(b, 1);
-        }
+        int rc = issue();
         hipError_t e = hipStreamEndCapture(b->stream, &g);
-        if (rc)
+        if (rc) {
+            if (g)
+                (void)hipGraphDestroy(g);
             return rc;
+        }
         if (e != hipSuccess)
             return fail(SSE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
         e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
@@ -928,15 +968,43 @@ static int graph_run(sse_batch *b, int sweeps, int measure)
             b->gexec = nullptr;
             return fail(SSE_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
         }
-        b->g_sweeps = sweeps;
-        b->g_measure = measure;
-        b->g_lds = lds;
-        b->g_rec_mode = b->wave.rec_mode;
-        b->g_order = b->use_order;
-        b->g_models = b->d_models;
+        b->gkey = k;
     }
     HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
     return SSE_OK;
+}
+
+static GraphKey graph_key(const sse_batch *b, int kind, int sweeps, int measure)
+{
+    GraphKey k;
+    k.kind = kind;
+    k.sweeps = sweeps;
+    k.measure = measure;
+    k.rec_mode = b->wave.rec_mode;
+    k.chase = b->wave.chase;
+    k.use_order = b->use_order;
+    k.models = b->d_models;
+    k.lds[0] = b->wave.lds_k2;
+    for (int g = 0; g < 7; ++g)
+        k.lds[1 + g] = b->wave.lds_k2_group[g];
+    k.lds[8] = b->wave.lds_k1;
+    return k;
+}
+
+extern "C" {
+
+// Capture `sweeps` x (sweep [+ fused measure]) on the batch stream.
+static int graph_run(sse_batch *b, int sweeps, int measure)
+{
+    return graph_replay(b, graph_key(b, 1, sweeps, measure), [&]() {
+        int rc = SSE_OK;
+        for (int k = 0; k < sweeps && !rc; ++k) {
+            rc = launch_sweep(b, 7);
+            if (!rc && measure)
+                rc = launch_measure(b, 1);
+        }
+        return rc;
+    });
 }
 
 // Device time of a sweep/run call: stage 0 records the start event, 1 the
@@ -991,13 +1059,9 @@ static int group_count(const sse_batch *b)
     return std::max(1, std::min(g, n));
 }
 
-static int groups_run(sse_batch *b, int n_sweeps, int measure)
+// Streams and events of G stream groups (created before any capture).
+static int groups_prepare(sse_batch *b, int G)
 {
-    const int n = b->n_walkers;
-    const int G = group_count(b);
-    // group 0 runs on the batch stream itself: G groups use G streams (the
-    // device maps streams onto a few hardware queues; streams sharing one
-    // queue would serialize behind each other)
     while ((int)b->gstreams.size() < G - 1) {
         hipStream_t gs;
         HIPCHECK(hipStreamCreateWithFlags(&gs, hipStreamNonBlocking));
@@ -1008,6 +1072,19 @@ static int groups_run(sse_batch *b, int n_sweeps, int measure)
     }
     if (!b->gfork)
         HIPCHECK(hipEventCreateWithFlags(&b->gfork, hipEventDisableTiming));
+    return SSE_OK;
+}
+
+static int groups_run(sse_batch *b, int n_sweeps, int measure)
+{
+    const int n = b->n_walkers;
+    const int G = group_count(b);
+    // group 0 runs on the batch stream itself: G groups use G streams (the
+    // device maps streams onto a few hardware queues; streams sharing one
+    // queue would serialize behind each other)
+    int rc0 = groups_prepare(b, G);
+    if (rc0)
+        return rc0;
     HIPCHECK(hipEventRecord(b->gfork, b->stream));
     for (int g = 1; g < G; ++g)
         HIPCHECK(hipStreamWaitEvent(b->gstreams[g - 1], b->gfork, 0));
@@ -1042,6 +1119,50 @@ static int groups_run(sse_batch *b, int n_sweeps, int measure)
     return SSE_OK;
 }
 
+// Stream groups through graph replays: the fork/join pipeline of a chunk of
+// kGroupChunk sweeps is captured once (the group streams join the capture of
+// the batch stream through the fork event) and replayed; a remainder shorter
+// than a chunk is launched directly.
+constexpr int kGroupChunk = 10;
+static int groups_graph_many(sse_batch *b, int n_sweeps, int measure)
+{
+    const int G = group_count(b);
+    int rc = groups_prepare(b, G);
+    if (rc)
+        return rc;
+    int done = 0;
+    for (; done + kGroupChunk <= n_sweeps; done += kGroupChunk) {
+        GraphKey k = graph_key(b, 2, kGroupChunk, measure);
+        k.groups = G;
+        rc = graph_replay(b, k, [&]() { return groups_run(b, kGroupChunk, measure); });
+        if (rc)
+            return rc;
+    }
+    return done < n_sweeps ? groups_run(b, n_sweeps - done, measure) : SSE_OK;
+}
+
+// The persistent work-queue launch: n_tasks tasks in the order `order`,
+// budgets per walker or n_sweeps each (sse_persist.h), captured as a graph
+// when graphs are on.
+static int persist_issue(sse_batch *b, int n_sweeps, int measure, const int32_t *budgets,
+                         int n_tasks, const int32_t *order)
+{
+    auto issue = [&]() {
+        int rc = wave_launch_persist(b->wave, b->stream, b->d_walkers, b->d_models, b->d_pool, b->d_out,
+                                     b->d_sum, b->d_sumsq, b->d_count, b->d_wlf, b->d_done, order,
+                                     n_tasks, budgets, n_sweeps, measure, b->d_queue, b->n_cu);
+        return rc ? fail(rc, "persistent launch failed") : SSE_OK;
+    };
+    if (!b->use_graph)
+        return issue();
+    GraphKey k = graph_key(b, 3, n_sweeps, measure);
+    k.n_tasks = n_tasks;
+    k.budgets = budgets;
+    k.order = order;
+    k.lds[0] = persist_lds_bytes(b->wave);
+    return graph_replay(b, k, issue);
+}
+
 // n_sweeps through replays of a captured chunk of up to 32 sweeps
 static int graph_run_many(sse_batch *b, int n_sweeps, int measure)
 {
@@ -1073,15 +1194,15 @@ static int launch_init_pending(sse_batch *b)
                            b->d_walkers, b->d_models, b->d_pool, 0, n, 1);
         HIPCHECK(hipGetLastError());
     } else {
-        int max_warmup = 0;
-        for (const auto &wd : b->wdesc)
-            max_warmup = std::max(max_warmup, (int)wd.diagonal_warmup_sweeps);
+        // K1's warm-up round r skips walkers with fewer warm-up sweeps, so the
+        // largest count among the pending walkers sets the number of rounds
         int rc = wave_launch_init(b->wave, b->stream, b->d_walkers, b->d_models, b->d_pool, n,
-                                  max_warmup);
+                                  b->pending_warmup);
         if (rc)
             return rc;
     }
     b->any_pending = false;
+    b->pending_warmup = 0;
     return SSE_OK;
 }
 
@@ -1104,8 +1225,11 @@ static int launch_pt_exchange(sse_batch *b);
 // events (event slots from k0), graph replays or plain launches.
 static int issue_sweeps(sse_batch *b, int n_sweeps, int measure, bool grouped, bool timed, int k0)
 {
+    if (b->mode == SSE_MODE_WAVE && b->persistent)
+        return persist_issue(b, n_sweeps, measure, nullptr, b->n_walkers,
+                             b->use_order ? b->d_order : nullptr);
     if (grouped)
-        return groups_run(b, n_sweeps, measure);
+        return b->use_graph ? groups_graph_many(b, n_sweeps, measure) : groups_run(b, n_sweeps, measure);
     if (timed) {
         for (int k = 0; k < n_sweeps; ++k) {
             hipEvent_t *e = &b->evpool[4 * (size_t)(k0 + k)];
@@ -1142,8 +1266,9 @@ static int sweep_call(sse_batch *b, int32_t n_sweeps, int32_t measure)
         if (rc)
             return rc;
     }
-    const bool grouped = b->mode == SSE_MODE_WAVE && group_count(b) > 1;
-    const bool timed = !grouped && b->use_timing && n_sweeps <= 256;
+    const bool grouped = b->mode == SSE_MODE_WAVE && !b->persistent && group_count(b) > 1;
+    const bool timed = !grouped && !(b->mode == SSE_MODE_WAVE && b->persistent) && b->use_timing &&
+                       n_sweeps <= 256;
     const bool pt = b->pt_ladders > 0 && b->pt_interval > 0;
     int rc0 = call_timer(b, 0, 0);
     if (rc0)
@@ -1245,6 +1370,13 @@ int sse_measure(sse_batch *b, double *out)
         return fail(SSE_ERR_INVALID, "null batch");
     if (!b->inited)
         return fail(SSE_ERR_STATE, "measure before init");
+    // slots restarted since the last sweep are initialised first (Carlo.init!
+    // precedes any measure!), so every measured walker has a valid state
+    if (b->any_pending) {
+        int rc = launch_init_pending(b);
+        if (rc)
+            return rc;
+    }
     int rc = launch_measure(b, 0);
     if (rc)
         return rc;
@@ -1264,6 +1396,66 @@ int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure)
     if (n_sweeps < 0)
         return fail(SSE_ERR_INVALID, "negative sweep count");
     return sweep_call(b, n_sweeps, measure);
+}
+
+int sse_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure)
+{
+    if (!b || !sweeps)
+        return fail(SSE_ERR_INVALID, "sse_run_budgets: bad arguments");
+    if (!b->inited)
+        return fail(SSE_ERR_STATE, "run before init");
+    if (b->mode != SSE_MODE_WAVE)
+        return fail(SSE_ERR_INVALID, "sse_run_budgets: wave mode only");
+    if (b->pt_ladders > 0 && b->pt_interval > 0)
+        return fail(SSE_ERR_INVALID, "sse_run_budgets: walkers of a temperature ladder advance "
+                                     "together (sse_run)");
+    const int n = b->n_walkers;
+    std::vector<int32_t> bud(n);
+    int64_t most = 0;
+    for (int w = 0; w < n; ++w) {
+        if (sweeps[w] < 0 || sweeps[w] > INT32_MAX)
+            return fail(SSE_ERR_INVALID, "sse_run_budgets: sweep counts must be in 0..2^31-1");
+        bud[w] = (int32_t)sweeps[w];
+        most = std::max(most, sweeps[w]);
+    }
+    // tasks: walkers with a budget, largest expected work (budget x bonds / T) first
+    std::vector<int32_t> order;
+    std::vector<double> work(n);
+    for (int w = 0; w < n; ++w)
+        if (bud[w] > 0) {
+            order.push_back(w);
+            work[w] = (double)bud[w] * b->work_key[w];
+        }
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) { return work[a] > work[c]; });
+    if (b->any_pending) {
+        int rc = launch_init_pending(b);
+        if (rc)
+            return rc;
+    }
+    int rc = call_timer(b, 0, 0);
+    if (rc)
+        return rc;
+    if (!order.empty()) {
+        HIPCHECK(bmemcpy(b, b->d_budget, bud.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+        HIPCHECK(bmemcpy(b, b->d_task_order, order.data(), sizeof(int32_t) * order.size(),
+                         hipMemcpyHostToDevice));
+        rc = persist_issue(b, 0, measure ? 1 : 0, b->d_budget, (int)order.size(), b->d_task_order);
+        if (rc)
+            return rc;
+    }
+    if (!measure && b->any_budget) {
+        hipLaunchKernelGGL(retire_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_walkers,
+                           n, b->d_done);
+        HIPCHECK(hipGetLastError());
+    }
+    rc = call_timer(b, 1, most);
+    if (rc)
+        return rc;
+    rc = check_walker_errors(b);
+    if (!rc)
+        rc = call_timer(b, 2, 0);
+    clear_phase_times(b);
+    return rc;
 }
 
 int sse_read_accumulators(sse_batch *b, double *sums, double *sumsq, int64_t *counts)
@@ -1520,22 +1712,28 @@ int sse_add_model(sse_batch *b, const sse_model_desc *model, int32_t *index)
     if (model->n_sites > b->max_sites)
         return fail(SSE_ERR_INVALID, "sse_add_model: more sites than the batch was created for");
     HIPCHECK(hipStreamSynchronize(b->stream));
+    // upload into locals; the batch's model lists (h_models, h_check,
+    // d_models, n_models) change together only after every step succeeded
     DevModel dm;
-    rc = upload_model(b, *model, dm);
+    HostModelCheck hc;
+    rc = upload_model(b, *model, dm, hc);
     if (rc)
         return rc;
-    b->h_models.push_back(dm);
+    std::vector<DevModel> hm = b->h_models;
+    hm.push_back(dm);
+    DevModel *nd = nullptr;
+    rc = dev_alloc(b, &nd, hm.size());
+    if (rc)
+        return rc;
+    if (bmemcpy(b, nd, hm.data(), sizeof(DevModel) * hm.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(SSE_ERR_HIP, "sse_add_model: model table upload failed");
+    b->h_models.swap(hm);
+    b->h_check.push_back(std::move(hc));
+    b->d_models = nd;
     if (((int64_t)dm.n_types << dm.dt_lgp) > 256 && b->wave.small_codes) {
         b->wave.small_codes = false;  // 10-byte worm records from now on: size LDS for them
         wave_resize_lds(b->wave, b->seen_max_n, b->seen_max_L);
     }
-    DevModel *nd = nullptr;
-    rc = dev_alloc(b, &nd, b->h_models.size());
-    if (rc)
-        return rc;
-    HIPCHECK(bmemcpy(b, nd, b->h_models.data(), sizeof(DevModel) * b->h_models.size(),
-                       hipMemcpyHostToDevice));
-    b->d_models = nd;
     *index = b->n_models++;
     return SSE_OK;
 }
@@ -1583,6 +1781,7 @@ int sse_slot_start(sse_batch *b, int32_t slot, const sse_walker_desc *wd, int64_
         if (rc)
             return rc;
     }
+    b->work_key[slot] = (double)m.n_bonds / wd->T;
     x.model = wd->model;
     x.warmup = wd->diagonal_warmup_sweeps;
     x.T = wd->T;
@@ -1618,6 +1817,7 @@ int sse_slot_start(sse_batch *b, int32_t slot, const sse_walker_desc *wd, int64_
     HIPCHECK(bmemset(b, b->d_wlf + 3 * (size_t)slot, 0, 3 * sizeof(double)));
     HIPCHECK(bmemset(b, b->d_done + slot, 0, sizeof(int32_t)));
     b->any_pending = true;
+    b->pending_warmup = std::max(b->pending_warmup, (int)wd->diagonal_warmup_sweeps);
     if (n_sweeps > 0)
         b->any_budget = true;
     b->inited = true;
@@ -1687,9 +1887,12 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         b->use_timing = value != 0;
         return SSE_OK;
     case SSE_OPT_PERSISTENT:
-        if (value != 0)  // switched off until the fault of its first GPU run is explained
-            return fail(SSE_ERR_INVALID, "SSE_OPT_PERSISTENT is disabled in this build: its first "
-                                         "GPU run ended in a memory fault whose cause is not found");
+        if (value != 0 && value != 1)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_PERSISTENT takes 0 or 1");
+        if (value && b->mode != SSE_MODE_WAVE)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_PERSISTENT needs wave mode");
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        b->persistent = value != 0;
         return SSE_OK;
     case SSE_OPT_K2_LDS: {
         if (value != 0 && (value < 8 * (int64_t)K2Stream::kNW + 256 || value > (int64_t)b->wave.lds_limit))
@@ -1716,6 +1919,12 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         b->n_groups = (int)value;
         for (size_t &g : b->wave.lds_k2_group)  // the groups hold other walkers now
             g = 0;
+        return SSE_OK;
+    case SSE_OPT_CHASE:
+        if (value != 0 && value != 1)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_CHASE takes 0 (plain) or 1 (record lookahead)");
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        b->wave.chase = (int)value;
         return SSE_OK;
     case SSE_OPT_WORM_RECORDS:
         if (value < 0 || value > 3)

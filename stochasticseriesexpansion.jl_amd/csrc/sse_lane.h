@@ -211,7 +211,9 @@ __global__ void lane_sweep_kernel(DevWalker *W, const DevModel *models, DevPool 
     if (wi >= w_end)
         return;
     DevWalker w = W[wi];
-    if (w.error || !w.active)
+    // a slot whose sweep budget ran out (retire set) waits for its
+    // retirement; it takes no further sweep
+    if (w.error || !w.active || w.retire)
         return;
     const DevModel &m = models[w.model];
     bool therm = w.sweeps > w.thermalization;
@@ -360,7 +362,8 @@ __global__ void lane_measure_kernel(DevWalker *W, const DevModel *models, double
     if (wi >= n_walkers)
         return;
     const DevWalker &w = W[wi];
-    if (w.error || (mode == 1 && !w.active))
+    // a pending slot's opstring and spins are not initialised yet
+    if (w.error || w.pending || (mode == 1 && !w.active))
         return;
     const DevModel &m = models[w.model];
     double *o = out + w.meas_offset;

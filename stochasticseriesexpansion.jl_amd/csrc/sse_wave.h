@@ -37,6 +37,7 @@ struct WaveConfig {
     size_t lds_limit = 65536;  // device limit on dynamic LDS per workgroup
     bool small_codes = true;   // every model has <= 256 chase codes (Rec8 records)
     int rec_mode = 0;          // worm-kernel record layout: 0 auto, 1 no Rec8, 2 global
+    int chase = 0;             // LDS-record chase: 0 plain, 1 with record lookahead
     size_t lds_k2_fixed = 0;   // worm-kernel LDS per walker set by the caller (0 = sized here)
     size_t lds_k2_group[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per stream group (0 = lds_k2)
 };
@@ -402,7 +403,9 @@ __device__ __forceinline__ void k1_walker_t(DevWalker *W, const DevModel *models
     DevWalker w = W[wi];
     if (w.error)
         return;
-    if (warm >= 0 ? (warm >= w.warmup || !w.pending) : !w.active)
+    // warm-up rounds touch pending walkers only; a sweep skips inactive
+    // walkers and slots whose sweep budget ran out (retire set)
+    if (warm >= 0 ? (warm >= w.warmup || !w.pending) : (!w.active || w.retire))
         return;
     const DevModel &m = models[w.model];
     tab.stage(m);
@@ -841,6 +844,25 @@ struct Rec8 {
         ((lds_u8 *)rec)[8u * r + 7u] = (uint8_t)e;
     }
     __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return (uint32_t)(rec[r] >> 56); }
+    // lookahead chase: each lane reads one record (lane-varying rank); the
+    // wave then takes the word of one lane with v_readlane
+    struct Cand {
+        uint32_t lo, hi;
+    };
+    __device__ __forceinline__ Cand load_lane(uint32_t r) const
+    {
+        const uint64_t x = rec[r];
+        return Cand{(uint32_t)x, (uint32_t)(x >> 32)};
+    }
+    __device__ __forceinline__ static Word pick(const Cand &c, uint32_t lane)
+    {
+        return Word{(uint32_t)__builtin_amdgcn_readlane((int)c.lo, (int)lane),
+                    (uint32_t)__builtin_amdgcn_readlane((int)c.hi, (int)lane)};
+    }
+    __device__ __forceinline__ static void set_ecode(Word &x, uint32_t e)
+    {
+        x.hi = (x.hi & 0x00FFFFFFu) | (e << 24);
+    }
 };
 
 struct Rec10 {
@@ -866,6 +888,21 @@ struct Rec10 {
     __device__ __forceinline__ static void set_link0(Word &x, uint32_t v) { x.lo = (x.lo & ~0xFFFFu) | v; }
     __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const { ec[r] = (uint16_t)e; }
     __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
+    struct Cand {
+        uint32_t lo, hi, e;
+    };
+    __device__ __forceinline__ Cand load_lane(uint32_t r) const
+    {
+        const uint64_t x = lk[r];
+        return Cand{(uint32_t)x, (uint32_t)(x >> 32), (uint32_t)ec[r]};
+    }
+    __device__ __forceinline__ static Word pick(const Cand &c, uint32_t lane)
+    {
+        return Word{(uint32_t)__builtin_amdgcn_readlane((int)c.lo, (int)lane),
+                    (uint32_t)__builtin_amdgcn_readlane((int)c.hi, (int)lane),
+                    (uint32_t)__builtin_amdgcn_readlane((int)c.e, (int)lane)};
+    }
+    __device__ __forceinline__ static void set_ecode(Word &x, uint32_t e) { x.e = e; }
 };
 
 struct RecG {
@@ -914,6 +951,58 @@ struct RecH {
     {
         return leg == 0 ? x.l.x : leg == 1 ? x.l.y : leg == 2 ? x.l.z : x.l.w;
     }
+    __device__ __forceinline__ static void set_link0(Word &x, uint32_t v) { x.l.x = v; }
+    __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const { ec[r] = (uint16_t)e; }
+    __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
+};
+
+// The four links of record r read by one vector load (lane j reads link
+// j & 3; lane-varying addresses are never turned into scalar loads) and made
+// wave-uniform with v_readlane.  Used inside the persistent launch, where the
+// diagonal/vertex-list phase rewrites the records between worm phases: the
+// scalar cache is not kept coherent with those vector stores within a launch.
+__device__ __forceinline__ U4 links_vec(const uint32_t *lk4, uint32_t r)
+{
+    const uint32_t v = as_global(lk4)[4u * r + (threadIdx.x & 3u)];
+    return U4{(uint32_t)__builtin_amdgcn_readlane((int)v, 0),
+              (uint32_t)__builtin_amdgcn_readlane((int)v, 1),
+              (uint32_t)__builtin_amdgcn_readlane((int)v, 2),
+              (uint32_t)__builtin_amdgcn_readlane((int)v, 3)};
+}
+
+// RecG with vector link loads (persistent launch)
+struct RecGV {
+    uint32_t *ec;
+    const uint32_t *lk;
+    static constexpr bool kChecked = true;
+    typedef uint64_t Key;
+    typedef RecG::Word Word;
+    __device__ __forceinline__ Word load(uint32_t r) const
+    {
+        const U4 l = links_vec(lk, r);
+        // the code: lane j reads code r + (j & 1) (r + 1 stays inside the
+        // slab: the link region follows the code array), so the load is a
+        // vector load as well
+        const uint32_t e = as_global(ec)[r + (threadIdx.x & 1u)];
+        return Word{l, (uint32_t)__builtin_amdgcn_readlane((int)e, 0)};
+    }
+    __device__ __forceinline__ static uint32_t ecode(const Word &x) { return x.e; }
+    __device__ __forceinline__ static uint32_t link(const Word &x, uint32_t leg) { return RecG::link(x, leg); }
+    __device__ __forceinline__ static void set_link0(Word &x, uint32_t v) { x.l.x = v; }
+    __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const { ec[r] = e; }
+    __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
+};
+
+// RecH with vector link loads (persistent launch)
+struct RecHV {
+    lds_u16 *ec;
+    const uint32_t *lk;
+    static constexpr bool kChecked = true;
+    typedef uint64_t Key;
+    typedef RecH::Word Word;
+    __device__ __forceinline__ Word load(uint32_t r) const { return Word{links_vec(lk, r), uni(ec[r])}; }
+    __device__ __forceinline__ static uint32_t ecode(const Word &x) { return x.e; }
+    __device__ __forceinline__ static uint32_t link(const Word &x, uint32_t leg) { return RecH::link(x, leg); }
     __device__ __forceinline__ static void set_link0(Word &x, uint32_t v) { x.l.x = v; }
     __device__ __forceinline__ void put_ecode(uint32_t r, uint32_t e) const { ec[r] = (uint16_t)e; }
     __device__ __forceinline__ uint32_t get_ecode(uint32_t r) const { return ec[r]; }
@@ -968,6 +1057,77 @@ __device__ __forceinline__ uint32_t chase_select(uint32_t mlo, uint32_t mhi, Con
     return eo;
 }
 
+// Worm start (sse.jl:241-251): draw (slot, leg) pairs until the leg belongs
+// to an operator (an operator's four legs are all linked), then the worm
+// function.  Returns the operator's rank r0, leg l0, the site's dimension and
+// the worm function wf0 (1-based); false when wf0 exceeds the chase table.
+__device__ __forceinline__ bool worm_start(const DevWalker &w, const DevModel &m, K2Stream &rs,
+                                           const gmem<uint64_t> *MASK, const gmem<uint32_t> *BASE,
+                                           uint32_t dmw, uint32_t &r0, uint32_t &l0, uint32_t &dim0,
+                                           uint32_t &wf0)
+{
+    const int64_t L = w.L;
+    int64_t p0;
+    for (;;) {
+        p0 = (int64_t)uni64(rs.range((uint64_t)L));
+        l0 = uni((uint32_t)rs.range(4));
+        const uint64_t mk = uni64(MASK[p0 >> 6]);
+        const int bit = (int)(p0 & 63);
+        if ((mk >> bit) & 1ull) {
+            r0 = uni(BASE[p0 >> 6] + (uint32_t)popc(mk & below_mask(bit)));
+            break;
+        }
+    }
+    const int2 b0 = m.bond_rec[op_bond(w.ops[p0])];
+    const uint32_t site0 = uni((uint32_t)((l0 & 1) ? b0.y : (b0.x & 0xFFFFF)));
+    dim0 = uni(m.site_dims[site0]);
+    wf0 = 1u + uni((uint32_t)rs.range((uint64_t)(dim0 - 1)));
+    return wf0 - 1u < dmw;
+}
+
+// The exact scatter (vertex_data.jl:113-125) of a step whose chase entry
+// carries the slow marker: the entry's own thresholds for up to four exits,
+// the exit list for more.  A fall-through or an invalid transition sets err
+// and returns efail (the code rewritten unchanged, exit leg 0, the worm field
+// of the closing test), which the caller turns into a loop exit.
+template <bool kOneWorm>
+__device__ __forceinline__ uint32_t chase_exact(const DevModel &m, const_char *CT, uint32_t eoff,
+                                                uint32_t mlo, uint32_t mhi, uint32_t e, uint32_t lgp,
+                                                uint32_t efail, uint32_t &err)
+{
+    ConstEntry *ES = (ConstEntry *)(CT + eoff);
+    const uint32_t cnt = uni(ES->count);
+    if (cnt <= 4u) {
+        const uint64_t mm = ((uint64_t)mhi << 32) | mlo;
+        uint32_t j = 0;
+        for (; j < cnt; ++j)
+            if (mm < uni64(ES->K[j]))
+                break;
+        if (j >= cnt) {
+            err = SSE_ERR_SCATTER;
+            return efail;
+        }
+        return uni(j < 3u ? ES->out[j] : ES->pad[0]);
+    }
+    const uint32_t trw = uni(ES->trans);
+    const WormTab tab = worm_tab((const char *)m.wtab, m);
+    const double xr = (double)(((uint64_t)mhi << 32) | mlo) * 0x1.0p-53;
+    uint32_t ex = trw & 0xFFFFFu;
+    const uint32_t eend = ex + (trw >> 20);
+    for (; ex < eend; ++ex) {
+        const uint64_t cb = uni64((uint64_t)__double_as_longlong(tab.cum[ex]));
+        if (xr < __longlong_as_double((long long)cb))
+            break;
+    }
+    if (ex >= eend) {
+        err = SSE_ERR_SCATTER;
+        return efail;
+    }
+    const uint32_t vc = uni(tab.eout[ex]);
+    const uint32_t ecn = (e >> lgp << lgp) | (vc & 1023u), lg = (vc >> 10) & 3u;
+    return kOneWorm ? chase_out1(ecn, lg) : chase_out(ecn, lg, vc >> 12);
+}
+
 // The worm update of one walker (sse.jl:193-303), executed uniformly by the
 // whole wave.  The chase is scalar work: the record and the next Philox word
 // leave LDS together and become wave-uniform at once, the transition entry
@@ -979,7 +1139,6 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
                           const gmem<uint64_t> *MASK, const gmem<uint32_t> *BASE, double &total,
                           int64_t &scat)
 {
-    const int64_t L = w.L;
     const uint32_t n = (uint32_t)w.n;
     const int64_t nworms = (int64_t)ceil(w.num_worms);
     const uint32_t dmw = uni((uint32_t)m.dt_mw);
@@ -991,23 +1150,8 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
     for (int64_t k = 0; k < nworms; ++k) {
         if (n == 0)
             continue;  // worm_traverse! returns 0 without draws (sse.jl:234-236)
-        int64_t p0;
-        uint32_t l0, r0;
-        for (;;) {  // sse.jl:241-247: an operator's four legs are all linked
-            p0 = (int64_t)uni64(rs.range((uint64_t)L));
-            l0 = uni((uint32_t)rs.range(4));
-            const uint64_t mk = uni64(MASK[p0 >> 6]);
-            const int bit = (int)(p0 & 63);
-            if ((mk >> bit) & 1ull) {
-                r0 = uni(BASE[p0 >> 6] + (uint32_t)popc(mk & below_mask(bit)));
-                break;
-            }
-        }
-        const int2 b0 = m.bond_rec[op_bond(w.ops[p0])];
-        const uint32_t site0 = uni((uint32_t)((l0 & 1) ? b0.y : (b0.x & 0xFFFFF)));
-        const uint32_t dim0 = uni(m.site_dims[site0]);
-        const uint32_t wf0 = 1u + uni((uint32_t)rs.range((uint64_t)(dim0 - 1)));
-        if (wf0 - 1u >= dmw)
+        uint32_t r0, l0, dim0, wf0;
+        if (!worm_start(w, m, rs, MASK, BASE, dmw, r0, l0, dim0, wf0))
             return SSE_ERR_STATE;
         const uint32_t lid0 = 4u * r0 + l0;
         // closing tests (sse.jl:290-297), both on the key of the next leg: the
@@ -1054,50 +1198,10 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
                 // the entry's address is formed here, not hoisted into the fast path
                 uint32_t eoff_s = eoff;
                 asm volatile("" : "+s"(eoff_s));
-                ConstEntry *ES = (ConstEntry *)(CT + eoff_s);
-                // the exact scatter (vertex_data.jl:113-125): the entry's own
-                // thresholds for up to four exits, the exit list for more; a
-                // fall-through or an invalid transition is an error.  A failure
-                // leaves the loop through the exit test (the code is rewritten
-                // unchanged, exit leg 0 of the record copy made the leg linked
-                // to (r0, l0), the worm field that test's), so the fast path
-                // keeps a single loop exit.
-                const uint32_t cnt = uni(ES->count);
                 const uint32_t efail = (e << 16) | (kOneWorm ? 0u : dim0 - wf0 - 1u);
-                if (cnt <= 4u) {
-                    const uint64_t mm = ((uint64_t)mhi << 32) | mlo;
-                    uint32_t j = 0;
-                    for (; j < cnt; ++j)
-                        if (mm < uni64(ES->K[j]))
-                            break;
-                    if (j >= cnt) {
-                        err = SSE_ERR_SCATTER;
-                        eo = efail;
-                        R::set_link0(x, lnk0);
-                    } else {
-                        eo = uni(j < 3u ? ES->out[j] : ES->pad[0]);
-                    }
-                } else {
-                    const uint32_t trw = uni(ES->trans);
-                    const WormTab tab = worm_tab((const char *)m.wtab, m);
-                    const double xr = (double)(((uint64_t)mhi << 32) | mlo) * 0x1.0p-53;
-                    uint32_t ex = trw & 0xFFFFFu;
-                    const uint32_t eend = ex + (trw >> 20);
-                    for (; ex < eend; ++ex) {
-                        const uint64_t cb = uni64((uint64_t)__double_as_longlong(tab.cum[ex]));
-                        if (xr < __longlong_as_double((long long)cb))
-                            break;
-                    }
-                    if (ex >= eend) {
-                        err = SSE_ERR_SCATTER;
-                        eo = efail;
-                        R::set_link0(x, lnk0);
-                    } else {
-                        const uint32_t vc = uni(tab.eout[ex]);
-                        const uint32_t ecn = (e >> lgp << lgp) | (vc & 1023u), lg = (vc >> 10) & 3u;
-                        eo = kOneWorm ? chase_out1(ecn, lg) : chase_out(ecn, lg, vc >> 12);
-                    }
-                }
+                eo = chase_exact<kOneWorm>(m, CT, eoff_s, mlo, mhi, e, lgp, efail, err);
+                if (err)  // exit leg 0 of the record copy becomes the leg linked to (r0, l0)
+                    R::set_link0(x, lnk0);
             }
             rec.put_ecode(r, eo >> 16);
             uint32_t nx;
@@ -1121,6 +1225,109 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
         const uint32_t steps = (uint32_t)(rs.bbase + rs.off - pos0);
         sc += steps;
         // worm length: scatters + 1 when the worm closed on re-entry
+        total += (double)(steps + endB);
+    }
+    scat = sc;
+    return 0;
+}
+
+// The worm update with record lookahead, for the LDS record layouts (Rec8,
+// Rec10).  Same steps, draws and results as worm_chase; what changes is when
+// the next record is read.  At the top of a step the current record x is
+// wave-uniform, so lane j (j = lane % 4) reads, with one LDS instruction, the
+// record of the operator linked to leg j of x.  While those loads are in
+// flight the scalar unit selects the exit; the next record is then the word of
+// lane exit-leg (v_readlane), and the LDS latency leaves the step's
+// dependency chain.  The loads were issued before this step rewrote the
+// current operator's chase code, so a hop onto the same operator (a site
+// with one operator, linked to itself through the periodic closure) patches
+// the code into the picked record.
+template <class R, bool kOneWorm>
+__device__ int worm_chase_la(DevWalker &w, const DevModel &m, K2Stream &rs, const R &rec,
+                             const gmem<uint64_t> *MASK, const gmem<uint32_t> *BASE, double &total,
+                             int64_t &scat)
+{
+    const uint32_t n = (uint32_t)w.n;
+    const int64_t nworms = (int64_t)ceil(w.num_worms);
+    const uint32_t dmw = uni((uint32_t)m.dt_mw);
+    const uint32_t lgp = uni((uint32_t)m.dt_lgp);
+    const uint32_t estride = dmw << 8;  // bytes per ecode
+    const_char *CT = (const_char *)(const void *)uni64((uint64_t)(const void *)m.dtab);
+    const uint32_t myleg = threadIdx.x & 3u;
+    total = 1.0;
+    uint32_t sc = 0;
+    for (int64_t k = 0; k < nworms; ++k) {
+        if (n == 0)
+            continue;  // worm_traverse! returns 0 without draws (sse.jl:234-236)
+        uint32_t r0, l0, dim0, wf0;
+        if (!worm_start(w, m, rs, MASK, BASE, dmw, r0, l0, dim0, wf0))
+            return SSE_ERR_STATE;
+        const uint32_t lid0 = 4u * r0 + l0;
+        constexpr uint32_t S = kOneWorm ? 6u : 14u;
+        typedef typename R::Key Key;
+        typename R::Word x = rec.load(r0);
+        const uint32_t lnk0 = uni(R::link(x, l0));
+        // closing tests as in worm_chase
+        const Key keyA = ((Key)lnk0 << S) | (kOneWorm ? 0u : dim0 - wf0 - 1u);
+        const Key keyB = ((Key)lid0 << S) | (kOneWorm ? 0u : wf0 - 1u);
+        Key lidS = (Key)lid0 << S;
+        uint32_t wfm = wf0 - 1u;
+        const uint64_t pos0 = rs.bbase + rs.off;
+        uint32_t err = 0;
+        Key kB = 0;
+        do {  // sse.jl:274-300
+            // the records linked to the current operator's four legs
+            const typename R::Cand cand = rec.load_lane(R::link(x, myleg) >> 2);
+            if (__builtin_expect(rs.off >= K2Stream::kNW, 0)) {
+                if (rs.off == K2Stream::kNW)
+                    rs.refill_next();
+                else
+                    rs.ensure(1);
+            }
+            const uint32_t off = uni(rs.off);
+            const uint32_t r = RecIndex<R, S>::of(lidS);
+            const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)rs.mlo, (int)off);
+            const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)rs.mhi, (int)off);
+            rs.off = off + 1u;
+            const uint32_t e = R::ecode(x);
+            const uint32_t legbits = kOneWorm ? (uint32_t)lidS & 0xC0u
+                                              : (uint32_t)(lidS >> 8) & 0xC0u;
+            const uint32_t eoff = kOneWorm ? (e << 8) | legbits : e * estride + (wfm << 8) + legbits;
+            ConstEntry *E = (ConstEntry *)(CT + eoff);
+            uint32_t eo = chase_select(mlo, mhi, E);
+            if (__builtin_expect((eo & kChaseSlowBit) != 0, 0)) {
+                uint32_t eoff_s = eoff;
+                asm volatile("" : "+s"(eoff_s));
+                const uint32_t efail = (e << 16) | (kOneWorm ? 0u : dim0 - wf0 - 1u);
+                eo = chase_exact<kOneWorm>(m, CT, eoff_s, mlo, mhi, e, lgp, efail, err);
+                if (err)
+                    R::set_link0(x, lnk0);
+            }
+            const uint32_t enew = eo >> 16;
+            rec.put_ecode(r, enew);
+            const uint32_t leg = (eo >> 14) & 3u;
+            uint32_t nx;
+            if constexpr (kOneWorm && std::is_same<R, Rec8>::value)
+                nx = Rec8::link_sh(x, eo);
+            else
+                nx = R::link(x, leg);
+            typename R::Word xn = R::pick(cand, leg);
+            // (the record index on the scalar unit: r itself is a vector
+            // register, the LDS address of the code store)
+            if ((nx >> 2) == (uint32_t)(lidS >> (S + 2)))
+                R::set_ecode(xn, enew);
+            const uint32_t wfn = kOneWorm ? 0u : eo & 0x3FFFu;
+            const Key nxS = (Key)nx << S;
+            kB = kOneWorm ? nxS : nxS | wfn;
+            lidS = nxS;
+            wfm = wfn;
+            x = xn;
+        } while (__builtin_expect(min(kB ^ keyA, kB ^ keyB) != 0, 1));
+        const uint32_t endB = kB != keyA;
+        if (err)
+            return (int)err;
+        const uint32_t steps = (uint32_t)(rs.bbase + rs.off - pos0);
+        sc += steps;
         total += (double)(steps + endB);
     }
     scat = sc;
@@ -1305,12 +1512,16 @@ __device__ bool k2_stage_global(const DevWalker &w, const DevModel &m)
 // Worm update of walker wi.  LDS holds the records only (the Philox words are
 // in registers).  RecG reads links through the scalar cache, which is
 // valid because K1 wrote them in an earlier kernel launch.
+// kFused: called inside the persistent launch (global link records read by
+// vector loads, RecHV / RecGV)
+template <bool kFused>
 __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
-                                          int full_sweep, int lds_bytes, int rec_mode, char *lds)
+                                          int full_sweep, int lds_bytes, int rec_mode, char *lds,
+                                          int chase)
 {
     const int lane = threadIdx.x;
     DevWalker w = W[wi];
-    if (w.error || !w.active)
+    if (w.error || !w.active || w.retire)
         return;
     const DevModel &m = models[w.model];
     const bool therm = w.sweeps > w.thermalization;
@@ -1340,12 +1551,21 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
         if (!ok) {
             err = SSE_ERR_STATE;
         } else {
-            RecH rec{lec, (const_u4 *)(const void *)rec_link(w)};
-            err = m.dt_mw == 1 ? worm_chase<RecH, true>(w, m, rs, rec, gmask, gbase, total, scat)
-                               : worm_chase<RecH, false>(w, m, rs, rec, gmask, gbase, total, scat);
-            __syncthreads();
-            if (!err)
-                worm_finish(w, m, rs, rec, gmask, gbase);
+            if constexpr (kFused) {
+                RecHV rec{lec, (const uint32_t *)rec_link(w)};
+                err = m.dt_mw == 1 ? worm_chase<RecHV, true>(w, m, rs, rec, gmask, gbase, total, scat)
+                                   : worm_chase<RecHV, false>(w, m, rs, rec, gmask, gbase, total, scat);
+                __syncthreads();
+                if (!err)
+                    worm_finish(w, m, rs, rec, gmask, gbase);
+            } else {
+                RecH rec{lec, (const_u4 *)(const void *)rec_link(w)};
+                err = m.dt_mw == 1 ? worm_chase<RecH, true>(w, m, rs, rec, gmask, gbase, total, scat)
+                                   : worm_chase<RecH, false>(w, m, rs, rec, gmask, gbase, total, scat);
+                __syncthreads();
+                if (!err)
+                    worm_finish(w, m, rs, rec, gmask, gbase);
+            }
         }
     } else if (fmt) {
         lds_u64 *llk = (lds_u64 *)(uint64_t *)q;
@@ -1356,15 +1576,23 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
             err = SSE_ERR_STATE;
         } else if (fmt == 8) {
             Rec8 rec{llk};
-            err = m.dt_mw == 1 ? worm_chase<Rec8, true>(w, m, rs, rec, gmask, gbase, total, scat)
-                               : worm_chase<Rec8, false>(w, m, rs, rec, gmask, gbase, total, scat);
+            if (chase == 1)
+                err = m.dt_mw == 1 ? worm_chase_la<Rec8, true>(w, m, rs, rec, gmask, gbase, total, scat)
+                                   : worm_chase_la<Rec8, false>(w, m, rs, rec, gmask, gbase, total, scat);
+            else
+                err = m.dt_mw == 1 ? worm_chase<Rec8, true>(w, m, rs, rec, gmask, gbase, total, scat)
+                                   : worm_chase<Rec8, false>(w, m, rs, rec, gmask, gbase, total, scat);
             __syncthreads();
             if (!err)
                 worm_finish(w, m, rs, rec, gmask, gbase);
         } else {
             Rec10 rec{llk, lec};
-            err = m.dt_mw == 1 ? worm_chase<Rec10, true>(w, m, rs, rec, gmask, gbase, total, scat)
-                               : worm_chase<Rec10, false>(w, m, rs, rec, gmask, gbase, total, scat);
+            if (chase == 1)
+                err = m.dt_mw == 1 ? worm_chase_la<Rec10, true>(w, m, rs, rec, gmask, gbase, total, scat)
+                                   : worm_chase_la<Rec10, false>(w, m, rs, rec, gmask, gbase, total, scat);
+            else
+                err = m.dt_mw == 1 ? worm_chase<Rec10, true>(w, m, rs, rec, gmask, gbase, total, scat)
+                                   : worm_chase<Rec10, false>(w, m, rs, rec, gmask, gbase, total, scat);
             __syncthreads();
             if (!err)
                 worm_finish(w, m, rs, rec, gmask, gbase);
@@ -1373,13 +1601,22 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
         if (!k2_stage_global(w, m)) {
             err = SSE_ERR_STATE;
         } else {
-            RecG rec{rec_vt(w), (const_u4 *)(const void *)rec_link(w)};
             // one-worm tables carry 14 * leg in the worm field: the one-worm chase
-            err = m.dt_mw == 1 ? worm_chase<RecG, true>(w, m, rs, rec, gmask, gbase, total, scat)
-                               : worm_chase<RecG, false>(w, m, rs, rec, gmask, gbase, total, scat);
-            __syncthreads();
-            if (!err)
-                worm_finish(w, m, rs, rec, gmask, gbase);
+            if constexpr (kFused) {
+                RecGV rec{rec_vt(w), (const uint32_t *)rec_link(w)};
+                err = m.dt_mw == 1 ? worm_chase<RecGV, true>(w, m, rs, rec, gmask, gbase, total, scat)
+                                   : worm_chase<RecGV, false>(w, m, rs, rec, gmask, gbase, total, scat);
+                __syncthreads();
+                if (!err)
+                    worm_finish(w, m, rs, rec, gmask, gbase);
+            } else {
+                RecG rec{rec_vt(w), (const_u4 *)(const void *)rec_link(w)};
+                err = m.dt_mw == 1 ? worm_chase<RecG, true>(w, m, rs, rec, gmask, gbase, total, scat)
+                                   : worm_chase<RecG, false>(w, m, rs, rec, gmask, gbase, total, scat);
+                __syncthreads();
+                if (!err)
+                    worm_finish(w, m, rs, rec, gmask, gbase);
+            }
         }
     }
     if (err) {
@@ -1415,16 +1652,18 @@ __device__ __forceinline__ void k2_walker(DevWalker *W, const DevModel *models, 
 }
 
 // K2.  rec_mode 0 picks the smallest record layout that fits (Rec8, Rec10,
-// RecH, then RecG); 1 skips Rec8; 2 forces RecG; 3 forces RecH.
+// RecH, then RecG); 1 skips Rec8; 2 forces RecG; 3 forces RecH.  chase 1
+// runs the LDS layouts with record lookahead, 0 without.
 __global__ __launch_bounds__(64) void wave_worm_kernel(DevWalker *W, const DevModel *models,
                                                        DevPool *pool, int w_begin, int w_end,
                                                        int full_sweep, int lds_bytes, int rec_mode,
-                                                       const int32_t *order)
+                                                       const int32_t *order, int chase)
 {
     extern __shared__ __align__(16) char lds[];
     const int pos = w_begin + (int)blockIdx.x;
     if (pos < w_end)
-        k2_walker(W, models, pool, walker_at(order, pos), full_sweep, lds_bytes, rec_mode, lds);
+        k2_walker<false>(W, models, pool, walker_at(order, pos), full_sweep, lds_bytes, rec_mode, lds,
+                         chase);
 }
 
 // ------------------------------------------------------------------ init
@@ -1630,7 +1869,8 @@ __device__ __forceinline__ void k3_walker(DevWalker *W, const DevModel *models, 
 {
     const int lane = threadIdx.x;
     const DevWalker w = W[wi];
-    if (w.error || (mode == 1 && !w.active))
+    // a pending slot's opstring and spins are not initialised yet
+    if (w.error || w.pending || (mode == 1 && !w.active))
         return;
     if (mode == 1 && !(w.sweeps > w.thermalization)) {
         if (w.retire && lane == 0) {
@@ -1801,7 +2041,7 @@ static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, 
         (void)hipEventRecord(ev[1], s);
     if (phases & 4)
         hipLaunchKernelGGL(wave_worm_kernel, dim3(n), dim3(64), lds2, s, W, M, pool, w0, w1,
-                           (phases & 7) == 7 ? 1 : 0, (int)lds2, c.rec_mode, order);
+                           (phases & 7) == 7 ? 1 : 0, (int)lds2, c.rec_mode, order, c.chase);
     if (ev)
         (void)hipEventRecord(ev[2], s);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;

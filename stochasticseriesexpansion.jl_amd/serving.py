@@ -40,6 +40,7 @@ class Job:
     admitted: float = -1.0
     finished: float = -1.0
     result: tuple = None
+    state: tuple = None     # final (scalars, ops, spins) when the loop keeps states
 
 
 def magnet_square(L, kind):
@@ -82,7 +83,8 @@ def footprint_bytes(n_sites, n_bonds, T):
 
 class ServingLoop:
     def __init__(self, n_slots=512, max_L=48, memory_budget=8 << 30, sweeps_per_step=10,
-                 lib=None):
+                 lib=None, keep_state=False):
+        self.keep_state = keep_state
         self.model_cache = {}
         self.flats = []
         proto = magnet_square(max_L, "heis").flat()
@@ -153,6 +155,8 @@ class ServingLoop:
                     job.finished = tnow
                     sums, sq, cnt = acc[slot]
                     job.result = (sums.copy(), sq.copy(), cnt)
+                    if self.keep_state:
+                        job.state = b.get_walker(slot)
                     used -= fp
                     free.append(slot)
                     done.append(job)
@@ -222,7 +226,8 @@ class SizeClassServing:
     (warm_models), as a service keeps its model cache warm."""
 
     def __init__(self, classes=((4000, 512, 10, 32), (float("inf"), 512, 2, 48)),
-                 memory_budget=8 << 30, lib=None):
+                 memory_budget=8 << 30, lib=None, keep_state=False):
+        self.keep_state = keep_state
         total = sum(c[1] for c in classes)
         self.classes = [_Class(n_max, slots, sps, max_L, memory_budget * slots // total, lib)
                         for n_max, slots, sps, max_L in classes]
@@ -290,6 +295,8 @@ class SizeClassServing:
                     job.finished = tnow
                     sums, sq, cnt = acc[slot]
                     job.result = (sums.copy(), sq.copy(), cnt)
+                    if self.keep_state:
+                        job.state = b.get_walker(slot)
                     used -= fp
                     free.append(slot)
                     out.append(job)

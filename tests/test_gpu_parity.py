@@ -37,14 +37,17 @@ def _compare(batch, oracles, tag):
             assert gs[key] == os_[key], f"{tag} walker {k}: {key} {gs[key]} != {os_[key]}"
 
 
-def _run_parity(sse, oracle_mod, model, T, n_walkers, sweeps, mode, seed0=0, records=None, **kw):
+def _run_parity(sse, oracle_mod, model, T, n_walkers, sweeps, mode, seed0=0, records=None,
+                chase=None, **kw):
     flat = model.flat()
     walkers = [sse.walker_desc(T=T, seed=seed0 + s, **kw) for s in range(n_walkers)]
     _say(f"parity: mode={mode} N={flat.n_sites} walkers={n_walkers} sweeps={sweeps} "
-         f"records={records}")
+         f"records={records} chase={chase}")
     batch = sse.Batch([flat], walkers, mode=mode)
     if records is not None:
         batch.set_option(sse.capi.SSE_OPT_WORM_RECORDS, records)
+    if chase is not None:
+        batch.set_option(sse.capi.SSE_OPT_CHASE, chase)
     oracles = [oracle_mod.OracleWalker(sse.capi, flat, w) for w in walkers]
     batch.init()
     for o in oracles:
@@ -109,6 +112,19 @@ def test_parity_worm_record_layouts(sse_pkg, oracle_mod, records):
                 thermalization=10, records=records)
     _run_parity(sse_pkg, oracle_mod, job_honeycomb(sse_pkg, 0.4), 0.4, 4, 20, 0, seed0=50,
                 thermalization=10, records=records)
+
+
+@pytest.mark.parametrize("records", [0, 1])
+def test_parity_lookahead_chase(sse_pkg, oracle_mod, records):
+    # the record-lookahead traversal over 8-byte (0) and 10-byte (1) LDS
+    # records: one-worm (S=1/2) and multi-worm (spin-1, dim-4 cluster) tables,
+    # operators alone on their site (self-linked through the closure) at 4x4
+    _run_parity(sse_pkg, oracle_mod, square_heisenberg(sse_pkg, 4), 0.5, 6, 25, 0,
+                thermalization=10, records=records, chase=1)
+    _run_parity(sse_pkg, oracle_mod, job_honeycomb(sse_pkg, 0.4), 0.4, 4, 20, 0, seed0=50,
+                thermalization=10, records=records, chase=1)
+    _run_parity(sse_pkg, oracle_mod, ladder_dimer(sse_pkg, 8, 0.6), 0.2, 4, 20, 0, seed0=80,
+                thermalization=10, records=records, chase=1)
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -335,23 +351,24 @@ def _assert_same_walkers(a, b, n, tag):
             assert sa[key] == sb[key], f"{tag} walker {k}: {key}"
 
 
-@pytest.mark.skip(reason="persistent multi-sweep kernel disabled: its first GPU run ended in a "
-                  "memory fault whose cause is not found; the option is rejected by the library")
-def test_persistent_equals_per_sweep_kernels(sse_pkg, oracle_mod):
-    """Persistent multi-sweep launches (several chunks per call, fused
-    measurement, calls without measurement) give the per-sweep kernels'
-    result bit for bit, on a batch mixing two models."""
+@pytest.mark.parametrize("graph", [1, 0])
+def test_persistent_equals_per_sweep_kernels(sse_pkg, oracle_mod, graph):
+    """The persistent work-queue launch (several calls, with and without the
+    fused measurement, graph replays or direct launches) gives the per-sweep
+    kernels' result bit for bit, on a batch mixing two models."""
     sq = square_heisenberg(sse_pkg, 6).flat()
     hc = job_honeycomb(sse_pkg, 0.5).flat()
     wd = [sse_pkg.walker_desc(model=s % 2, T=0.3 + 0.05 * (s % 3), seed=300 + s, thermalization=12)
           for s in range(10)]
-    _say("persistent vs per-sweep kernels: 10 walkers, 2 models")
+    _say(f"persistent vs per-sweep kernels: 10 walkers, 2 models, graph={graph}")
     a = sse_pkg.Batch([sq, hc], wd)
     b = sse_pkg.Batch([sq, hc], wd)
+    a.set_option(sse_pkg.capi.SSE_OPT_PERSISTENT, 1)
+    a.set_option(sse_pkg.capi.SSE_OPT_GRAPH, graph)
     _eager(sse_pkg, b)
     a.init()
     b.init()
-    for rep, (k, meas) in enumerate(((37, True), (9, False), (70, True), (2, True))):
+    for rep, (k, meas) in enumerate(((37, True), (9, False), (70, True), (2, True), (37, True))):
         a.run(k, measure=meas)
         b.run(k, measure=meas)
         ms, sw = a.call_time()
@@ -364,7 +381,7 @@ def test_persistent_equals_per_sweep_kernels(sse_pkg, oracle_mod):
         o = oracle_mod.OracleWalker(sse_pkg.capi, [sq, hc][w], sse_pkg.walker_desc(
             T=wd[w].T, seed=wd[w].seed, thermalization=12))
         o.init()
-        o.sweep(37 + 9 + 70 + 2)
+        o.sweep(37 + 9 + 70 + 2 + 37)
         gs, gops, gspins = a.get_walker(w)
         os_, oops, ospins = o.state()
         assert np.array_equal(gops, oops) and np.array_equal(gspins, ospins)
@@ -374,16 +391,15 @@ def test_persistent_equals_per_sweep_kernels(sse_pkg, oracle_mod):
     b.close()
 
 
-@pytest.mark.skip(reason="persistent multi-sweep kernel disabled: its first GPU run ended in a "
-                  "memory fault whose cause is not found; the option is rejected by the library")
-def test_persistent_many_walkers_waits(sse_pkg):
-    """More walkers than fit at once and tiny sweeps, so workers often take a
-    walker's next sweep while its previous one still runs and must wait."""
+def test_persistent_many_walkers(sse_pkg):
+    """More walkers than the launch has workgroups: each workgroup takes
+    several walkers from the task counter."""
     flat = square_heisenberg(sse_pkg, 4).flat()
     wd = [sse_pkg.walker_desc(T=0.5, seed=900 + s, thermalization=3) for s in range(3000)]
     _say("persistent: 3000 walkers x 12 sweeps vs per-sweep kernels")
     a = sse_pkg.Batch([flat], wd)
     b = sse_pkg.Batch([flat], wd)
+    a.set_option(sse_pkg.capi.SSE_OPT_PERSISTENT, 1)
     _eager(sse_pkg, b)
     a.init()
     b.init()
@@ -394,6 +410,49 @@ def test_persistent_many_walkers_waits(sse_pkg):
         assert c1 == c2 and np.array_equal(s1, s2) and np.array_equal(q1, q2)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("records", [0, 2, 3])
+def test_run_budgets_against_oracle(sse_pkg, oracle_mod, records):
+    """sse_run_budgets: every walker of a heterogeneous batch advances by its
+    own sweep count in one persistent launch (0 for some), twice; each walker
+    equals its oracle twin after the same number of sweeps, in the LDS
+    records and in the global-memory layouts (RecGV / RecHV, links read by
+    vector loads)."""
+    rng = np.random.default_rng(21)
+    flats, walkers = [], []
+    for k in range(12):
+        L = int(rng.choice([4, 6, 8]))
+        model = square_heisenberg(sse_pkg, L, d=float(rng.uniform(-0.5, 0.5)), hz=0.1)
+        flats.append(model.flat())
+        beta = float(np.exp(rng.uniform(np.log(0.5), np.log(8.0))))
+        walkers.append(sse_pkg.walker_desc(model=k, T=1.0 / beta, seed=700 + k, thermalization=5))
+    _say(f"run_budgets: 12 walkers, records={records}")
+    batch = sse_pkg.Batch(flats, walkers)
+    batch.set_option(sse_pkg.capi.SSE_OPT_WORM_RECORDS, records)
+    batch.init()
+    b1 = rng.integers(0, 20, size=12)
+    b1[3] = 0
+    b2 = rng.integers(0, 20, size=12)
+    batch.run_budgets(b1, measure=True)
+    batch.run_budgets(b2, measure=True)
+    acc = batch.accumulators()
+    for k, w in enumerate(walkers):
+        o = oracle_mod.OracleWalker(sse_pkg.capi, flats[k], sse_pkg.walker_desc(
+            T=w.T, seed=w.seed, thermalization=5))
+        o.init()
+        rows = o.run_series(int(b1[k] + b2[k]))
+        gs, gops, gspins = batch.get_walker(k)
+        os_, oops, ospins = o.state()
+        assert np.array_equal(gops, oops) and np.array_equal(gspins, ospins), f"walker {k}"
+        for key in KEYS:
+            assert gs[key] == os_[key], f"walker {k}: {key}"
+        assert gs["sweeps"] == b1[k] + b2[k]
+        s1, s2, cnt = acc[k]
+        assert cnt == len(rows)
+        if cnt:
+            assert np.allclose(s1, rows.sum(axis=0), rtol=1e-12, atol=1e-12), f"walker {k}"
+    batch.close()
 
 
 def test_walker_groups_equal_single_stream(sse_pkg, oracle_mod):
