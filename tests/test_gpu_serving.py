@@ -100,3 +100,32 @@ def test_size_classes_equal_alone(sse_pkg):
         ms, ms2, mcnt = by_id[job.job_id].result
         assert mcnt == cnt == job.sweeps
         assert np.array_equal(ms, s) and np.array_equal(ms2, s2), f"job {job.job_id} differs"
+
+
+def test_streamed_bins_add_up(sse_pkg):
+    """With per-slot budgets every step streams one bin per measuring job:
+    bins hold exactly the job's step budget of measurements (a shorter last
+    one at most), their weighted sum is the job's final accumulator, and the
+    bins merge into a Carlo results entry."""
+    from importlib import import_module
+    sv = import_module("sse_amd.serving")
+    print("serving: streamed bins", flush=True)
+    jobs = _jobs(sv, 6, sweeps=40)
+    seen = []
+    loop = sv.SizeClassServing(classes=((200, 3, 3, 8), (float("inf"), 3, 2, 8)),
+                               on_bin=lambda job, k, row: seen.append((job.job_id, k)))
+    done, stats = loop.run(jobs, time_limit=120)
+    loop.close()
+    assert stats["jobs_done"] == 6 and stats["jobs_unfinished"] == 0
+    assert stats["bins_streamed"] == len(seen) > 6
+    for job in done:
+        s, s2, cnt = job.result
+        assert sum(job.bin_counts) == cnt == job.sweeps
+        assert all(k == job.step_budget for k in job.bin_counts[:-1])
+        assert 1 <= job.bin_counts[-1] <= job.step_budget
+        total = sum(k * row[:-1] for k, row in zip(job.bin_counts, job.bins))
+        assert np.allclose(total, s, rtol=1e-10, atol=1e-10), f"job {job.job_id}"
+        flat = sv.magnet_square(*job.model_key).flat()
+        entry = sv.job_results(job, flat)
+        if entry is not None:
+            assert "Energy" in entry["results"] and "SignMag" in entry["results"]
