@@ -182,6 +182,19 @@ int sse_read_accumulators(sse_batch *b, double *sums, double *sumsq,
  * temperature ladders that exchange between sweeps.  No reference
  * counterpart: the reference sweeps one MC per call.                        */
 int sse_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure);
+/* sse_run_budgets without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory during the
+ * call (serving loops that forbid it, harnesses that do).  The task counter
+ * and the budgets live in a static device array of each GPU the library
+ * runs on (about 0.75 MiB, held until the process ends), so the batch may
+ * hold at most SSE_NOALLOC_MAX_WALKERS walkers (SSE_ERR_INVALID beyond).
+ * Tasks run in the batch's schedule order; results are identical to
+ * sse_run_budgets.  No events or graphs are made, so sse_call_time is not
+ * updated.  Calls on one GPU share its array: calls that overlap on one GPU
+ * (two host threads, two batches) race and must be serialized by the
+ * caller.                                                                  */
+#define SSE_NOALLOC_MAX_WALKERS 65536
+int sse_run_budgets_noalloc(sse_batch *b, const int64_t *sweeps, int32_t measure);
 int sse_reset_accumulators(sse_batch *b);
 /* WormLengthFraction = (sum of worm lengths) / n, which the reference
  * measures inside the worm update of every thermalized sweep with n != 0

@@ -180,6 +180,7 @@ def load_library(path=None):
     lib.sse_measure_width.restype = C.c_int32
     lib.sse_run.argtypes = [vp, C.c_int32, C.c_int32]
     lib.sse_run_budgets.argtypes = [vp, C.POINTER(C.c_int64), C.c_int32]
+    lib.sse_run_budgets_noalloc.argtypes = [vp, C.POINTER(C.c_int64), C.c_int32]
     lib.sse_read_accumulators.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                           C.POINTER(C.c_int64)]
     lib.sse_reset_accumulators.argtypes = [vp]
@@ -236,7 +237,8 @@ EXPORTED_SYMBOLS = (
     "sse_slot_start", "sse_slot_stop", "sse_slot_poll", "sse_pool_usage", "sse_set_option",
     "sse_call_time", "sse_lds_bytes", "sse_pt_log_weight_ratio", "sse_pt_change_parameter",
     "sse_pt_get_parameter", "sse_pt_set_ladders", "sse_pt_exchange", "sse_pt_get_ladders",
-    "sse_vertex_transitions", "sse_read_worm_fraction", "sse_run_budgets")
+    "sse_vertex_transitions", "sse_read_worm_fraction", "sse_run_budgets",
+    "sse_run_budgets_noalloc")
 
 
 def vertex_transitions(dims, leg_states, weights, tolerance, lp_tolerance):
@@ -341,6 +343,15 @@ class Batch:
             raise ValueError("one sweep count per walker")
         check(self.lib, self.lib.sse_run_budgets(self.handle, _ptr(sw, C.c_int64),
                                                  int(bool(measure))))
+
+    def run_budgets_noalloc(self, sweeps, measure=True):
+        """run_budgets through the entry point that allocates no device
+        memory (static per-GPU scratch; at most SSE_NOALLOC_MAX_WALKERS)."""
+        sw = np.ascontiguousarray(sweeps, dtype=np.int64)
+        if sw.shape != (len(self.walkers),):
+            raise ValueError("one sweep count per walker")
+        check(self.lib, self.lib.sse_run_budgets_noalloc(self.handle, _ptr(sw, C.c_int64),
+                                                         int(bool(measure))))
 
     def accumulators(self):
         widths = self._slot_widths

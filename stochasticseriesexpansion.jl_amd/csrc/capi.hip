@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -1456,6 +1457,101 @@ int sse_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure)
         rc = call_timer(b, 2, 0);
     clear_phase_times(b);
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- no-alloc
+// Device scratch of sse_run_budgets_noalloc, one copy per GPU for the life of
+// the process: [0, 256) the task counter, then the caller's budgets as int64
+// [SSE_NOALLOC_MAX_WALKERS], then the same narrowed to int32.  Host code uses
+// only the per-device address from hipGetSymbolAddress, never the name.
+constexpr size_t kNoAllocBudget64 = 256;
+constexpr size_t kNoAllocBudget32 = kNoAllocBudget64 + 8 * (size_t)SSE_NOALLOC_MAX_WALKERS;
+constexpr size_t kNoAllocBytes = kNoAllocBudget32 + 4 * (size_t)SSE_NOALLOC_MAX_WALKERS;
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+__global__ void budget_narrow_kernel(const int64_t *in, int32_t *out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = (int32_t)in[i];
+}
+
+// scratch address per device, taken once per device with that device current
+static char *noalloc_scratch(int device)
+{
+    static std::mutex mu;
+    static char *ptr[64] = {};
+    if (device < 0 || device >= 64)
+        return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[device]) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess)
+            return nullptr;
+        if (cur != device && hipSetDevice(device) != hipSuccess)
+            return nullptr;
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device)
+            (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return nullptr;
+        ptr[device] = (char *)p;
+    }
+    return ptr[device];
+}
+
+extern "C" {
+
+int sse_run_budgets_noalloc(sse_batch *b, const int64_t *sweeps, int32_t measure)
+{
+    if (!b || !sweeps)
+        return fail(SSE_ERR_INVALID, "sse_run_budgets_noalloc: bad arguments");
+    if (!b->inited)
+        return fail(SSE_ERR_STATE, "run before init");
+    if (b->mode != SSE_MODE_WAVE)
+        return fail(SSE_ERR_INVALID, "sse_run_budgets_noalloc: wave mode only");
+    if (b->n_walkers > SSE_NOALLOC_MAX_WALKERS)
+        return fail(SSE_ERR_INVALID, "sse_run_budgets_noalloc: the batch has more walkers than the "
+                                     "static scratch holds (SSE_NOALLOC_MAX_WALKERS)");
+    if (b->pt_ladders > 0 && b->pt_interval > 0)
+        return fail(SSE_ERR_INVALID, "sse_run_budgets_noalloc: walkers of a temperature ladder "
+                                     "advance together (sse_run)");
+    const int n = b->n_walkers;
+    for (int w = 0; w < n; ++w)
+        if (sweeps[w] < 0 || sweeps[w] > INT32_MAX)
+            return fail(SSE_ERR_INVALID, "sse_run_budgets_noalloc: sweep counts must be in 0..2^31-1");
+    char *sc = noalloc_scratch(b->device);
+    if (!sc)
+        return fail(SSE_ERR_HIP, "sse_run_budgets_noalloc: no address for the device scratch");
+    if (b->any_pending) {
+        int rc = launch_init_pending(b);
+        if (rc)
+            return rc;
+    }
+    int64_t *bud64 = (int64_t *)(sc + kNoAllocBudget64);
+    int32_t *bud32 = (int32_t *)(sc + kNoAllocBudget32);
+    HIPCHECK(hipMemcpyAsync(bud64, sweeps, sizeof(int64_t) * n, hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(budget_narrow_kernel, dim3((n + 255) / 256), dim3(256), 0, b->stream, bud64,
+                       bud32, n);
+    HIPCHECK(hipGetLastError());
+    // tasks in the batch's schedule order (largest expected sweep first);
+    // walkers with budget 0 end their task at once.  The counter is cleared
+    // by the launch's reset kernel.
+    int rc = wave_launch_persist(b->wave, b->stream, b->d_walkers, b->d_models, b->d_pool, b->d_out,
+                                 b->d_sum, b->d_sumsq, b->d_count, b->d_wlf, b->d_done,
+                                 b->use_order ? b->d_order : nullptr, n, bud32, 0, measure ? 1 : 0,
+                                 (int32_t *)sc, b->n_cu);
+    if (rc)
+        return fail(rc, "persistent launch failed");
+    if (!measure && b->any_budget) {
+        hipLaunchKernelGGL(retire_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_walkers,
+                           n, b->d_done);
+        HIPCHECK(hipGetLastError());
+    }
+    return check_walker_errors(b);
 }
 
 int sse_read_accumulators(sse_batch *b, double *sums, double *sumsq, int64_t *counts)
