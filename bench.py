@@ -7,24 +7,29 @@ beta=8, Philox seeds 0..4095, random initial spins, cutoff grown by the
 reference rule, 2000 thermalization sweeps (untimed setup).  One step = one
 sweep (diagonal update + vertex list + worm update) of every walker followed
 by the measurement (energy, uniform and staggered magnetization family),
-accumulated on the device.  The timed steps run as one call (or in calls of
---chunk sweeps); the walkers are split into --streams groups, each swept on
-its own HIP stream so one group's slow worm-update tail overlaps the others'
-sweeps.  The process asks for --hw-queues (8) hardware queues
-(GPU_MAX_HW_QUEUES) and by default the library makes one group per 512
-walkers up to one per queue (8 for 4096 walkers); more streams than queues
-would share queues and serialize.
+accumulated on the device.  After thermalization the library's autotuner
+measures its launch forms (stream groups at several counts, the persistent
+work-queue launch at two LDS sizes, the record-lookahead chase) on real
+sweeps (untimed setup) and keeps the fastest; then the warm-up steps and
+the timed steps, in one call.  The process asks for --hw-queues (8) hardware
+queues (GPU_MAX_HW_QUEUES) so up to 8 stream groups get a queue each.
 
 Reported beside it, in the same run:
-  * the serving stream (open-loop Poisson arrivals of mixed jobs, 20 s of
-    arrivals by default): p50/p99 time-to-result of the jobs done;
+  * the roofline of the dominant kernel in the timed configuration: the same
+    number of steps again with HIP events around every kernel launch on the
+    stream it runs on (direct launches), giving each kernel's average launch
+    duration and the algorithmic bytes it moved;
+  * the serving stream (open-loop Poisson arrivals at 200 jobs/s of jobs
+    with 10^3-10^5 sweeps, 20 s of arrivals, results counted for 60 s):
+    p50/p99 time-to-result over all submitted jobs (unfinished ones at their
+    censoring time), completion fraction, bins streamed;
   * the plain GPU form: the same walkers (thermalized states) run one walker
     per launch with the lane kernel, and with the wave kernels;
-  * the CPU baseline: the C restatement oracle (one walker per thread) on
-    thermalized states copied from the GPU, for a bounded time;
-  * the roofline of the dominant kernel (the worm update): three extra
-    sweeps on one stream (untimed diagnostic) with HIP events around each
-    kernel give its launch duration for the whole batch.
+  * the plain single walker (8x8, beta=2, cutoff from 64) on the GPU and on
+    one CPU thread;
+  * the CPU baseline: the C restatement oracle, one walker per thread on the
+    process's CPU share, on thermalized states copied from the GPU, for a
+    bounded time.
 
 Vertex updates per walker-sweep U = L + W (W = scatter calls); algorithmic
 HBM bytes per walker-sweep B = 16L + 16n + 12W + 3N.
@@ -120,20 +125,46 @@ def cpu_baseline(sse, flat, wds, states, seconds, threads):
             "ns_per_operator_sweep": wall * threads / nops * 1e9 if nops else None}
 
 
-def measured_traffic(kernel, walkers, lattice, beta):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated) when it covers this
-    workload, else None."""
+def measured_traffic(kernel, walkers_per_launch):
+    """HBM bytes per launch of `kernel` over `walkers_per_launch` walkers of
+    the batched 16x16 beta=8 workload, from the committed rocprofv3 counter
+    summary of the timed configuration (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated), or None when it does not cover that launch."""
     import re
     path = os.path.join(ROOT, "profiles",
-                        "worm_kernel_hbm_traffic_heisenberg16x16_beta8_4096walkers.md")
+                        "hbm_traffic_timed_configuration_heisenberg16x16_beta8_4096walkers.md")
     try:
         text = open(path).read()
     except OSError:
         return None
-    want = f"traffic_workload: walkers={walkers} lattice={lattice} beta={beta:g} kernel={kernel}"
-    m = re.search(r"traffic_bytes_per_launch: (\d+)", text)
-    return int(m.group(1)) if (want in text and m) else None
+    m = re.search(rf"traffic: kernel={kernel} walkers_per_launch={walkers_per_launch} "
+                  rf"bytes_per_launch=(\d+)", text)
+    return int(m.group(1)) if m else None
+
+
+def issue_bound(kernel):
+    """Instruction-issue figures of `kernel` from the committed rocprofv3
+    counter summary of the same workload (SQ_INSTS_SALU / SQ_INSTS_VALU per
+    launch, the launch's duration): the rate at which the launch issued
+    scalar and vector instructions against MI355X's issue peak (256 CUs, one
+    scalar and one wave-wide vector instruction per CU per cycle at 2.4 GHz),
+    or None when the summary does not cover the kernel."""
+    import re
+    path = os.path.join(ROOT, "profiles", "issue_counters_heisenberg16x16_beta8_4096walkers.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(rf"issue: kernel={kernel} salu_per_launch=(\d+) valu_per_launch=(\d+) "
+                  rf"launch_ms=([0-9.]+)", text)
+    if not m:
+        return None
+    salu, valu, ms = int(m.group(1)), int(m.group(2)), float(m.group(3))
+    peak = 256 * 2.4e9  # instructions per second, per type
+    return {"salu_per_launch": salu, "valu_per_launch": valu, "launch_ms": ms,
+            "salu_frac": salu / (ms * 1e-3) / peak, "valu_frac": valu / (ms * 1e-3) / peak,
+            "peak_per_type": peak, "unit": "wave-instructions/s",
+            "source": os.path.relpath(path, ROOT)}
 
 
 def copy_calibration(mb):
@@ -181,7 +212,35 @@ def _ladder_flat(jl):
     return m.flat()
 
 
-def run_batch(sse, flats, wds, thermalization, steps, what):
+def tune(batch, what, max_calls=12):
+    """Run measured sweeps until the library's autotuner has chosen (setup,
+    untimed); returns its choice."""
+    for _ in range(max_calls):
+        batch.run(8, measure=True)
+        choice = batch.tuning(0)
+        if choice is not None:
+            log(f"{what}: tuned {choice}")
+            return choice
+    return None
+
+
+def parallelism(choice, walkers):
+    if not choice:
+        return "one wavefront per walker (untuned defaults)"
+    chase = "record-lookahead chase" if choice["chase"] else "scalar chase"
+    lds = ("worm LDS sized from the largest walker" if not choice["k2_lds"]
+           else f"worm LDS {choice['k2_lds']} B")
+    if choice["persistent"]:
+        return (f"autotuned: persistent work-queue launch (one workgroup per walker task, "
+                f"{walkers} tasks per call, HIP-graph replay), {lds}, {chase}")
+    return (f"autotuned: {choice['groups']} stream groups of per-sweep kernels (one stream and "
+            f"hardware queue per group, direct launches), one wavefront per walker, {lds}, "
+            f"{chase}")
+
+
+def run_batch(sse, flats, wds, thermalization, steps, what, budgets=None):
+    """Thermalize, tune, then time `steps` sweeps (equal budgets) or one
+    budgeted call per step (budgets: per-walker sweeps per call)."""
     batch = sse.Batch(flats, wds)
     t0 = time.perf_counter()
     batch.init()
@@ -191,20 +250,39 @@ def run_batch(sse, flats, wds, thermalization, steps, what):
         batch.sweep(k)
         done += k
     log(f"{what}: thermalized {thermalization} sweeps in {time.perf_counter() - t0:.1f} s")
-    batch.run(1, measure=True)
+    if budgets is None:
+        choice = tune(batch, what)
+        batch.run(2, measure=True)
+    else:
+        choice = None
+        for _ in range(12):  # budgeted calls tune over their first calls
+            batch.run_budgets(budgets, measure=True)
+            choice = batch.tuning(1)
+            if choice is not None:
+                break
+        log(f"{what}: tuned {choice}")
     batch.reset_accumulators()
     t1 = time.perf_counter()
-    batch.set_option(sse.capi.SSE_OPT_TIMING, 0)
-    batch.run(steps, measure=True)
+    if budgets is None:
+        batch.run(steps, measure=True)
+    else:
+        for _ in range(steps):
+            batch.run_budgets(budgets, measure=True)
     dt = time.perf_counter() - t1
     Ls, ns, Ws, wsw = batch.work_counters()
     batch.close()
     return {"vertex_updates_per_s": (Ls + Ws) / dt, "sweeps_per_s": wsw / dt,
-            "walkers": len(wds), "timed_sweeps": steps, "thermalization_sweeps": thermalization,
-            "mean_L": Ls / wsw, "mean_n": ns / wsw, "mean_W": Ws / wsw}
+            "walkers": len(wds), "thermalization_sweeps": thermalization,
+            "timed": (f"{steps} sweeps of every walker" if budgets is None else
+                      f"{steps} budgeted calls, {int(np.sum(budgets))} walker-sweeps each"),
+            "mean_L": Ls / wsw, "mean_n": ns / wsw, "mean_W": Ws / wsw,
+            "parallelism": parallelism(choice, len(wds)) if budgets is None else
+            (f"persistent work-queue launch with per-walker budgets (largest walker "
+             f"{int(np.min(budgets))} sweep(s) per call, smallest {int(np.max(budgets))}), tuned "
+             f"{choice}")}
 
 
-def extra_heterogeneous(sse, walkers=2048, thermalization=500, steps=20, seed=3):
+def heterogeneous_models(sse, walkers=2048, seed=3):
     """2048 walkers, square L in {8,12,16,24,32}, beta log-uniform [0.5,16],
     XXZ with Jz ~ U[0.5,1.5], Jxy=1, h=0.1; one model per walker."""
     rng = np.random.default_rng(seed)
@@ -221,16 +299,26 @@ def extra_heterogeneous(sse, walkers=2048, thermalization=500, steps=20, seed=3)
     log(f"heterogeneous: {walkers} per-walker tables built in {time.perf_counter() - t0:.1f} s")
     from sse_amd.sse_data import FlatModel, SSEData
     bonds = {L: [sse.SSEBond(b.type, (b.i, b.j)) for b in m.lattice.bonds] for L, m in base.items()}
-    flats, wds = [], []
+    flats = []
     for k in range(walkers):
         L = int(sizes[k])
         m = base[L]
         flats.append(FlatModel(SSEData(types[k], bonds[L]), m.opstring_estimators,
                                m.normalization_site_count(),
                                (base_flat[L].est_coef, base_flat[L].est_mval)))
-        wds.append(sse.walker_desc(model=k, T=1.0 / float(betas[k]), seed=100000 + k,
-                                   thermalization=thermalization))
-    r = run_batch(sse, flats, wds, thermalization, steps, "heterogeneous")
+    return flats, betas
+
+
+def extra_heterogeneous(sse, walkers=2048, thermalization=300, steps=6, seed=3):
+    """The heterogeneous XXZ batch, each walker advancing by its own budget per
+    call: about 4 sweeps of the largest walker's expected work (bonds x beta)
+    per call, so that every walker's budget takes about the same time."""
+    flats, betas = heterogeneous_models(sse, walkers, seed)
+    wds = [sse.walker_desc(model=k, T=1.0 / float(betas[k]), seed=100000 + k,
+                           thermalization=thermalization) for k in range(walkers)]
+    cost = np.array([flats[k].n_bonds * float(betas[k]) for k in range(walkers)])
+    budgets = np.minimum(400, np.maximum(1, np.round(4 * cost.max() / cost))).astype(np.int64)
+    r = run_batch(sse, flats, wds, thermalization, steps, "heterogeneous", budgets=budgets)
     r["workload"] = ("heterogeneous XXZ batch: 2048 walkers, L in {8,12,16,24,32}, "
                      "beta log-uniform [0.5,16], Jz ~ U[0.5,1.5], h=0.1")
     return r
@@ -251,41 +339,139 @@ def extra_ladder(sse, walkers=1024, thermalization=500, steps=20, seed=4):
     return r
 
 
-def extra_serving(sse, seconds=30.0, sweep_scale=1e-3, slots=1024):
+def serving_stream(sse, seconds=20.0, count_for=60.0, sweep_scale=1.0, slots=1024):
     """Open-loop Poisson stream (200 jobs/s, seed 7) of Heisenberg/XXZ jobs on
-    L in 8..48, beta log-uniform [1,32]; requested sweeps log-uniform
-    [1e3,1e5] scaled by sweep_scale to bound the run."""
+    L in 8..48, beta log-uniform [1,32], sweeps log-uniform [1e3,1e5] (times
+    sweep_scale); arrivals for `seconds`, results counted until `count_for`."""
     from importlib import import_module
     sv = import_module("sse_amd.serving")
     jobs = sv.poisson_jobs(int(200 * seconds), rate=200.0, seed=7, sweep_scale=sweep_scale)
     # size classes by estimated operator count, each its own batch, host
-    # thread and stream (small jobs do not wait behind large walkers)
+    # thread and stream; within a class per-slot budgets balance the jobs
     loop = sv.SizeClassServing(classes=((2000, slots * 3 // 16, 10, 48),
                                         (8000, slots * 5 // 16, 4, 48),
                                         (30000, slots * 5 // 16, 2, 48),
                                         (60000, slots * 2 // 16, 1, 48),
                                         (float("inf"), slots * 1 // 16, 1, 48)))
-    done, stats = loop.run(jobs, time_limit=seconds * 3,
+    done, stats = loop.run(jobs, time_limit=count_for,
                            log=lambda m: print(f"serving: {m}", file=sys.stderr, flush=True))
     loop.close()
-    stats["workload"] = (f"serving: Poisson 200 jobs/s seed 7 for {seconds:g} s (results "
-                         f"counted for {3 * seconds:g} s), L 8..48, beta 1..32, sweeps "
-                         f"log-uniform [1e3,1e5] x {sweep_scale:g}, {slots} slots in 5 size "
-                         f"classes (n <= 2000 / 8000 / 30000 / 60000 / more), models prebuilt")
-    stats["jobs_submitted"] = len(jobs)
+    scale = "" if sweep_scale == 1.0 else f" x {sweep_scale:g}"
+    stats["workload"] = (f"serving: Poisson 200 jobs/s seed 7 for {seconds:g} s of arrivals "
+                         f"(results counted for {count_for:g} s), Heisenberg/XXZ on L 8..48, "
+                         f"beta log-uniform 1..32, sweeps log-uniform [1e3,1e5]{scale}, "
+                         f"{slots} slots in 5 size classes (n <= 2000 / 8000 / 30000 / 60000 / "
+                         f"more) with per-slot budgets, models prebuilt")
     return stats
 
 
+def single_walker(sse, thermalization=10000, sweeps=5000):
+    """The plain single walker: 8x8 Heisenberg, beta=2, seed 1, initial cutoff
+    64, on the GPU (wave and lane kernels, one walker per batch) and on one
+    CPU thread (the oracle)."""
+    from oracle import pyoracle
+    model = sse.MagnetModel({"lattice": {"unitcell": sse.UnitCells.square, "size": (8, 8)},
+                             "J": 1.0, "measure": ["magnetization", "staggered_magnetization"]})
+    flat = model.flat()
+    wd = sse.walker_desc(T=0.5, seed=1, init_opstring_cutoff=64, thermalization=thermalization)
+    out = {"workload": f"8x8 Heisenberg beta=2, seed 1, cutoff from 64, {thermalization} "
+                       f"thermalization sweeps, {sweeps} timed sweeps with measurement"}
+    for name, mode in (("gpu_wave", sse.SSE_MODE_WAVE), ("gpu_lane", sse.SSE_MODE_LANE)):
+        b = sse.Batch([flat], [wd], mode=mode)
+        b.set_option(sse.capi.SSE_OPT_TIMING, 0)
+        b.init()
+        b.run(thermalization, measure=False)
+        b.reset_accumulators()
+        t0 = time.perf_counter()
+        b.run(sweeps, measure=True)
+        dt = time.perf_counter() - t0
+        Ls, ns, Ws, nsw = b.work_counters()
+        out[name] = {"vertex_updates_per_s": (Ls + Ws) / dt, "sweeps_per_s": nsw / dt,
+                     "mean_n": ns / nsw}
+        b.close()
+    o = pyoracle.OracleWalker(sse.capi, flat, wd)
+    o.init()
+    o.run_timed(thermalization, measure=False)
+    o.work_counters()
+    t, _ = o.run_timed(sweeps, measure=True)
+    Ls, ns, Ws, nsw = o.work_counters()
+    out["cpu_one_thread"] = {"vertex_updates_per_s": (Ls + Ws) / t, "sweeps_per_s": nsw / t,
+                             "kind": "port (C restatement oracle)"}
+    return out
+
+
+def summarize_rocprof(directory, last=0, regex=None):
+    """Markdown tables from rocprofv3 CSV output under `directory`: per kernel
+    and launch grid, the kernel-trace durations (calls, mean/min/max ms) and,
+    for counter collections, each counter's mean per dispatch over the last
+    `last` dispatches of that kernel and grid (all when 0)."""
+    import csv
+    import glob
+    import re
+    pat = re.compile(regex) if regex else None
+    lines = []
+
+    def col(row, *names):
+        for n in names:
+            if n in row:
+                return row[n]
+        return None
+
+    for path in sorted(glob.glob(os.path.join(directory, "**", "*kernel_trace.csv"), recursive=True)):
+        groups = {}
+        for r in csv.DictReader(open(path)):
+            name = (col(r, "Kernel_Name") or "").split("(")[0]
+            if pat and not pat.search(name):
+                continue
+            grid = col(r, "Grid_Size_X", "Grid_Size") or "?"
+            t = (int(col(r, "End_Timestamp")) - int(col(r, "Start_Timestamp"))) * 1e-6
+            groups.setdefault((name, grid), []).append(t)
+        lines.append(f"kernel trace {os.path.relpath(path, directory)}:")
+        lines.append("| kernel | grid | calls | avg ms | min ms | max ms | avg ms, last |")
+        lines.append("|---|---|---|---|---|---|---|")
+        for (name, grid), ts in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
+            tail = ts[-last:] if last else ts
+            lines.append(f"| {name} | {grid} | {len(ts)} | {np.mean(ts):.4f} | {min(ts):.4f} | "
+                         f"{max(ts):.4f} | {np.mean(tail):.4f} ({len(tail)}) |")
+    for path in sorted(glob.glob(os.path.join(directory, "**", "*counter_collection.csv"),
+                                 recursive=True)):
+        groups = {}
+        for r in csv.DictReader(open(path)):
+            name = (col(r, "Kernel_Name") or "").split("(")[0]
+            if pat and not pat.search(name):
+                continue
+            grid = col(r, "Grid_Size", "Grid_Size_X") or "?"
+            disp = int(col(r, "Dispatch_Id", "Correlation_Id") or 0)
+            cname = col(r, "Counter_Name")
+            groups.setdefault((name, grid, cname), {}).setdefault(disp, 0.0)
+            groups[(name, grid, cname)][disp] += float(col(r, "Counter_Value"))
+        lines.append(f"counters {os.path.relpath(path, directory)}:")
+        lines.append("| kernel | grid | counter | dispatches | mean per dispatch (last) |")
+        lines.append("|---|---|---|---|---|")
+        for (name, grid, cname), d in sorted(groups.items()):
+            vals = [d[k] for k in sorted(d)]
+            tail = vals[-last:] if last else vals
+            lines.append(f"| {name} | {grid} | {cname} | {len(vals)} | {np.mean(tail):.6g} "
+                         f"({len(tail)}) |")
+    return "\n".join(lines)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-rocprof":
+        sp = argparse.ArgumentParser(description="summarize rocprofv3 CSV output")
+        sp.add_argument("--summarize-rocprof", dest="directory", required=True)
+        sp.add_argument("--last", type=int, default=0)
+        sp.add_argument("--kernel-regex", default=None)
+        a = sp.parse_args()
+        print(summarize_rocprof(a.directory, a.last, a.kernel_regex))
+        return
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--chunk", type=int, default=0,
-                    help="sweeps per timed call (0: all timed steps in one call)")
     ap.add_argument("--streams", type=int, default=0,
-                    help="walker groups, each swept on its own HIP stream (0: the "
-                         "library's choice, one per 512 walkers up to one per hardware queue)")
+                    help="fix the walker groups (each swept on its own HIP stream) instead of "
+                         "letting the autotuner choose the launch form (0: autotuned)")
     ap.add_argument("--hw-queues", type=int, default=8,
                     help="GPU_MAX_HW_QUEUES for this process (HIP's default is 4): set "
                          "before the HIP runtime starts, so each walker group's stream "
@@ -294,19 +480,21 @@ def main():
     ap.add_argument("--lattice", type=int, default=16)
     ap.add_argument("--beta", type=float, default=8.0)
     ap.add_argument("--thermalization", type=int, default=2000)
-    ap.add_argument("--plain-walkers", type=int, default=2)
-    ap.add_argument("--plain-sweeps", type=int, default=3)
+    ap.add_argument("--plain-walkers", type=int, default=8)
+    ap.add_argument("--plain-sweeps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads of the CPU baseline (0: the process's CPU share, "
+                         "min(nproc, 16))")
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="end with three device-to-device copies of this many MiB "
                          "(reference kernel for HBM traffic counters)")
     ap.add_argument("--serving-seconds", type=float, default=20.0,
-                    help="arrival window of the serving stream reported in the default line "
-                         "(p50/p99 time-to-result; 0 skips it, --extra runs 30 s)")
+                    help="arrival window of the serving stream (0 skips it); results are "
+                         "counted for three times as long")
     ap.add_argument("--extra", action="store_true",
-                    help="also run the heterogeneous, ladder and serving workloads")
+                    help="also run the heterogeneous and ladder workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); walkers are independent "
@@ -334,28 +522,19 @@ def main():
         sc, _, _ = batch.get_walker(0, with_ops=False, with_spins=False)
         log(f"thermalization {done}/{args.thermalization} sweeps ({time.perf_counter() - t0:.1f} s)"
             f" walker0 L={sc['cutoff']} n={sc['num_operators']} worms={sc['num_worms']:.2f}")
-    # warm-up and timed region with the walkers split over `streams` stream
-    # groups; the timed region is `steps` sweeps (+ fused measurement) of every
-    # walker in calls of `chunk` sweeps
-    batch.set_option(sse.capi.SSE_OPT_STREAMS, args.streams)
-    queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4)
-    groups = args.streams or min(queues, max(2, args.walkers // 512))  # the library's rule
+    if args.streams:
+        batch.set_option(sse.capi.SSE_OPT_STREAMS, args.streams)
+        choice = None
+    else:
+        choice = tune(batch, "batched workload")
     if args.warmup:
         batch.run(args.warmup, measure=True)
     batch.reset_accumulators()
-    chunk = max(1, min(args.chunk if args.chunk > 0 else args.steps, args.steps))
-    call_ms, dev_ms = [], 0.0
-    done = 0
-    while done < args.steps:
-        k = min(chunk, args.steps - done)
-        t1 = time.perf_counter()
-        batch.run(k, measure=True)
-        call_ms.append((time.perf_counter() - t1) * 1e3 / k)
-        ms, sw = batch.call_time()
-        dev_ms += ms
-        done += k
+    t1 = time.perf_counter()
+    batch.run(args.steps, measure=True)
+    total_s = time.perf_counter() - t1
+    dev_ms, _ = batch.call_time()
     Ls, ns, Ws, wsweeps = batch.work_counters()
-    total_s = sum(call_ms[i] * min(chunk, args.steps - i * chunk) for i in range(len(call_ms))) / 1e3
     U = Ls + Ws
     B = 16 * Ls + 16 * ns + 12 * Ws + 3 * N * wsweeps
     value = U / total_s
@@ -363,44 +542,66 @@ def main():
         f"{wsweeps} walker-sweeps, <L>={Ls / wsweeps:.0f} <n>={ns / wsweeps:.0f} "
         f"<W>={Ws / wsweeps:.0f}")
 
-    # roofline of the dominant kernel: three more sweeps on one stream with
-    # HIP events around each kernel (untimed diagnostic); bytes per launch
-    # from the per-walker-sweep averages of the timed region
-    batch.set_option(sse.capi.SSE_OPT_STREAMS, 1)
-    ktimes = np.zeros(3)
-    for _ in range(3):
-        batch.run(1, measure=True)
-        ktimes += batch.phase_times()
-    batch.set_option(sse.capi.SSE_OPT_STREAMS, args.streams)
+    # roofline of the dominant kernel in the timed configuration: the same
+    # number of steps again with events around every launch on its stream
+    batch.reset_accumulators()
+    batch.set_option(sse.capi.SSE_OPT_TIMING, 2)
+    batch.run(args.steps, measure=True)
+    batch.set_option(sse.capi.SSE_OPT_TIMING, 1)
+    diag_ms, _ = batch.call_time()
+    kms, klaunch = batch.kernel_times()
+    dL, dn, dW, dsw = batch.work_counters()
+    names = ["diag_vlist", "worm", "measure", "persistent"]
+    kernels = {"diag_vlist": "wave_diag_vlist_kernel", "worm": "wave_worm_kernel",
+               "measure": "wave_measure_kernel", "persistent": "wave_persist_kernel"}
+    # algorithmic bytes of each kernel over the region (per walker-sweep:
+    # K1 12L + 16n + 2N, K2 12W + N, K3 4L; the persistent launch all of B)
+    region_bytes = {"diag_vlist": 12 * dL + 16 * dn + 2 * N * dsw, "worm": 12 * dW + N * dsw,
+                    "measure": 4 * dL,
+                    "persistent": 16 * dL + 16 * dn + 12 * dW + 3 * N * dsw}
+    per_kernel = {}
+    for i, nm in enumerate(names):
+        if klaunch[i]:
+            avg = kms[i] / klaunch[i]
+            bpl = region_bytes[nm] / klaunch[i]
+            per_kernel[nm] = {"launches": int(klaunch[i]), "avg_launch_ms": avg,
+                              "algorithmic_bytes_per_launch": bpl,
+                              "achieved_gbs_per_launch": bpl / (avg * 1e-3) / 1e9}
+    dom = max(per_kernel, key=lambda nm: kms[names.index(nm)])
+    d = per_kernel[dom]
+    if dom == "persistent":
+        walkers_per_launch = args.walkers
+    else:
+        g = args.streams or (choice["groups"] if choice else 1)
+        walkers_per_launch = args.walkers // max(1, g)
+    log("kernels in the timed configuration: " + ", ".join(
+        f"{nm} {v['launches']} x {v['avg_launch_ms']:.3f} ms" for nm, v in per_kernel.items()))
     lds = batch.lds_bytes()
-    names = ["diag_vlist", "worm", "measure"]
-    kms = {n: ktimes[i] / 3 for i, n in enumerate(names)}
-    nw = args.walkers
-    per_launch = {"diag_vlist": nw * (12 * Ls + 16 * ns) / wsweeps + 2 * N * nw,
-                  "worm": nw * 12 * Ws / wsweeps + N * nw,
-                  "measure": nw * 4 * Ls / wsweeps}
-    dom = max(names, key=lambda n: kms[n])
-    kname = {"diag_vlist": "wave_diag_vlist_kernel", "worm": "wave_worm_kernel",
-             "measure": "wave_measure_kernel"}[dom]
-    achieved = per_launch[dom] / (kms[dom] * 1e-3) / 1e9
-    log("single-stream kernels ms/launch: " + ", ".join(f"{n}={kms[n]:.3f}" for n in names))
-
-    states = [batch.get_walker(k) for k in range(max(args.plain_walkers, 8))]
+    states = [batch.get_walker(k) for k in range(max(args.plain_walkers, 16))]
     baselines = {}
     if not args.no_baselines:
         log("plain form: one walker per launch (lane kernel)")
-        plain_lane = plain_form(sse, flat, wds, states[:args.plain_walkers], args.plain_sweeps,
-                                sse.SSE_MODE_LANE)
+        plain_lane = plain_form(sse, flat, wds, states[:args.plain_walkers],
+                                max(1, args.plain_sweeps // 2), sse.SSE_MODE_LANE)
         log("plain form: one walker per launch (wave kernels)")
         plain_wave = plain_form(sse, flat, wds, states[:args.plain_walkers], args.plain_sweeps,
                                 sse.SSE_MODE_WAVE)
         baselines = {"plain_lane_vertex_updates_per_s": plain_lane,
                      "plain_wave_vertex_updates_per_s": plain_wave,
+                     "plain_sample": f"{args.plain_walkers} thermalized walkers, each its own "
+                                     f"batch: {args.plain_sweeps} sweeps (wave kernels) / "
+                                     f"{max(1, args.plain_sweeps // 2)} (lane kernel) each",
                      "batched_over_plain_lane": value / plain_lane,
                      "batched_over_plain_wave": value / plain_wave}
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        nproc = os.cpu_count() or 1
+        threads = args.cpu_threads or min(16, nproc)
         log(f"cpu baseline: {threads} threads x {args.cpu_seconds} s")
         cpu = cpu_baseline(sse, flat, wds, states, args.cpu_seconds, threads)
+        cpu["host_nproc"] = nproc
+        cpu["sample"] += (f"; {threads} threads = the process's CPU share on the GPU host "
+                          f"(min(nproc, 16), nproc = {nproc})")
+        log("plain single walker (8x8, beta=2)")
+        baselines["single_walker"] = single_walker(sse)
     else:
         cpu = None
     batch.close()
@@ -420,39 +621,46 @@ def main():
         "data": "synthetic (Philox-seeded random initial spins, thermalized chains)",
         "config": {"workload": f"batched Heisenberg {args.lattice}x{args.lattice} beta={args.beta:g}",
                    "walkers": args.walkers, "thermalization_sweeps": args.thermalization,
-                   "parallelism": f"one wavefront per walker, {groups} stream groups on "
-                                  f"{os.environ.get('GPU_MAX_HW_QUEUES')} hardware queues"},
+                   "parallelism": parallelism(choice, args.walkers) if not args.streams else
+                   f"{args.streams} stream groups (fixed), one wavefront per walker"},
         "sweeps_per_s": wsweeps / total_s,
         "hbm_gbs_algorithmic": B / total_s / 1e9,
         "hbm_roofline_frac": B / total_s / 1e9 / HBM_PEAK_GBS,
-        "step_ms_per_call": {"chunk": chunk, "p50": float(np.percentile(call_ms, 50)),
-                             "p99": float(np.percentile(call_ms, 99))},
         "device_ms_per_step": dev_ms / args.steps,
         "per_walker_sweep": {"L": Ls / wsweeps, "n": ns / wsweeps, "W": Ws / wsweeps,
                              "U": U / wsweeps, "B": B / wsweeps},
-        "kernel_ms_per_launch_single_stream": kms,
+        "tuning": choice,
+        "kernels_timed_configuration": per_kernel,
+        "kernel_region_ms_per_step": diag_ms / args.steps,
         "lds_bytes_per_walker": {"diag_vlist": lds[0], "worm": lds[1], "worm_walkers_per_cu": lds[2]},
-        "roofline": {"bound": "hbm", "kernel": kname,
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(kname, args.walkers, args.lattice, args.beta),
-                     "algorithmic_bytes_per_launch": per_launch[dom]},
+        "roofline": {"bound": "hbm", "kernel": kernels[dom],
+                     "achieved": d["achieved_gbs_per_launch"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": d["achieved_gbs_per_launch"] / HBM_PEAK_GBS,
+                     "traffic": (measured_traffic(kernels[dom], walkers_per_launch)
+                                 if (args.walkers, args.lattice, args.beta) == (4096, 16, 8.0)
+                                 else None),
+                     "algorithmic_bytes_per_launch": d["algorithmic_bytes_per_launch"],
+                     "avg_launch_ms": d["avg_launch_ms"], "launches": d["launches"],
+                     "aggregate_achieved_gbs": region_bytes[dom] / (diag_ms * 1e-3) / 1e9,
+                     "issue": issue_bound(kernels[dom])},
         "cpu_baseline": cpu,
     }
     out.update(baselines)
-    if args.serving_seconds > 0 and not args.extra:
+    if args.serving_seconds > 0:
         log(f"serving stream: {args.serving_seconds:g} s of arrivals")
         try:
-            st = extra_serving(sse, seconds=args.serving_seconds)
-            out["serving"] = {k: st[k] for k in ("p50_time_to_result_s", "p99_time_to_result_s",
-                                                 "jobs_done", "jobs_submitted", "wall_s",
-                                                 "vertex_updates_per_s", "workload")}
+            st = serving_stream(sse, seconds=args.serving_seconds,
+                                count_for=3 * args.serving_seconds)
+            keys = ("p50_time_to_result_s", "p99_time_to_result_s", "p50_is_lower_bound",
+                    "p99_is_lower_bound", "jobs_submitted", "jobs_done", "jobs_unfinished",
+                    "completion_fraction", "p50_done_only_s", "p99_done_only_s", "wall_s",
+                    "vertex_updates_per_s", "bins_streamed", "workload")
+            out["serving"] = {k: st.get(k) for k in keys}
         except Exception as e:  # the headline line is reported either way
             out["serving"] = {"error": f"{type(e).__name__}: {e}"}
     if args.extra:
         extras = {}
-        for name, fn in (("heterogeneous", extra_heterogeneous), ("ladder", extra_ladder),
-                         ("serving", extra_serving)):
+        for name, fn in (("heterogeneous", extra_heterogeneous), ("ladder", extra_ladder)):
             log(f"extra workload: {name}")
             extras[name] = fn(sse)
         out["extra_workloads"] = extras

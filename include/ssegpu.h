@@ -224,6 +224,11 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links,
  * (diagonal+vertex list, worm, measure), measured with HIP events on the
  * batch stream.                                                            */
 int sse_phase_times(const sse_batch *b, double *ms3);
+/* With SSE_OPT_TIMING 2: per kernel of the last sse_run/sse_sweep call the
+ * summed launch duration in ms and the launch count, [0] diagonal update +
+ * vertex list, [1] worm update, [2] measurement, [3] persistent launch.
+ * No reference counterpart (measurement).                                  */
+int sse_kernel_times(const sse_batch *b, double *ms4, int64_t *launches4);
 /* Device time (ms, HIP events on the batch stream) of the last sse_sweep /
  * sse_run call and its sweep count (0, 0 with timing off).  No reference
  * counterpart: the reference runs one walker per process.                  */
@@ -233,6 +238,14 @@ int sse_call_time(const sse_batch *b, double *ms, int64_t *sweeps);
  * worm-kernel workgroups a CU can hold at that size (HIP occupancy query).
  * No reference counterpart (tuning introspection).                         */
 int sse_lds_bytes(const sse_batch *b, int64_t *k1, int64_t *k2, int32_t *k2_per_cu);
+/* The autotuner's choice for sse_run (which 0) or sse_run_budgets (which 1):
+ * persistent launch (0/1), stream groups, chase variant, worm-kernel LDS
+ * setting (0 = sized from the operator counts) and the measured vertex
+ * updates per second; candidates receives the number of forms measured.
+ * SSE_ERR_STATE while still measuring.  No reference counterpart (tuning
+ * introspection).                                                          */
+int sse_tuning(const sse_batch *b, int32_t which, int32_t *persistent, int32_t *groups,
+               int32_t *chase, int64_t *k2_lds, double *rate, int32_t *candidates);
 /* Sum over walkers of cutoff L, operators n and scatter calls W since the
  * last reset (counters taken from the chains themselves).                  */
 int sse_work_counters(sse_batch *b, int64_t *L_sum, int64_t *n_sum,
@@ -293,12 +306,22 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
                        int64_t *proposed, int64_t *rounds);
 
 /* ---- options -------------------------------------------------------------
- * SSE_OPT_GRAPH  (default 1): sse_run/sse_sweep replay a captured HIP graph of
- *                up to 32 sweeps (+ measures) instead of launching each kernel.
+ * SSE_OPT_GRAPH  (default 1): sse_run/sse_sweep replay captured HIP graphs
+ *                instead of launching each kernel: up to 32 sweeps (+
+ *                measures) on one stream, the persistent launch of a call.  2
+ *                also captures the stream groups' fork/join pipeline (10
+ *                sweeps per graph, the group streams joined to the capture by
+ *                events); replays of that multi-stream graph measured slower
+ *                than direct launches (6.3 against 4.1 ms per sweep of 4096
+ *                16x16 beta=8 walkers in 8 groups), so it is not the default.
  * SSE_OPT_TIMING (default 1): every sweep/run call is bracketed by HIP events
  *                (sse_call_time); single-stream calls of <= 256 sweeps also
  *                record events around each kernel (sse_phase_times) and
- *                launch eagerly.
+ *                launch eagerly.  2: in addition, sse_run/sse_sweep record
+ *                events around every kernel launch on the stream it runs on
+ *                (each stream group's diagonal/vertex-list, worm and
+ *                measurement kernels, or the persistent launch), with direct
+ *                launches instead of graph replays (sse_kernel_times).
  * SSE_OPT_WORM_RECORDS (default 0): record layout of the worm kernel.  0 picks
  *                the smallest that fits (8-byte LDS records for n <= 4096 and
  *                <= 256 chase codes, 10-byte LDS records for n <= 16384, chase
@@ -338,6 +361,19 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
  *                instructions per step; slower where the step is bound by
  *                instruction issue, as on the batched Heisenberg workload).
  *                Results are identical.
+ * SSE_OPT_TUNE (default 1): autotuner.  The first measuring sse_run calls of
+ *                a wave-mode batch run their sweeps in turn (one sweep each,
+ *                twice round-robin) with the candidate launch forms: stream
+ *                groups at 2, 4 and hardware-queue-count groups; the
+ *                persistent work-queue launch with the worm-kernel LDS sized
+ *                from the largest walker or capped at 32 KiB; then the
+ *                winner with the record-lookahead chase.  Each is scored by
+ *                vertex updates (the chains' own counters) per second; the
+ *                fastest is kept.  sse_run_budgets tunes the same way over
+ *                its first calls (work-queue forms only).  Setting
+ *                SSE_OPT_STREAMS, SSE_OPT_PERSISTENT, SSE_OPT_K2_LDS or
+ *                SSE_OPT_CHASE turns tuning off; setting SSE_OPT_TUNE restarts
+ *                it.  Results are identical for every choice.
  * SSE_OPT_ORDER (default 1): launch schedule.  1 = walkers in order of their
  *                expected operator count (bonds / T), largest first, so the
  *                longest chains start first and each stream group holds walkers
@@ -351,7 +387,8 @@ enum sse_option {
     SSE_OPT_STREAMS = 5,
     SSE_OPT_K2_LDS = 6,
     SSE_OPT_ORDER = 7,
-    SSE_OPT_CHASE = 8
+    SSE_OPT_CHASE = 8,
+    SSE_OPT_TUNE = 9
 };
 int sse_set_option(sse_batch *b, int32_t option, int64_t value);
 

@@ -26,6 +26,7 @@ SSE_OPT_STREAMS = 5
 SSE_OPT_K2_LDS = 6
 SSE_OPT_ORDER = 7
 SSE_OPT_CHASE = 8
+SSE_OPT_TUNE = 9
 
 STATUS = {0: "SSE_OK", 1: "SSE_ERR_INVALID", 2: "SSE_ERR_HIP", 3: "SSE_ERR_OOM",
           4: "SSE_ERR_SCATTER", 5: "SSE_ERR_NO_DEVICE", 6: "SSE_ERR_STATE"}
@@ -181,6 +182,9 @@ def load_library(path=None):
     lib.sse_run.argtypes = [vp, C.c_int32, C.c_int32]
     lib.sse_run_budgets.argtypes = [vp, C.POINTER(C.c_int64), C.c_int32]
     lib.sse_run_budgets_noalloc.argtypes = [vp, C.POINTER(C.c_int64), C.c_int32]
+    lib.sse_tuning.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                               C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                               C.POINTER(C.c_int32)]
     lib.sse_read_accumulators.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                           C.POINTER(C.c_int64)]
     lib.sse_reset_accumulators.argtypes = [vp]
@@ -194,6 +198,7 @@ def load_library(path=None):
     lib.sse_get_vertex_list.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.c_int64,
                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     lib.sse_phase_times.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.sse_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     lib.sse_call_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     lib.sse_lds_bytes.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int32)]
@@ -238,7 +243,7 @@ EXPORTED_SYMBOLS = (
     "sse_call_time", "sse_lds_bytes", "sse_pt_log_weight_ratio", "sse_pt_change_parameter",
     "sse_pt_get_parameter", "sse_pt_set_ladders", "sse_pt_exchange", "sse_pt_get_ladders",
     "sse_vertex_transitions", "sse_read_worm_fraction", "sse_run_budgets",
-    "sse_run_budgets_noalloc")
+    "sse_run_budgets_noalloc", "sse_tuning", "sse_kernel_times")
 
 
 def vertex_transitions(dims, leg_states, weights, tolerance, lp_tolerance):
@@ -460,11 +465,34 @@ class Batch:
         check(self.lib, self.lib.sse_phase_times(self.handle, _ptr(t, C.c_double)))
         return t
 
+    def kernel_times(self):
+        """(ms sums, launch counts) per kernel of the last call with
+        SSE_OPT_TIMING 2: diag_vlist, worm, measure, persistent."""
+        ms = np.zeros(4)
+        n = np.zeros(4, dtype=np.int64)
+        check(self.lib, self.lib.sse_kernel_times(self.handle, _ptr(ms, C.c_double),
+                                                  _ptr(n, C.c_int64)))
+        return ms, n
+
     def call_time(self):
         """(device ms, sweeps) of the last sweep/run call (HIP events)."""
         ms, sw = C.c_double(), C.c_int64()
         check(self.lib, self.lib.sse_call_time(self.handle, C.byref(ms), C.byref(sw)))
         return ms.value, sw.value
+
+    def tuning(self, which=0):
+        """The autotuner's choice for sse_run (0) or sse_run_budgets (1) as a
+        dict, or None while it is still measuring."""
+        p, g, ch, cand = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        lds, rate = C.c_int64(), C.c_double()
+        st = self.lib.sse_tuning(self.handle, which, C.byref(p), C.byref(g), C.byref(ch),
+                                 C.byref(lds), C.byref(rate), C.byref(cand))
+        if st == 6:
+            return None
+        check(self.lib, st)
+        return {"persistent": bool(p.value), "groups": g.value, "chase": ch.value,
+                "k2_lds": lds.value, "vertex_updates_per_s": rate.value,
+                "candidates_measured": cand.value}
 
     def lds_bytes(self):
         """(K1 bytes, K2 bytes, K2 walkers per CU) of the current launches."""

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <mutex>
 #include <cstdio>
@@ -98,6 +99,21 @@ struct sse_batch {
     std::vector<hipEvent_t> evpool;  // 4 per sweep of the last timed call
     // captured sweep(+measure) sequence, replayed by sse_run/sse_sweep
     bool use_graph = true, use_timing = true;
+    // stream groups replayed from one captured multi-stream graph (SSE_OPT_GRAPH
+    // 2): measured slower than direct launches on this runtime, so opt-in
+    bool graph_groups = false;
+    // SSE_OPT_TIMING 2: events around every kernel launch of a call on the
+    // stream it runs on (stream groups and the persistent launch included;
+    // direct launches), summed per kernel by sse_kernel_times
+    bool kernel_timing = false;
+    double kern_ms[4] = {0, 0, 0, 0};
+    int64_t kern_launches[4] = {0, 0, 0, 0};
+    struct KernEv {
+        int kind;          // 0 K1 + K2 + K3 of one group-sweep (1 without K3), 3 persistent
+        size_t e;          // first of 4 events in kevpool
+    };
+    std::vector<KernEv> kern_ev;
+    std::vector<hipEvent_t> kevpool;
     hipGraphExec_t gexec = nullptr;
     GraphKey gkey;
     // persistent work-queue launch (SSE_OPT_PERSISTENT, sse_run_budgets):
@@ -108,6 +124,21 @@ struct sse_batch {
     // expected operator count per sweep of each walker's current job (bonds /
     // T of its model; a scheduling estimate kept by create and slot_start)
     std::vector<double> work_key;
+    // autotuner (SSE_OPT_TUNE): launch forms measured on the caller's own
+    // sweeps; all of them give the same chains and accumulators
+    struct TuneCfg {
+        int persistent, groups, chase;
+        size_t k2_lds;  // SSE_OPT_K2_LDS value (0 = sized from the operator counts)
+    };
+    struct Tuner {
+        std::vector<TuneCfg> cands;
+        std::vector<double> rate;  // best vertex updates per second seen per candidate
+        std::vector<int> runs;
+        int best = -1;             // chosen candidate (-1: still measuring)
+    };
+    bool tune = true;         // tuning enabled (an explicit launch option turns it off)
+    Tuner tune_run, tune_bud; // sse_run/sse_sweep calls; sse_run_budgets calls
+    unsigned long long *d_work = nullptr;  // vertex-update sum of a tuning measurement
     double phase_ms[3] = {0, 0, 0};
     int64_t phase_launches = 0;
     // wave mode: walkers split into groups, each swept on its own stream, so
@@ -748,6 +779,8 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
     if (!rc)
         rc = dev_alloc(b, &b->d_queue, 1);
     if (!rc)
+        rc = dev_alloc(b, &b->d_work, 1);
+    if (!rc)
         rc = dev_alloc(b, &b->d_budget, (size_t)n_walkers);
     if (!rc)
         rc = dev_alloc(b, &b->d_task_order, (size_t)n_walkers);
@@ -858,6 +891,8 @@ int sse_batch_destroy(sse_batch *b)
             (void)hipEventDestroy(b->ev[i]);
     for (hipEvent_t e : b->evpool)
         (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->kevpool)
+        (void)hipEventDestroy(e);
     if (b->gexec)
         (void)hipGraphExecDestroy(b->gexec);
     for (hipStream_t gs : b->gstreams)
@@ -899,6 +934,16 @@ static int launch_sweep(sse_batch *b, int phases, hipEvent_t *ev = nullptr)
     return SSE_OK;
 }
 
+static int ensure_kernel_events(sse_batch *b, size_t count)
+{
+    while (b->kevpool.size() < count) {
+        hipEvent_t e;
+        HIPCHECK(hipEventCreate(&e));
+        b->kevpool.push_back(e);
+    }
+    return SSE_OK;
+}
+
 static int ensure_events(sse_batch *b, size_t count)
 {
     while (b->evpool.size() < count) {
@@ -934,6 +979,38 @@ static void clear_phase_times(sse_batch *b)
     for (int i = 0; i < 3; ++i)
         b->phase_ms[i] = 0;
     b->phase_launches = 0;
+}
+
+// Per-kernel durations of the last call's timed launches (SSE_OPT_TIMING 2),
+// read after the call has synchronized.
+static void collect_kernel_times(sse_batch *b)
+{
+    for (int i = 0; i < 4; ++i) {
+        b->kern_ms[i] = 0;
+        b->kern_launches[i] = 0;
+    }
+    for (const auto &k : b->kern_ev) {
+        float a = 0, c = 0, d = 0;
+        const hipEvent_t *e = &b->kevpool[k.e];
+        if (k.kind == 3) {
+            (void)hipEventElapsedTime(&a, e[0], e[1]);
+            b->kern_ms[3] += a;
+            b->kern_launches[3] += 1;
+            continue;
+        }
+        (void)hipEventElapsedTime(&a, e[0], e[1]);
+        (void)hipEventElapsedTime(&c, e[1], e[2]);
+        b->kern_ms[0] += a;
+        b->kern_ms[1] += c;
+        b->kern_launches[0] += 1;
+        b->kern_launches[1] += 1;
+        if (k.kind == 0) {
+            (void)hipEventElapsedTime(&d, e[2], e[3]);
+            b->kern_ms[2] += d;
+            b->kern_launches[2] += 1;
+        }
+    }
+    b->kern_ev.clear();
 }
 
 static int launch_measure(sse_batch *b, int mode);
@@ -1076,10 +1153,17 @@ static int groups_prepare(sse_batch *b, int G)
     return SSE_OK;
 }
 
-static int groups_run(sse_batch *b, int n_sweeps, int measure)
+static int groups_run(sse_batch *b, int n_sweeps, int measure, bool ktime = false)
 {
     const int n = b->n_walkers;
     const int G = group_count(b);
+    size_t ev0 = 0;
+    if (ktime) {
+        ev0 = b->kern_ev.size() * 4;
+        int rc = ensure_kernel_events(b, ev0 + 4 * (size_t)G * n_sweeps);
+        if (rc)
+            return rc;
+    }
     // group 0 runs on the batch stream itself: G groups use G streams (the
     // device maps streams onto a few hardware queues; streams sharing one
     // queue would serialize behind each other)
@@ -1103,12 +1187,21 @@ static int groups_run(sse_batch *b, int n_sweeps, int measure)
             if (w0 >= w1)
                 continue;
             const int32_t *ord = b->use_order ? b->d_order : nullptr;
+            hipEvent_t *e = nullptr;
+            if (ktime) {
+                const size_t i = ev0 + 4 * ((size_t)k * G + g);
+                e = &b->kevpool[i];
+                b->kern_ev.push_back({measure ? 0 : 1, i});
+                HIPCHECK(hipEventRecord(e[0], gs));
+            }
             int rc = wave_launch_sweep(b->wave, gs, b->d_walkers, b->d_models, b->d_pool, w0, w1, 7,
-                                       nullptr, ord, g);
+                                       e, ord, g);
             if (!rc && measure)
                 rc = wave_launch_measure(b->wave, gs, b->d_walkers, b->d_models, b->d_out, b->d_sum,
                                          b->d_sumsq, b->d_count, b->d_wlf, w0, w1, 1, b->d_done,
                                          ord);
+            if (!rc && ktime)
+                rc = hipEventRecord(e[3], gs) == hipSuccess ? SSE_OK : SSE_ERR_HIP;
             if (rc)
                 return fail(rc, "group sweep launch failed");
         }
@@ -1226,11 +1319,31 @@ static int launch_pt_exchange(sse_batch *b);
 // events (event slots from k0), graph replays or plain launches.
 static int issue_sweeps(sse_batch *b, int n_sweeps, int measure, bool grouped, bool timed, int k0)
 {
-    if (b->mode == SSE_MODE_WAVE && b->persistent)
-        return persist_issue(b, n_sweeps, measure, nullptr, b->n_walkers,
-                             b->use_order ? b->d_order : nullptr);
-    if (grouped)
-        return b->use_graph ? groups_graph_many(b, n_sweeps, measure) : groups_run(b, n_sweeps, measure);
+    if (b->mode == SSE_MODE_WAVE && b->persistent) {
+        if (!b->kernel_timing)
+            return persist_issue(b, n_sweeps, measure, nullptr, b->n_walkers,
+                                 b->use_order ? b->d_order : nullptr);
+        const size_t ev0 = b->kern_ev.size() * 4;
+        int rc = ensure_kernel_events(b, ev0 + 4);
+        if (rc)
+            return rc;
+        hipEvent_t *e = &b->kevpool[ev0];
+        b->kern_ev.push_back({3, ev0});
+        HIPCHECK(hipEventRecord(e[0], b->stream));
+        const bool graph = b->use_graph;
+        b->use_graph = false;
+        rc = persist_issue(b, n_sweeps, measure, nullptr, b->n_walkers,
+                           b->use_order ? b->d_order : nullptr);
+        b->use_graph = graph;
+        HIPCHECK(hipEventRecord(e[1], b->stream));
+        return rc;
+    }
+    if (grouped) {
+        if (b->kernel_timing)
+            return groups_run(b, n_sweeps, measure, true);
+        return b->use_graph && b->graph_groups ? groups_graph_many(b, n_sweeps, measure)
+                                               : groups_run(b, n_sweeps, measure);
+    }
     if (timed) {
         for (int k = 0; k < n_sweeps; ++k) {
             hipEvent_t *e = &b->evpool[4 * (size_t)(k0 + k)];
@@ -1256,24 +1369,194 @@ static int issue_sweeps(sse_batch *b, int n_sweeps, int measure, bool grouped, b
     return SSE_OK;
 }
 
+// ---------------------------------------------------------------- autotuner
+// Sum over walkers of acc_L + acc_W (the vertex updates counted by the chains).
+__global__ void work_sum_kernel(const DevWalker *W, int n, unsigned long long *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        atomicAdd(out, (unsigned long long)(W[i].acc_L + W[i].acc_W));
+}
+__global__ void work_zero_kernel(unsigned long long *out) { *out = 0ull; }
+
+static int work_sum(sse_batch *b, unsigned long long *v)
+{
+    hipLaunchKernelGGL(work_zero_kernel, dim3(1), dim3(1), 0, b->stream, b->d_work);
+    hipLaunchKernelGGL(work_sum_kernel, dim3((b->n_walkers + 255) / 256), dim3(256), 0, b->stream,
+                       b->d_walkers, b->n_walkers, b->d_work);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(bmemcpy(b, v, b->d_work, sizeof *v, hipMemcpyDeviceToHost));
+    return SSE_OK;
+}
+
+static void tune_apply(sse_batch *b, const sse_batch::TuneCfg &c)
+{
+    b->persistent = c.persistent != 0;
+    b->n_groups = c.groups;
+    b->wave.chase = c.chase;
+    b->wave.lds_k2_fixed = c.k2_lds;
+    wave_resize_lds(b->wave, b->seen_max_n, b->seen_max_L);
+    for (size_t &g : b->wave.lds_k2_group)
+        g = 0;
+}
+
+// Candidates: the stream-group pipeline at 2, 4 and (queue count) groups
+// (not for budgeted calls, which always use the work-queue launch), and the
+// persistent work-queue launch with the worm-kernel LDS sized from the
+// largest walker or capped at 32 KiB (more workgroups per CU; larger walkers
+// then take the global record layouts).  The record-lookahead chase is tried
+// on the winner afterwards.
+static void tune_init(sse_batch *b, sse_batch::Tuner &t, bool budgets)
+{
+    t.cands.clear();
+    const int n = b->n_walkers;
+    if (!budgets) {
+        std::vector<int> gs = {2, 4, hw_queues()};
+        std::sort(gs.begin(), gs.end());
+        gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
+        for (int g : gs)
+            if (g <= std::max(1, n / 32))
+                t.cands.push_back({0, g, 0, 0});
+        if (t.cands.empty())
+            t.cands.push_back({0, 1, 0, 0});
+    }
+    t.cands.push_back({1, 0, 0, 0});
+    t.cands.push_back({1, 0, 0, 32768});
+    t.rate.assign(t.cands.size(), 0.0);
+    t.runs.assign(t.cands.size(), 0);
+    t.best = -1;
+}
+
+// Next candidate to measure (each twice, round-robin), or -1 when done.
+static int tune_next(const sse_batch::Tuner &t)
+{
+    for (int round = 0; round < 2; ++round)
+        for (size_t i = 0; i < t.cands.size(); ++i)
+            if (t.runs[i] == round)
+                return (int)i;
+    return -1;
+}
+
+// Record one measurement; when every candidate ran twice, keep the fastest
+// (after adding its lookahead-chase twin once).
+static void tune_record(sse_batch::Tuner &t, int i, double updates, double seconds)
+{
+    const double rate = seconds > 0 ? updates / seconds : 0.0;
+    t.rate[i] = std::max(t.rate[i], rate);
+    t.runs[i] += 1;
+    if (tune_next(t) >= 0)
+        return;
+    int best = 0;
+    for (size_t k = 1; k < t.cands.size(); ++k)
+        if (t.rate[k] > t.rate[best])
+            best = (int)k;
+    sse_batch::TuneCfg tw = t.cands[best];
+    if (tw.chase == 0) {
+        tw.chase = 1;
+        bool have = false;
+        for (auto &c : t.cands)
+            have |= c.persistent == tw.persistent && c.groups == tw.groups && c.chase == 1 &&
+                    c.k2_lds == tw.k2_lds;
+        if (!have) {
+            t.cands.push_back(tw);
+            t.rate.push_back(0.0);
+            t.runs.push_back(1);  // one measurement suffices for the twin
+            return;
+        }
+    }
+    t.best = best;
+}
+
+static int issue_sweeps(sse_batch *b, int n_sweeps, int measure, bool grouped, bool timed, int k0);
+static int sweep_call_rest(sse_batch *b, int32_t n_sweeps, int32_t measure, int tuned_sweeps);
+
+// Tuning inside an sse_run call: candidate segments of two sweeps each (one
+// when only one is left; direct launches: no graph capture is timed), the
+// worm-kernel LDS resized from the operator counts between segments, until a
+// choice is made or the call's sweeps run out.  Returns the sweeps it ran, or
+// -(error code).
+static int tune_in_call(sse_batch *b, int n_sweeps, int measure)
+{
+    sse_batch::Tuner &t = b->tune_run;
+    if (t.cands.empty())
+        tune_init(b, t, false);
+    const bool graph = b->use_graph;
+    b->use_graph = false;
+    int used = 0;
+    int rc = SSE_OK;
+    while (used < n_sweeps && t.best < 0) {
+        const int i = tune_next(t);
+        tune_apply(b, t.cands[i]);
+        const bool grouped = !b->persistent && group_count(b) > 1;
+        unsigned long long w0 = 0, w1 = 0;
+        rc = work_sum(b, &w0);
+        if (rc)
+            break;
+        const int k = std::min(2, n_sweeps - used);
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = issue_sweeps(b, k, measure, grouped, false, 0);
+        if (!rc && hipStreamSynchronize(b->stream) != hipSuccess)
+            rc = fail(SSE_ERR_HIP, "tuning: stream synchronize failed");
+        const auto t1 = std::chrono::steady_clock::now();
+        if (!rc)
+            rc = work_sum(b, &w1);
+        if (!rc)
+            rc = check_walker_errors(b);  // also resizes the worm-kernel LDS
+        if (rc)
+            break;
+        used += k;
+        tune_record(t, i, (double)(w1 - w0), std::chrono::duration<double>(t1 - t0).count());
+    }
+    b->use_graph = graph;
+    return rc ? -rc : used;
+}
+
 // Body of sse_sweep (measure 0) and sse_run: n_sweeps sweeps of every active
 // walker, each followed by the measurement of the thermalized ones when
 // measure is set, and an exchange round after every pt_interval-th sweep
 // (counted across calls) when temperature ladders are set.
 static int sweep_call(sse_batch *b, int32_t n_sweeps, int32_t measure)
 {
+    b->kern_ev.clear();
     if (b->any_pending) {
         int rc = launch_init_pending(b);
         if (rc)
             return rc;
     }
+    // the autotuner measures on the first measuring calls of a wave-mode
+    // batch (thermalization from random states is no steady state), not
+    // between the exchanges of temperature ladders
+    if (b->mode == SSE_MODE_WAVE && b->tune && !(b->pt_ladders > 0 && b->pt_interval > 0)) {
+        if (b->tune_run.best >= 0) {
+            tune_apply(b, b->tune_run.cands[b->tune_run.best]);
+        } else if (measure && n_sweeps >= 1) {
+            int rc0 = call_timer(b, 0, 0);
+            if (rc0)
+                return rc0;
+            const int used = tune_in_call(b, n_sweeps, 1);
+            if (used < 0)
+                return -used;
+            if (b->tune_run.best >= 0)
+                tune_apply(b, b->tune_run.cands[b->tune_run.best]);
+            return sweep_call_rest(b, n_sweeps - used, measure, used);
+        }
+    }
+    return sweep_call_rest(b, n_sweeps, measure, 0);
+}
+
+// sweep_call after tuning; `tuned_sweeps` were already run (and timed from
+// the call's start event when > 0)
+static int sweep_call_rest(sse_batch *b, int32_t n_sweeps, int32_t measure, int tuned_sweeps)
+{
     const bool grouped = b->mode == SSE_MODE_WAVE && !b->persistent && group_count(b) > 1;
     const bool timed = !grouped && !(b->mode == SSE_MODE_WAVE && b->persistent) && b->use_timing &&
                        n_sweeps <= 256;
     const bool pt = b->pt_ladders > 0 && b->pt_interval > 0;
-    int rc0 = call_timer(b, 0, 0);
-    if (rc0)
-        return rc0;
+    if (!tuned_sweeps) {
+        int rc0 = call_timer(b, 0, 0);
+        if (rc0)
+            return rc0;
+    }
     if (timed) {
         int rc = ensure_events(b, 4 * (size_t)n_sweeps);
         if (rc)
@@ -1302,13 +1585,15 @@ static int sweep_call(sse_batch *b, int32_t n_sweeps, int32_t measure)
                            b->d_walkers, b->n_walkers, b->d_done);
         HIPCHECK(hipGetLastError());
     }
-    int rc1 = call_timer(b, 1, n_sweeps);
+    int rc1 = call_timer(b, 1, n_sweeps + tuned_sweeps);
     if (rc1)
         return rc1;
     int rc = check_walker_errors(b);
+    if (b->kernel_timing)
+        collect_kernel_times(b);
     if (!rc)
         rc = call_timer(b, 2, 0);
-    if (timed)
+    if (timed && !tuned_sweeps)
         collect_phase_times(b, n_sweeps, measure != 0);
     else
         clear_phase_times(b);
@@ -1433,9 +1718,32 @@ int sse_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure)
         if (rc)
             return rc;
     }
+    // autotuner: one candidate per call (each twice), then the fastest
+    sse_batch::Tuner &t = b->tune_bud;
+    int cand = -1;
+    const bool graph = b->use_graph;
+    const bool persistent = b->persistent;
+    if (b->tune && measure) {
+        if (t.cands.empty())
+            tune_init(b, t, true);
+        if (t.best >= 0) {
+            tune_apply(b, t.cands[t.best]);
+        } else {
+            cand = tune_next(t);
+            tune_apply(b, t.cands[cand]);
+            b->use_graph = false;
+        }
+    }
+    unsigned long long w0 = 0, w1 = 0;
+    if (cand >= 0) {
+        int rw = work_sum(b, &w0);
+        if (rw)
+            return rw;
+    }
     int rc = call_timer(b, 0, 0);
     if (rc)
         return rc;
+    const auto t0 = std::chrono::steady_clock::now();
     if (!order.empty()) {
         HIPCHECK(bmemcpy(b, b->d_budget, bud.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
         HIPCHECK(bmemcpy(b, b->d_task_order, order.data(), sizeof(int32_t) * order.size(),
@@ -1444,6 +1752,17 @@ int sse_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure)
         if (rc)
             return rc;
     }
+    if (cand >= 0) {
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        const auto t1 = std::chrono::steady_clock::now();
+        int rw = work_sum(b, &w1);
+        if (rw)
+            return rw;
+        tune_record(t, cand, (double)(w1 - w0), std::chrono::duration<double>(t1 - t0).count());
+        b->use_graph = graph;
+    }
+    if (b->tune && measure)  // sse_run keeps its own launch form
+        b->persistent = persistent;
     if (!measure && b->any_budget) {
         hipLaunchKernelGGL(retire_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_walkers,
                            n, b->d_done);
@@ -1977,10 +2296,16 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         return fail(SSE_ERR_INVALID, "null batch");
     switch (option) {
     case SSE_OPT_GRAPH:
+        if (value < 0 || value > 2)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_GRAPH takes 0, 1 or 2");
         b->use_graph = value != 0;
+        b->graph_groups = value == 2;
         return SSE_OK;
     case SSE_OPT_TIMING:
+        if (value < 0 || value > 2)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_TIMING takes 0, 1 or 2");
         b->use_timing = value != 0;
+        b->kernel_timing = value == 2;
         return SSE_OK;
     case SSE_OPT_PERSISTENT:
         if (value != 0 && value != 1)
@@ -1989,11 +2314,21 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
             return fail(SSE_ERR_INVALID, "SSE_OPT_PERSISTENT needs wave mode");
         HIPCHECK(hipStreamSynchronize(b->stream));
         b->persistent = value != 0;
+        b->tune = false;  // the caller chose the launch form
+        return SSE_OK;
+    case SSE_OPT_TUNE:
+        if (value != 0 && value != 1)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_TUNE takes 0 or 1");
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        b->tune = value != 0;
+        b->tune_run = sse_batch::Tuner();
+        b->tune_bud = sse_batch::Tuner();
         return SSE_OK;
     case SSE_OPT_K2_LDS: {
         if (value != 0 && (value < 8 * (int64_t)K2Stream::kNW + 256 || value > (int64_t)b->wave.lds_limit))
             return fail(SSE_ERR_INVALID, "SSE_OPT_K2_LDS: 0 or a size the device allows");
         HIPCHECK(hipStreamSynchronize(b->stream));
+        b->tune = false;
         b->wave.lds_k2_fixed = (size_t)value & ~(size_t)15;
         wave_resize_lds(b->wave, b->seen_max_n, b->seen_max_L);
         for (size_t &g : b->wave.lds_k2_group)
@@ -2012,6 +2347,7 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         if (value < 0 || value > 64)
             return fail(SSE_ERR_INVALID, "SSE_OPT_STREAMS takes 0 (automatic) or 1..64 walker groups");
         HIPCHECK(hipStreamSynchronize(b->stream));
+        b->tune = false;
         b->n_groups = (int)value;
         for (size_t &g : b->wave.lds_k2_group)  // the groups hold other walkers now
             g = 0;
@@ -2020,6 +2356,7 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         if (value != 0 && value != 1)
             return fail(SSE_ERR_INVALID, "SSE_OPT_CHASE takes 0 (plain) or 1 (record lookahead)");
         HIPCHECK(hipStreamSynchronize(b->stream));
+        b->tune = false;
         b->wave.chase = (int)value;
         return SSE_OK;
     case SSE_OPT_WORM_RECORDS:
@@ -2032,6 +2369,33 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
     default:
         return fail(SSE_ERR_INVALID, "unknown option");
     }
+}
+
+int sse_tuning(const sse_batch *b, int32_t which, int32_t *persistent, int32_t *groups,
+               int32_t *chase, int64_t *k2_lds, double *rate, int32_t *candidates)
+{
+    if (!b || (which != 0 && which != 1))
+        return fail(SSE_ERR_INVALID, "sse_tuning: which is 0 (sse_run) or 1 (sse_run_budgets)");
+    const sse_batch::Tuner &t = which ? b->tune_bud : b->tune_run;
+    int measured = 0;
+    for (int r : t.runs)
+        measured += r > 0;
+    if (candidates)
+        *candidates = measured;
+    if (t.best < 0)
+        return fail(SSE_ERR_STATE, "sse_tuning: no choice made yet");
+    const sse_batch::TuneCfg &c = t.cands[t.best];
+    if (persistent)
+        *persistent = c.persistent;
+    if (groups)
+        *groups = c.groups;
+    if (chase)
+        *chase = c.chase;
+    if (k2_lds)
+        *k2_lds = (int64_t)c.k2_lds;
+    if (rate)
+        *rate = t.rate[t.best];
+    return SSE_OK;
 }
 
 int sse_lds_bytes(const sse_batch *b, int64_t *k1, int64_t *k2, int32_t *k2_per_cu)
@@ -2059,6 +2423,17 @@ int sse_call_time(const sse_batch *b, double *ms, int64_t *sweeps)
         *ms = b->call_ms;
     if (sweeps)
         *sweeps = b->call_sweeps;
+    return SSE_OK;
+}
+
+int sse_kernel_times(const sse_batch *b, double *ms4, int64_t *launches4)
+{
+    if (!b || !ms4 || !launches4)
+        return fail(SSE_ERR_INVALID, "sse_kernel_times: bad arguments");
+    for (int i = 0; i < 4; ++i) {
+        ms4[i] = b->kern_ms[i];
+        launches4[i] = b->kern_launches[i];
+    }
     return SSE_OK;
 }
 

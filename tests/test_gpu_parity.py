@@ -455,18 +455,22 @@ def test_run_budgets_against_oracle(sse_pkg, oracle_mod, records):
     batch.close()
 
 
-def test_walker_groups_equal_single_stream(sse_pkg, oracle_mod):
-    """Walker groups on separate streams (default) give the single-stream
-    result bit for bit: calls of several lengths with and without the fused
-    measurement, a batch mixing two models, and the oracle for two walkers."""
+@pytest.mark.parametrize("graph", [1, 2])
+def test_walker_groups_equal_single_stream(sse_pkg, oracle_mod, graph):
+    """Walker groups on separate streams give the single-stream result bit for
+    bit: calls of several lengths with and without the fused measurement, a
+    batch mixing two models, and the oracle for two walkers; direct launches
+    (graph 1) and replays of the captured multi-stream pipeline (graph 2:
+    chunks of 10 sweeps, the rest launched directly)."""
     sq = square_heisenberg(sse_pkg, 6).flat()
     hc = job_honeycomb(sse_pkg, 0.5).flat()
     wd = [sse_pkg.walker_desc(model=s % 2, T=0.3 + 0.05 * (s % 3), seed=300 + s, thermalization=12)
           for s in range(10)]
-    _say("walker groups vs one stream: 10 walkers, 2 models")
+    _say(f"walker groups vs one stream: 10 walkers, 2 models, graph={graph}")
     a = sse_pkg.Batch([sq, hc], wd)
     b = sse_pkg.Batch([sq, hc], wd)
     a.set_option(sse_pkg.capi.SSE_OPT_STREAMS, 3)
+    a.set_option(sse_pkg.capi.SSE_OPT_GRAPH, graph)
     _eager(sse_pkg, b)
     a.init()
     b.init()
