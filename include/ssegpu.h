@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SSEGPU_ABI_VERSION 1
+#define SSEGPU_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 enum sse_status {
@@ -60,36 +60,44 @@ enum sse_status {
 #define SSE_VCODE_INVALID (-1)
 
 /* ---- VertexData of one bond type (src/vertex_data.jl:13-28) -------------
- * Only two-site bonds (leg_count 4) are supported by the device kernels.
- * Legs are 0-based: 0,1 = lower legs of bond sites 0,1 (state before the
- * operator), 2,3 = upper legs (state after).                               */
+ * A bond type acts on n_sites sites (VertexData{NSites}, 1 <= NSites <= 4),
+ * so its vertices have leg_count = 2 * n_sites legs, 0-based: legs
+ * 0..n_sites-1 are the lower legs of bond sites 0..n_sites-1 (state before
+ * the operator), n_sites..2*n_sites-1 the upper legs (state after).  The
+ * wave kernels (SSE_MODE_WAVE) take two-site bonds; the lane kernels
+ * (SSE_MODE_LANE) any n_sites.                                              */
+#define SSE_MAX_BOND_SITES 4
 typedef struct sse_vertex_data_desc {
-    int32_t dims[2];                 /* local dimension of the two bond sites  */
+    int32_t n_sites;                 /* NSites, 1..SSE_MAX_BOND_SITES           */
+    int32_t dims[SSE_MAX_BOND_SITES];/* local dimension of each bond site      */
     double energy_offset;            /* src/vertex_data.jl:129-135             */
     int32_t n_vertices;
     const double *weights;           /* [n_vertices] |w|                       */
     const int8_t *signs;             /* [n_vertices] sign of w                 */
-    const uint8_t *leg_states;       /* [n_vertices][4], 1-based local states  */
-    const int32_t *diagonal_vertices;/* [dims0*dims1] vcode or -1; index
-                                        (s0-1) + dims0*(s1-1) (join_idx)       */
+    const uint8_t *leg_states;       /* [n_vertices][2*n_sites], 1-based       */
+    const int32_t *diagonal_vertices;/* [prod(dims)] vcode or -1; index
+                                        sum_k (s_k - 1) * prod_{j<k} dims_j,
+                                        the first site fastest (join_idx)      */
     int32_t max_worm_count;
-    const int32_t *transition_offset;/* [n_vertices][max_worm_count][4]: first
-                                        exit index or -1 (invalid Transition)  */
+    const int32_t *transition_offset;/* [n_vertices][max_worm_count][2*n_sites]:
+                                        first exit index or -1 (invalid)       */
     const int32_t *transition_count; /* same shape: number of exits            */
     int32_t n_exits;
     const double *exit_cumprob;      /* [n_exits] cumulative probabilities     */
     const int32_t *exit_target;      /* [n_exits] target vcode                 */
-    const uint8_t *exit_leg;         /* [n_exits] 0-based exit leg             */
+    const uint8_t *exit_leg;         /* [n_exits] 0-based exit leg (< 2*n_sites) */
     const uint8_t *exit_worm;        /* [n_exits] 1-based exit worm function   */
 } sse_vertex_data_desc;
 
-/* ---- SSEData + estimator tables (src/sse_data.jl:15-22) ------------------ */
+/* ---- SSEData + estimator tables (src/sse_data.jl:15-22) ------------------
+ * All bonds of a model act on the same number of sites (SSEData{NSites}). */
 typedef struct sse_model_desc {
     int32_t n_sites;
     const uint8_t *site_dims;        /* [n_sites]                              */
     int32_t n_bonds;
+    int32_t bond_n_sites;            /* NSites of every bond and bond type     */
     const int32_t *bond_type;        /* [n_bonds] 0-based VertexData index     */
-    const int32_t *bond_sites;       /* [n_bonds][2] 0-based site indices      */
+    const int32_t *bond_sites;       /* [n_bonds][bond_n_sites] 0-based sites  */
     int32_t n_types;
     const sse_vertex_data_desc *types;
     double energy_offset;            /* sum over bonds of type offsets         */
@@ -215,8 +223,9 @@ int sse_set_walker(sse_batch *b, int32_t walker, const sse_walker_scalars *s,
  * 1 = diagonal update, 2 = vertex list, 4 = worm update.  Used to compare
  * intermediate results with a CPU restatement.                            */
 int sse_sweep_phases(sse_batch *b, int32_t phases);
-/* Vertex list of walker w after the last vertex-list phase:
- * links [4*cutoff] linked leg ids (4p+leg) or -1, first/last [n_sites]. */
+/* Vertex list of walker w after the last vertex-list phase: links
+ * [leg_count*cutoff] linked leg ids (leg_count*p + leg) or -1, first/last
+ * [n_sites]; leg_count = 2 * bond_n_sites.                                */
 int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links,
                         int64_t links_capacity, int32_t *first, int32_t *last);
 

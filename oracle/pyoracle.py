@@ -59,9 +59,9 @@ def lib(capi):
                                C.POINTER(C.c_uint8)]
     L.so_get_vertex_list.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int64)]
-    L.so_make_vertex_list.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
-                                      C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
-                                      C.POINTER(C.c_int64)]
+    L.so_make_vertex_list.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_uint32), C.c_int64, C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.so_run_timed.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
     L.so_run_timed.restype = C.c_double
     L.so_run_series.argtypes = [vp, C.c_int64, C.POINTER(C.c_double)]
@@ -152,12 +152,13 @@ class OracleWalker:
 
     def vertex_list(self):
         L = self.scalars()["cutoff"]
-        links = np.zeros(max(1, 4 * L), dtype=np.int64)
+        legs = 2 * self.flat.bond_n_sites
+        links = np.zeros(max(1, legs * L), dtype=np.int64)
         first = np.zeros(self.flat.n_sites, dtype=np.int64)
         last = np.zeros(self.flat.n_sites, dtype=np.int64)
         self.L.so_get_vertex_list(self.h, _p(links, C.c_int64), _p(first, C.c_int64),
                                   _p(last, C.c_int64))
-        return links[:4 * L], first, last
+        return links[:legs * L], first, last
 
     def run_series(self, n_sweeps):
         out = np.zeros((n_sweeps, self.flat.measure_width))
@@ -190,16 +191,19 @@ class OracleWalker:
 
 
 def make_vertex_list(capi, n_sites, bond_sites, ops):
+    """make_vertex_list! of the oracle; bond_sites is [n_bonds][NSites]."""
     L_ = lib(capi)
     ops = np.ascontiguousarray(ops, dtype=np.uint32)
     bs = np.ascontiguousarray(bond_sites, dtype=np.int32)
+    ns = bs.shape[1] if bs.ndim == 2 else 2
+    legs = 2 * ns
     L = len(ops)
-    links = np.zeros(max(1, 4 * L), dtype=np.int64)
+    links = np.zeros(max(1, legs * L), dtype=np.int64)
     first = np.zeros(n_sites, dtype=np.int64)
     last = np.zeros(n_sites, dtype=np.int64)
-    L_.so_make_vertex_list(n_sites, _p(bs, C.c_int32), _p(ops, C.c_uint32), L,
+    L_.so_make_vertex_list(n_sites, ns, _p(bs, C.c_int32), _p(ops, C.c_uint32), L,
                            _p(links, C.c_int64), _p(first, C.c_int64), _p(last, C.c_int64))
-    return links[:4 * L], first, last
+    return links[:legs * L], first, last
 
 
 def philox_block(capi, ctr, key):

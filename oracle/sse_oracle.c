@@ -171,7 +171,7 @@ struct so_walker {
     int64_t n, L, cap;
     uint32_t *ops;
     uint8_t *state;
-    int64_t *vert;          /* [4*cap] linked leg id 4p+leg, or -1 */
+    int64_t *vert;          /* [legs*cap] linked leg id legs*p+leg, or -1 */
     int64_t *v_first, *v_last;
     uint64_t key, ctr;
     int64_t thermalization, sweeps, init_cutoff;
@@ -186,6 +186,26 @@ static inline uint32_t op_bond(uint32_t op) { return op >> SSE_OP_BOND_SHIFT; }
 static inline uint32_t op_vcode(uint32_t op) { return (op >> 1) & ((1u << SSE_OP_VCODE_BITS) - 1); }
 static inline uint32_t op_make(uint32_t bond, uint32_t vcode) { return 1u | (vcode << 1) | (bond << SSE_OP_BOND_SHIFT); }
 static inline const sse_vertex_data_desc *bond_vd(const sse_model_desc *m, uint32_t b) { return &m->types[m->bond_type[b]]; }
+/* Bonds act on NS = bond_n_sites sites (SSEData{NSites}); an operator has
+ * leg_count = 2 NS legs, leg ids leg_count * p + leg, legs 0..NS-1 lower and
+ * NS..2NS-1 upper; site_of_leg(l) = l mod NS (vertex_data.jl:190).          */
+static inline int nsb_of(const sse_model_desc *m) { return m->bond_n_sites; }
+static inline const int32_t *bond_sites_of(const sse_model_desc *m, uint32_t b)
+{
+    return m->bond_sites + (size_t)m->bond_n_sites * b;
+}
+/* join_idx over the bond's sites, the first site fastest (util.jl:3-13) */
+static int bond_join(const sse_model_desc *m, const sse_vertex_data_desc *vd, uint32_t b,
+                     const uint8_t *state)
+{
+    const int32_t *bs = bond_sites_of(m, b);
+    int idx = 0, stride = 1;
+    for (int i = 0; i < nsb_of(m); ++i) {
+        idx += (state[bs[i]] - 1) * stride;
+        stride *= vd->dims[i];
+    }
+    return idx;
+}
 
 so_walker *so_walker_create(const sse_model_desc *model, const sse_walker_desc *wd)
 {
@@ -225,7 +245,7 @@ static void ensure_cap(so_walker *w, int64_t L)
         return;
     int64_t cap = L;
     w->ops = (uint32_t *)realloc(w->ops, sizeof(uint32_t) * (size_t)cap);
-    w->vert = (int64_t *)realloc(w->vert, sizeof(int64_t) * 4 * (size_t)cap);
+    w->vert = (int64_t *)realloc(w->vert, sizeof(int64_t) * 2 * (size_t)nsb_of(w->m) * (size_t)cap);
     for (int64_t p = w->cap; p < cap; ++p)
         w->ops[p] = 0;
     w->cap = cap;
@@ -251,9 +271,7 @@ static void diagonal_update(so_walker *w)
         if (op == 0) {
             uint32_t b = (uint32_t)so_range(w->key, &w->ctr, (uint64_t)m->n_bonds);  /* :152 */
             const sse_vertex_data_desc *vd = bond_vd(m, b);
-            int s0 = m->bond_sites[2 * b], s1 = m->bond_sites[2 * b + 1];
-            int idx = (w->state[s0] - 1) + vd->dims[0] * (w->state[s1] - 1); /* join_idx */
-            int32_t dv = vd->diagonal_vertices[idx];
+            int32_t dv = vd->diagonal_vertices[bond_join(m, vd, b, w->state)]; /* :156-158 */
             double weight = dv < 0 ? 0.0 : vd->weights[dv >> 1];            /* :160-162 */
             double p_make = pm_raw / (double)(w->L - w->n);                   /* :164 */
             if (next_float(w->key, &w->ctr) < p_make * weight) {              /* :166 */
@@ -272,19 +290,24 @@ static void diagonal_update(so_walker *w)
                     w->n -= 1;
                 }
             } else {
-                const uint8_t *ls = vd->leg_states + 4 * (vc >> 1);           /* :183-187 */
-                w->state[m->bond_sites[2 * b]] = ls[2];
-                w->state[m->bond_sites[2 * b + 1]] = ls[3];
+                const int ns = nsb_of(m);                                      /* :183-187 */
+                const uint8_t *ls = vd->leg_states + 2 * ns * (vc >> 1);
+                const int32_t *bs = bond_sites_of(m, b);
+                for (int i = 0; i < ns; ++i)
+                    w->state[bs[i]] = ls[ns + i];
             }
         }
     }
 }
 
-/* make_vertex_list! (vertex_list.jl:15-54); legs 0,1 lower, 2,3 upper */
-void so_make_vertex_list(int32_t n_sites, const int32_t *bond_sites, const uint32_t *ops,
-                         int64_t L, int64_t *vert, int64_t *v_first, int64_t *v_last)
+/* make_vertex_list! (vertex_list.jl:15-54) for bonds of ns sites: legs
+ * 0..ns-1 lower, ns..2ns-1 upper, leg id 2ns*p + leg */
+void so_make_vertex_list(int32_t n_sites, int32_t ns, const int32_t *bond_sites,
+                         const uint32_t *ops, int64_t L, int64_t *vert, int64_t *v_first,
+                         int64_t *v_last)
 {
-    for (int64_t k = 0; k < 4 * L; ++k)
+    const int64_t legs = 2 * (int64_t)ns;
+    for (int64_t k = 0; k < legs * L; ++k)
         vert[k] = -1;
     for (int i = 0; i < n_sites; ++i)
         v_first[i] = v_last[i] = -1;
@@ -293,16 +316,16 @@ void so_make_vertex_list(int32_t n_sites, const int32_t *bond_sites, const uint3
         if (op == 0)
             continue;
         uint32_t b = op_bond(op);
-        for (int s = 0; s < 2; ++s) {
-            int site = bond_sites[2 * b + s];
+        for (int s = 0; s < ns; ++s) {
+            int site = bond_sites[(size_t)ns * b + s];
             int64_t last = v_last[site];
             if (last != -1) {
-                vert[last] = 4 * p + s;
-                vert[4 * p + s] = last;
+                vert[last] = legs * p + s;
+                vert[legs * p + s] = last;
             } else {
-                v_first[site] = 4 * p + s;
+                v_first[site] = legs * p + s;
             }
-            v_last[site] = 4 * p + 2 + s;
+            v_last[site] = legs * p + ns + s;
         }
     }
     for (int i = 0; i < n_sites; ++i) {
@@ -318,7 +341,7 @@ static int scatter(const sse_vertex_data_desc *vd, uint32_t vcode, int leg_in, i
                    double r, int *leg_out, int *worm_out, uint32_t *new_vc)
 {
     int vi = (int)(vcode >> 1);
-    int k = (vi * vd->max_worm_count + (worm_in - 1)) * 4 + leg_in;
+    int k = (vi * vd->max_worm_count + (worm_in - 1)) * 2 * vd->n_sites + leg_in;
     int32_t off = vd->transition_offset[k];
     int32_t cnt = vd->transition_count[k];
     if (off < 0)
@@ -340,16 +363,17 @@ static int64_t worm_traverse(so_walker *w, int64_t *scatters)
     const sse_model_desc *m = w->m;
     if (w->n == 0)
         return 0;
+    const int ns = nsb_of(m), legs = 2 * ns;
     int64_t p0;
     int l0;
     for (;;) { /* sse.jl:241-247 */
         p0 = (int64_t)so_range(w->key, &w->ctr, (uint64_t)w->L);
-        l0 = (int)so_range(w->key, &w->ctr, 4);
-        if (w->vert[4 * p0 + l0] >= 0)
+        l0 = (int)so_range(w->key, &w->ctr, (uint64_t)legs);
+        if (w->vert[legs * p0 + l0] >= 0)
             break;
     }
     uint32_t b0 = op_bond(w->ops[p0]);
-    int site0 = m->bond_sites[2 * b0 + (l0 & 1)];
+    int site0 = bond_sites_of(m, b0)[l0 % ns];
     int wf0 = 1 + (int)so_range(w->key, &w->ctr, (uint64_t)(m->site_dims[site0] - 1)); /* :251 */
 
     int leg_in = l0, wf = wf0;
@@ -368,14 +392,14 @@ static int64_t worm_traverse(so_walker *w, int64_t *scatters)
             return len;
         }
         w->ops[p] = op_make(b, nv);
-        int dim_out = m->site_dims[m->bond_sites[2 * b + (leg_out & 1)]];
+        int dim_out = m->site_dims[bond_sites_of(m, b)[leg_out % ns]];
         if (p == p0 && leg_out == l0 && wf_out == dim_out - wf0)
             break;
         len += 1;
         wf = wf_out;
-        int64_t nx = w->vert[4 * p + leg_out];
-        leg_in = (int)(nx & 3);
-        p = nx >> 2;
+        int64_t nx = w->vert[legs * p + leg_out];
+        leg_in = (int)(nx % legs);
+        p = nx / legs;
         if (p == p0 && leg_in == l0 && wf == wf0)
             break;
     }
@@ -407,14 +431,15 @@ static void worm_update(so_walker *w, int thermalized)
             w->num_worms = x > hi ? hi : (x < 1.0 ? 1.0 : x);
         }
     }
+    const int64_t legs = 2 * (int64_t)nsb_of(m);
     for (int i = 0; i < m->n_sites; ++i) {                                    /* :219-228 */
         int64_t f = w->v_first[i];
         if (f < 0) {
             w->state[i] = (uint8_t)(1 + so_range(w->key, &w->ctr, m->site_dims[i]));
         } else {
-            uint32_t op = w->ops[f >> 2];
+            uint32_t op = w->ops[f / legs];
             const sse_vertex_data_desc *vd = bond_vd(m, op_bond(op));
-            w->state[i] = vd->leg_states[4 * (op_vcode(op) >> 1) + (f & 3)];
+            w->state[i] = vd->leg_states[legs * (op_vcode(op) >> 1) + (f % legs)];
         }
     }
 }
@@ -443,8 +468,8 @@ int so_sweep_phases(so_walker *w, int phases)
         diagonal_update(w);
     if (phases & 2) {
         ensure_cap(w, w->L);
-        so_make_vertex_list(w->m->n_sites, w->m->bond_sites, w->ops, w->L, w->vert, w->v_first,
-                            w->v_last);
+        so_make_vertex_list(w->m->n_sites, nsb_of(w->m), w->m->bond_sites, w->ops, w->L, w->vert,
+                            w->v_first, w->v_last);
     }
     if (phases & 4)
         worm_update(w, thermalized);
@@ -501,11 +526,13 @@ void so_measure(so_walker *w, double *out)
                 continue;
             uint32_t b = op_bond(op), vc = op_vcode(op);
             if (!(vc & 1)) {
-                const uint8_t *ls = bond_vd(m, b)->leg_states + 4 * (vc >> 1);
-                for (int l = 0; l < 2; ++l) {
-                    int site = m->bond_sites[2 * b + l];
-                    tmp += coef[site] * (mv[site * D + ls[2 + l] - 1] - mv[site * D + ls[l] - 1]);
-                    st[site] = ls[2 + l];
+                const int ns = nsb_of(m);
+                const uint8_t *ls = bond_vd(m, b)->leg_states + 2 * ns * (vc >> 1);
+                const int32_t *bs = bond_sites_of(m, b);
+                for (int l = 0; l < ns; ++l) {
+                    int site = bs[l];
+                    tmp += coef[site] * (mv[site * D + ls[ns + l] - 1] - mv[site * D + ls[l] - 1]);
+                    st[site] = ls[ns + l];
                 }
             }
             mag += tmp;
@@ -572,7 +599,7 @@ int so_set_state(so_walker *w, const sse_walker_scalars *s, const uint32_t *ops,
 
 void so_get_vertex_list(const so_walker *w, int64_t *links, int64_t *first, int64_t *last)
 {
-    memcpy(links, w->vert, sizeof(int64_t) * 4 * (size_t)w->L);
+    memcpy(links, w->vert, sizeof(int64_t) * 2 * (size_t)nsb_of(w->m) * (size_t)w->L);
     memcpy(first, w->v_first, sizeof(int64_t) * (size_t)w->m->n_sites);
     memcpy(last, w->v_last, sizeof(int64_t) * (size_t)w->m->n_sites);
 }

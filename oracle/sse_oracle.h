@@ -41,8 +41,9 @@ void so_get_spins(const so_walker *w, uint8_t *spins);
 int so_set_state(so_walker *w, const sse_walker_scalars *s, const uint32_t *ops,
                  const uint8_t *spins);
 void so_get_vertex_list(const so_walker *w, int64_t *links, int64_t *first, int64_t *last);
-void so_make_vertex_list(int32_t n_sites, const int32_t *bond_sites, const uint32_t *ops,
-                         int64_t L, int64_t *links, int64_t *first, int64_t *last);
+void so_make_vertex_list(int32_t n_sites, int32_t bond_n_sites, const int32_t *bond_sites,
+                         const uint32_t *ops, int64_t L, int64_t *links, int64_t *first,
+                         int64_t *last);
 /* Run n sweeps (+ measure when thermalized) and return elapsed seconds;
  * accumulates measurements into sums (width doubles) when non-NULL.        */
 double so_run_timed(so_walker *w, int64_t n_sweeps, int measure, double *sums);

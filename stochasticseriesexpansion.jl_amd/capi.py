@@ -32,8 +32,12 @@ STATUS = {0: "SSE_OK", 1: "SSE_ERR_INVALID", 2: "SSE_ERR_HIP", 3: "SSE_ERR_OOM",
           4: "SSE_ERR_SCATTER", 5: "SSE_ERR_NO_DEVICE", 6: "SSE_ERR_STATE"}
 
 
+MAX_BOND_SITES = 4
+
+
 class VertexDataDesc(C.Structure):
-    _fields_ = [("dims", C.c_int32 * 2), ("energy_offset", C.c_double),
+    _fields_ = [("n_sites", C.c_int32), ("dims", C.c_int32 * MAX_BOND_SITES),
+                ("energy_offset", C.c_double),
                 ("n_vertices", C.c_int32), ("weights", C.POINTER(C.c_double)),
                 ("signs", C.POINTER(C.c_int8)), ("leg_states", C.POINTER(C.c_uint8)),
                 ("diagonal_vertices", C.POINTER(C.c_int32)), ("max_worm_count", C.c_int32),
@@ -45,7 +49,8 @@ class VertexDataDesc(C.Structure):
 
 class ModelDesc(C.Structure):
     _fields_ = [("n_sites", C.c_int32), ("site_dims", C.POINTER(C.c_uint8)),
-                ("n_bonds", C.c_int32), ("bond_type", C.POINTER(C.c_int32)),
+                ("n_bonds", C.c_int32), ("bond_n_sites", C.c_int32),
+                ("bond_type", C.POINTER(C.c_int32)),
                 ("bond_sites", C.POINTER(C.c_int32)), ("n_types", C.c_int32),
                 ("types", C.POINTER(VertexDataDesc)), ("energy_offset", C.c_double),
                 ("normalization_site_count", C.c_double), ("n_estimators", C.c_int32),
@@ -98,7 +103,9 @@ class ModelDescs:
         self.keep.append(types)
         for t, d in enumerate(fm.types):
             vd = types[t]
-            vd.dims[0], vd.dims[1] = d["dims"]
+            vd.n_sites = len(d["dims"])
+            for k, dk in enumerate(d["dims"]):
+                vd.dims[k] = dk
             vd.energy_offset = d["energy_offset"]
             vd.n_vertices = d["n_vertices"]
             vd.weights = _ptr(self._arr(d["weights"], np.float64), C.c_double)
@@ -117,6 +124,7 @@ class ModelDescs:
         m.n_sites = fm.n_sites
         m.site_dims = _ptr(self._arr(fm.site_dims, np.uint8), C.c_uint8)
         m.n_bonds = fm.n_bonds
+        m.bond_n_sites = fm.bond_n_sites
         m.bond_type = _ptr(self._arr(fm.bond_type, np.int32), C.c_int32)
         m.bond_sites = _ptr(self._arr(fm.bond_sites, np.int32), C.c_int32)
         m.n_types = len(fm.types)
@@ -416,14 +424,15 @@ class Batch:
     def vertex_list(self, w):
         sc, _, _ = self.get_walker(w, with_ops=False, with_spins=False)
         L = sc["cutoff"]
-        n_sites = self.flats[self.walkers[w].model].n_sites
-        links = np.zeros(max(1, 4 * L), dtype=np.int32)
+        flat = self.flats[self.walkers[w].model]
+        n_sites, legs = flat.n_sites, 2 * flat.bond_n_sites
+        links = np.zeros(max(1, legs * L), dtype=np.int32)
         first = np.zeros(n_sites, dtype=np.int32)
         last = np.zeros(n_sites, dtype=np.int32)
         check(self.lib, self.lib.sse_get_vertex_list(self.handle, w, _ptr(links, C.c_int32),
                                                      len(links), _ptr(first, C.c_int32),
                                                      _ptr(last, C.c_int32)))
-        return links[:4 * L], first, last
+        return links[:legs * L], first, last
 
     # ---- serving slots
     def add_model(self, flat):

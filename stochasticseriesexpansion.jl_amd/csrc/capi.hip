@@ -209,6 +209,11 @@ static int validate_model(const sse_model_desc &m, int k)
         snprintf(buf, sizeof buf, "model %d: empty graph or missing arrays", k);
         return fail(SSE_ERR_INVALID, buf);
     }
+    const int ns = m.bond_n_sites;
+    if (ns < 1 || ns > SSE_MAX_BOND_SITES) {
+        snprintf(buf, sizeof buf, "model %d: bond_n_sites must be 1..%d", k, SSE_MAX_BOND_SITES);
+        return fail(SSE_ERR_INVALID, buf);
+    }
     if (m.n_sites >= (1 << 20) || m.n_bonds >= (1 << 21) || m.n_types >= (1 << 11)) {
         snprintf(buf, sizeof buf, "model %d: too many sites/bonds/types", k);
         return fail(SSE_ERR_INVALID, buf);
@@ -218,35 +223,53 @@ static int validate_model(const sse_model_desc &m, int k)
             snprintf(buf, sizeof buf, "model %d: site %d has dimension 0 (not on any bond)", k, i);
             return fail(SSE_ERR_INVALID, buf);
         }
+    for (int t = 0; t < m.n_types; ++t) {
+        const sse_vertex_data_desc &vd = m.types[t];
+        bool ok = vd.n_sites == ns && vd.n_vertices > 0 && vd.n_vertices <= 512 &&
+                  vd.max_worm_count >= 1 && vd.max_worm_count <= 4096;
+        int64_t states = 1;
+        for (int i = 0; ok && i < ns; ++i) {
+            ok = vd.dims[i] >= 1 && vd.dims[i] <= 255;
+            states *= vd.dims[i];
+        }
+        if (!ok || states > (1 << 20)) {
+            snprintf(buf, sizeof buf, "model %d type %d: unsupported table shape", k, t);
+            return fail(SSE_ERR_INVALID, buf);
+        }
+    }
     for (int b = 0; b < m.n_bonds; ++b) {
-        int t = m.bond_type[b], s0 = m.bond_sites[2 * b], s1 = m.bond_sites[2 * b + 1];
-        if (t < 0 || t >= m.n_types || s0 < 0 || s0 >= m.n_sites || s1 < 0 || s1 >= m.n_sites ||
-            s0 == s1) {
+        const int t = m.bond_type[b];
+        bool ok = t >= 0 && t < m.n_types;
+        for (int i = 0; ok && i < ns; ++i) {
+            const int si = m.bond_sites[ns * b + i];
+            ok = si >= 0 && si < m.n_sites;
+            for (int j = 0; ok && j < i; ++j)
+                ok = m.bond_sites[ns * b + j] != si;
+        }
+        if (!ok) {
             snprintf(buf, sizeof buf, "model %d: bond %d invalid", k, b);
             return fail(SSE_ERR_INVALID, buf);
         }
         const sse_vertex_data_desc &vd = m.types[t];
-        if (vd.dims[0] != m.site_dims[s0] || vd.dims[1] != m.site_dims[s1]) {
-            snprintf(buf, sizeof buf, "model %d: bond %d site dimensions inconsistent", k, b);
-            return fail(SSE_ERR_INVALID, buf);
-        }
+        for (int i = 0; i < ns; ++i)
+            if (vd.dims[i] != m.site_dims[m.bond_sites[ns * b + i]]) {
+                snprintf(buf, sizeof buf, "model %d: bond %d site dimensions inconsistent", k, b);
+                return fail(SSE_ERR_INVALID, buf);
+            }
     }
     for (int t = 0; t < m.n_types; ++t) {
         const sse_vertex_data_desc &vd = m.types[t];
-        if (vd.n_vertices <= 0 || vd.n_vertices > 512 || vd.dims[0] < 1 || vd.dims[1] < 1 ||
-            vd.dims[0] > 255 || vd.dims[1] > 255 || vd.max_worm_count < 1 ||
-            vd.max_worm_count > 4096) {
-            snprintf(buf, sizeof buf, "model %d type %d: unsupported table shape", k, t);
-            return fail(SSE_ERR_INVALID, buf);
-        }
-        for (int i = 0; i < vd.dims[0] * vd.dims[1]; ++i) {
+        int64_t states = 1;
+        for (int i = 0; i < ns; ++i)
+            states *= vd.dims[i];
+        for (int64_t i = 0; i < states; ++i) {
             int dv = vd.diagonal_vertices[i];
             if (dv >= 0 && ((dv >> 1) >= vd.n_vertices || !(dv & 1))) {
                 snprintf(buf, sizeof buf, "model %d type %d: bad diagonal vertex", k, t);
                 return fail(SSE_ERR_INVALID, buf);
             }
         }
-        int nt = vd.n_vertices * vd.max_worm_count * 4;
+        int nt = vd.n_vertices * vd.max_worm_count * 2 * ns;
         for (int i = 0; i < nt; ++i) {
             int off = vd.transition_offset[i], cnt = vd.transition_count[i];
             if (off >= 0 && (cnt < 0 || off + cnt > vd.n_exits)) {
@@ -255,7 +278,7 @@ static int validate_model(const sse_model_desc &m, int k)
             }
         }
         for (int e = 0; e < vd.n_exits; ++e) {
-            if (vd.exit_leg[e] > 3 || (vd.exit_target[e] >> 1) >= vd.n_vertices ||
+            if (vd.exit_leg[e] >= 2 * ns || (vd.exit_target[e] >> 1) >= vd.n_vertices ||
                 vd.exit_target[e] < 0 || vd.exit_worm[e] < 1 || vd.exit_worm[e] > vd.max_worm_count) {
                 snprintf(buf, sizeof buf, "model %d type %d: exit %d out of range", k, t, e);
                 return fail(SSE_ERR_INVALID, buf);
@@ -289,16 +312,20 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
         size_t w, s, ls, dv, to, tc, cum, tgt, el, ew;
     };
     std::vector<VOff> voff(m.n_types);
+    const int ns = m.bond_n_sites, legs = 2 * ns;
     int max_dim = 1;
     for (int i = 0; i < m.n_sites; ++i)
         max_dim = m.site_dims[i] > max_dim ? m.site_dims[i] : max_dim;
     for (int t = 0; t < m.n_types; ++t) {
         const sse_vertex_data_desc &vd = m.types[t];
-        size_t nt = (size_t)vd.n_vertices * vd.max_worm_count * 4;
+        size_t nt = (size_t)vd.n_vertices * vd.max_worm_count * legs;
+        size_t states = 1;
+        for (int i = 0; i < ns; ++i)
+            states *= (size_t)vd.dims[i];
         voff[t].w = put(vd.weights, sizeof(double) * vd.n_vertices);
         voff[t].s = put(vd.signs, vd.n_vertices);
-        voff[t].ls = put(vd.leg_states, 4 * (size_t)vd.n_vertices);
-        voff[t].dv = put(vd.diagonal_vertices, sizeof(int32_t) * vd.dims[0] * vd.dims[1]);
+        voff[t].ls = put(vd.leg_states, (size_t)legs * vd.n_vertices);
+        voff[t].dv = put(vd.diagonal_vertices, sizeof(int32_t) * states);
         voff[t].to = put(vd.transition_offset, sizeof(int32_t) * nt);
         voff[t].tc = put(vd.transition_count, sizeof(int32_t) * nt);
         voff[t].cum = put(vd.exit_cumprob, sizeof(double) * vd.n_exits);
@@ -306,14 +333,16 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
         voff[t].el = put(vd.exit_leg, vd.n_exits);
         voff[t].ew = put(vd.exit_worm, vd.n_exits);
     }
+    // packed bond records: the first two sites and the type (one site:
+    // site1 = -1; wider bonds read bond_sites)
     std::vector<int2> brec(m.n_bonds);
     for (int k = 0; k < m.n_bonds; ++k) {
-        brec[k].x = m.bond_sites[2 * k] | (m.bond_type[k] << 20);
-        brec[k].y = m.bond_sites[2 * k + 1];
+        brec[k].x = m.bond_sites[ns * k] | (m.bond_type[k] << 20);
+        brec[k].y = ns >= 2 ? m.bond_sites[ns * k + 1] : -1;
     }
     size_t o_sd = put(m.site_dims, m.n_sites);
     size_t o_bt = put(m.bond_type, sizeof(int32_t) * m.n_bonds);
-    size_t o_bs = put(m.bond_sites, sizeof(int32_t) * 2 * m.n_bonds);
+    size_t o_bs = put(m.bond_sites, sizeof(int32_t) * ns * m.n_bonds);
     size_t o_br = put(brec.data(), sizeof(int2) * m.n_bonds);
     size_t ne = (size_t)m.n_estimators;
     size_t o_ec = put(m.est_coef, sizeof(double) * ne * m.n_sites);
@@ -328,7 +357,7 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
         hdr.push_back((uint32_t)trans.size());
         hdr.push_back((uint32_t)vd.n_vertices);
         hdr.push_back((uint32_t)vd.max_worm_count);
-        int nt = vd.n_vertices * vd.max_worm_count * 4;
+        int nt = vd.n_vertices * vd.max_worm_count * legs;
         for (int i = 0; i < nt; ++i) {
             int off = vd.transition_offset[i], cnt = vd.transition_count[i];
             if (off < 0 || cnt <= 0)
@@ -373,7 +402,9 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
     while ((1 << lgp) < 2 * dnv)
         ++lgp;
     const size_t P = (size_t)1 << lgp;
-    std::vector<DirectEntry> dt((size_t)m.n_types * P * dmw * 4);
+    // (the wave kernels' tables; built for two-site bonds only, the bonds
+    // the wave kernels take)
+    std::vector<DirectEntry> dt(ns == 2 ? (size_t)m.n_types * P * dmw * 4 : 1);
     for (auto &x : dt) {  // invalid transition: every word takes the exact path, which fails
         for (int j = 0; j < 4; ++j) {
             x.K[j] = 0;
@@ -383,7 +414,7 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
         x.trans = 0xFFFFFFFFu;
         x.pad[0] = x.pad[1] = 0;
     }
-    {
+    if (ns == 2) {
         int ebase = 0;
         for (int t = 0; t < m.n_types; ++t) {
             const sse_vertex_data_desc &vd = m.types[t];
@@ -431,7 +462,7 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
     size_t o_dtab = put(dt.data(), sizeof(DirectEntry) * dt.size());
     // diagonal-update table (layout: DevModel::k1tab)
     std::vector<char> k1((size_t)16 * m.n_types);
-    for (int t = 0; t < m.n_types; ++t) {
+    for (int t = 0; t < m.n_types && ns == 2; ++t) {
         const sse_vertex_data_desc &vd = m.types[t];
         auto align8 = [&]() { k1.resize((k1.size() + 7) & ~(size_t)7); };
         align8();
@@ -464,11 +495,13 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
         const sse_vertex_data_desc &vd = m.types[t];
         DevVD &x = types[t];
         x.dims0 = vd.dims[0];
-        x.dims1 = vd.dims[1];
+        x.dims1 = ns >= 2 ? vd.dims[1] : 1;
+        x.nsites = ns;
+        for (int i = 0; i < 4; ++i)
+            x.dims[i] = i < ns ? vd.dims[i] : 1;
         x.nv = vd.n_vertices;
         x.max_worm = vd.max_worm_count;
         x.n_exits = vd.n_exits;
-        x.pad = 0;
         x.energy_offset = vd.energy_offset;
         x.weights = (const double *)(d + voff[t].w);
         x.signs = (const int8_t *)(d + voff[t].s);
@@ -485,6 +518,8 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
     HIPCHECK(bmemcpy(b, d, blob.data(), blob.size(), hipMemcpyHostToDevice));
     dm.n_sites = m.n_sites;
     dm.n_bonds = m.n_bonds;
+    dm.nsb = ns;
+    dm.pad_nsb = 0;
     dm.n_types = m.n_types;
     dm.n_est = m.n_estimators;
     dm.est_max_dim = m.n_estimators ? m.est_max_dim : 1;
@@ -523,9 +558,9 @@ static size_t env_size(const char *name, size_t dflt)
 }
 
 // host-side bump allocation of a slab (only while no kernel runs)
-static char *host_slab(sse_batch *b, int64_t cap)
+static char *host_slab(sse_batch *b, int64_t cap, int legs)
 {
-    size_t bytes = slab_bytes(cap);
+    size_t bytes = slab_bytes(cap, legs);
     if (b->pool_host_next + bytes > b->pool_size)
         return nullptr;
     char *p = b->pool_base + b->pool_host_next;
@@ -684,6 +719,9 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
         int rc = validate_model(models[k], k);
         if (rc)
             return rc;
+        if (mode == SSE_MODE_WAVE && models[k].bond_n_sites != 2)
+            return fail(SSE_ERR_INVALID, "sse_batch_create: the wave kernels take two-site bonds; "
+                                         "run models with other bonds in SSE_MODE_LANE");
     }
     for (int w = 0; w < n_walkers; ++w) {
         const sse_walker_desc &wd = walkers[w];
@@ -732,7 +770,7 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
         L0[w] = wd.init_opstring_cutoff >= 0 ? wd.init_opstring_cutoff
                                              : (int64_t)std::nearbyint((double)m.n_sites * wd.T);
         cap0[w] = slab_capacity(L0[w]);
-        need += slab_bytes(cap0[w]);
+        need += slab_bytes(cap0[w], 2 * m.bond_n_sites);
         b->widths[w] = 5 + 5 * m.n_estimators;
         b->offsets[w] = off;
         b->nsites[w] = m.n_sites;
@@ -806,7 +844,7 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
             x.ctr = 0;
             x.n_sites = b->nsites[w];
             x.last_wlf = NAN;
-            char *s = host_slab(b, cap0[w]);
+            char *s = host_slab(b, cap0[w], 2 * b->h_models[wd.model].nsb);
             if (!s) {
                 rc = fail(SSE_ERR_OOM, "slab pool too small");
                 break;
@@ -1980,7 +2018,7 @@ int sse_set_walker(sse_batch *b, int32_t walker, const sse_walker_scalars *s, co
         return rc;
     if (s->cutoff > w.cap) {
         int64_t cap = slab_capacity(s->cutoff);
-        char *slab = host_slab(b, cap);
+        char *slab = host_slab(b, cap, 2 * m.nsb);
         if (!slab)
             return fail(SSE_ERR_OOM, "slab pool exhausted");
         w.ops = (uint32_t *)slab;
@@ -2017,8 +2055,9 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links, int64_t li
     DevWalker w;
     HIPCHECK(bmemcpy(b, &w, b->d_walkers + walker, sizeof w, hipMemcpyDeviceToHost));
     if (b->mode == SSE_MODE_LANE) {
-        if (links && links_capacity >= 4 * w.L && w.L > 0)
-            HIPCHECK(bmemcpy(b, links, w.links, sizeof(int32_t) * 4 * w.L, hipMemcpyDeviceToHost));
+        const int64_t legs = 2 * b->h_models[w.model].nsb;
+        if (links && links_capacity >= legs * w.L && w.L > 0)
+            HIPCHECK(bmemcpy(b, links, w.links, sizeof(int32_t) * legs * w.L, hipMemcpyDeviceToHost));
     }
     std::vector<int64_t> pos;  // wave path: operator rank -> slot
     if (b->mode != SSE_MODE_LANE) {
@@ -2086,7 +2125,7 @@ static int compact_pool(sse_batch *b)
     for (auto &w : h) {
         bool live = w.active || w.pending || w.remaining != 0;
         int64_t cap = live ? w.cap : 64;
-        size_t bytes = slab_bytes(cap);
+        size_t bytes = slab_bytes(cap, 2 * b->h_models[w.model].nsb);
         if (next + bytes > b->pool_size) {
             (void)hipFree(np);
             return fail(SSE_ERR_OOM, "pool compaction: live slabs exceed the pool");
@@ -2124,6 +2163,8 @@ int sse_add_model(sse_batch *b, const sse_model_desc *model, int32_t *index)
     int rc = validate_model(*model, b->n_models);
     if (rc)
         return rc;
+    if (b->mode == SSE_MODE_WAVE && model->bond_n_sites != 2)
+        return fail(SSE_ERR_INVALID, "sse_add_model: the wave kernels take two-site bonds");
     if (model->n_sites > b->max_sites)
         return fail(SSE_ERR_INVALID, "sse_add_model: more sites than the batch was created for");
     HIPCHECK(hipStreamSynchronize(b->stream));
@@ -2176,7 +2217,7 @@ int sse_slot_start(sse_batch *b, int32_t slot, const sse_walker_desc *wd, int64_
                                                : (int64_t)std::nearbyint((double)m.n_sites * wd->T);
     int64_t cap = slab_capacity(L0);
     if (cap > x.cap || !x.ops) {
-        char *s = host_slab(b, cap);
+        char *s = host_slab(b, cap, 2 * m.nsb);
         if (!s) {
             x.active = 0;
             x.pending = 0;
@@ -2185,7 +2226,7 @@ int sse_slot_start(sse_batch *b, int32_t slot, const sse_walker_desc *wd, int64_
             rc = compact_pool(b);
             if (rc)
                 return rc;
-            s = host_slab(b, cap);
+            s = host_slab(b, cap, 2 * m.nsb);
             if (!s)
                 return fail(SSE_ERR_OOM, "slab pool full; raise SSE_POOL_MB");
         }

@@ -4,8 +4,9 @@
 //   DevModel[n_models]      model graph + per-bond-type VertexData tables
 //   DevWalker[n_walkers]    per-walker scalars and slab pointers
 //   slab pool               per walker one slab: ops[cap] (u32), then a record
-//                           region used as dense links[4*cap] (i32) by the lane
-//                           kernels, or by the wave kernels as compact records
+//                           region used as dense links[legs*cap] (i32, legs =
+//                           2 * sites per bond) by the lane kernels, or by the
+//                           wave kernels (two-site bonds) as compact records
 //                           indexed by operator rank r (r-th non-identity slot):
 //                             vt[cap]      vcode | bond type << 10
 //                             link[4*cap]  linked leg id 4r'+leg'
@@ -25,14 +26,15 @@
 #define SSE_WAVE 64
 
 struct DevVD {
-    int32_t dims0, dims1, nv, max_worm, n_exits, pad;
+    int32_t dims0, dims1, nv, max_worm, n_exits, nsites;
+    int32_t dims[4];           // local dimension of each of the nsites bond sites
     double energy_offset;
     const double *weights;     // [nv]
     const int8_t *signs;       // [nv]
-    const uint8_t *leg_states; // [nv][4]
-    const int32_t *diag_vertex;// [dims0*dims1]
-    const int32_t *toff;       // [nv][max_worm][4]
-    const int32_t *tcnt;       // [nv][max_worm][4]
+    const uint8_t *leg_states; // [nv][2*nsites]
+    const int32_t *diag_vertex;// [prod(dims)]
+    const int32_t *toff;       // [nv][max_worm][2*nsites]
+    const int32_t *tcnt;       // [nv][max_worm][2*nsites]
     const double *cum;         // [n_exits]
     const int32_t *tgt;        // [n_exits]
     const uint8_t *eleg;       // [n_exits]
@@ -41,10 +43,11 @@ struct DevVD {
 
 struct DevModel {
     int32_t n_sites, n_bonds, n_types, n_est, est_max_dim, max_dim;
+    int32_t nsb, pad_nsb;      // sites per bond (all bonds alike), leg count 2*nsb
     double energy_offset, norm;
     const uint8_t *site_dims;  // [n_sites]
     const int32_t *bond_type;  // [n_bonds]
-    const int32_t *bond_sites; // [n_bonds][2]
+    const int32_t *bond_sites; // [n_bonds][nsb]
     const DevVD *types;        // [n_types]
     const double *est_coef;    // [n_est][n_sites]
     const double *est_mval;    // [n_est][n_sites][est_max_dim]
@@ -151,7 +154,7 @@ struct DevWalker {
     int32_t error, n_sites;
     // buffers
     uint32_t *ops;    // [cap]
-    int32_t *links;   // [4*cap]
+    int32_t *links;   // record region (dense links [legs*cap] or compact records)
     uint8_t *spins;   // [n_sites]
     int32_t *first;   // [n_sites]
     int32_t *last;    // [n_sites]
@@ -395,9 +398,15 @@ __device__ __host__ __forceinline__ int64_t slab_capacity(int64_t L)
     return (c + 63) & ~(int64_t)63;
 }
 
-__device__ __host__ __forceinline__ uint64_t slab_bytes(int64_t cap)
+// Slab of a walker with leg_count legs per operator: ops[cap], then a record
+// region that holds either the compact records of the wave kernels (20 bytes
+// per slot plus the chunk masks and bases) or the dense links of the lane
+// kernels (4 * legs bytes per slot), whichever is larger.
+__device__ __host__ __forceinline__ uint64_t slab_bytes(int64_t cap, int legs = 4)
 {
-    uint64_t b = (uint64_t)cap * 4u + (uint64_t)cap * 20u + ((uint64_t)cap / 64u + 1u) * 16u;
+    const uint64_t wave = (uint64_t)cap * 20u + ((uint64_t)cap / 64u + 1u) * 16u;
+    const uint64_t lane = (uint64_t)cap * 4u * (uint64_t)legs;
+    uint64_t b = (uint64_t)cap * 4u + (wave > lane ? wave : lane);
     return (b + 255u) & ~(uint64_t)255u;
 }
 

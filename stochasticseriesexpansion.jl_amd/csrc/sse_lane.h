@@ -7,15 +7,19 @@
 #include "sse_common.h"
 #include "../../include/ssegpu.h"
 
+// Bonds act on nsb = m.nsb sites (SSEData{NSites}); an operator has
+// legs = 2 * nsb legs, leg ids are legs * p + leg, legs 0..nsb-1 lower and
+// nsb..2nsb-1 upper (site_of_leg(l) = l mod nsb, vertex_data.jl:190).
+
 // Grow the cutoff if n >= L/2, reallocating the slab from the pool when the
 // capacity is exceeded.  Returns false (walker untouched) on pool exhaustion.
-__device__ bool lane_grow(DevWalker &w, DevPool *pool)
+__device__ bool lane_grow(DevWalker &w, DevPool *pool, int legs)
 {
     if ((double)w.n >= (double)w.L * 0.5) {
         int64_t nl = grown_cutoff(w.L);
         if (nl > w.cap) {
             int64_t cap = slab_capacity(nl);
-            char *s = pool_alloc(pool, slab_bytes(cap));
+            char *s = pool_alloc(pool, slab_bytes(cap, legs));
             if (!s) {
                 w.error = SSE_ERR_OOM;
                 return false;
@@ -34,10 +38,24 @@ __device__ bool lane_grow(DevWalker &w, DevPool *pool)
     return true;
 }
 
+// compound state index of the bond's sites, the first site fastest (join_idx)
+__device__ __forceinline__ int lane_join(const DevModel &m, const DevVD &vd, uint32_t b,
+                                         const uint8_t *spins)
+{
+    const int32_t *bs = m.bond_sites + (size_t)m.nsb * b;
+    int idx = 0, stride = 1;
+    for (int i = 0; i < m.nsb; ++i) {
+        idx += (spins[bs[i]] - 1) * stride;
+        stride *= vd.dims[i];
+    }
+    return idx;
+}
+
 __device__ void lane_diagonal(DevWalker &w, const DevModel &m, LaneRng &rng)
 {
     const double pm_raw = (double)m.n_bonds / w.T;
     const double pr_raw = w.T / (double)m.n_bonds;
+    const int nsb = m.nsb;
     uint32_t *ops = w.ops;
     uint8_t *spins = w.spins;
     int64_t n = w.n;
@@ -46,11 +64,8 @@ __device__ void lane_diagonal(DevWalker &w, const DevModel &m, LaneRng &rng)
         uint32_t op = ops[p];
         if (op == 0u) {
             uint32_t b = (uint32_t)rng.range((uint64_t)m.n_bonds);
-            int2 br = m.bond_rec[b];
-            const DevVD &vd = m.types[br.x >> 20];
-            int s0 = br.x & 0xFFFFF, s1 = br.y;
-            int idx = (spins[s0] - 1) + vd.dims0 * (spins[s1] - 1);
-            int32_t dv = vd.diag_vertex[idx];
+            const DevVD &vd = m.types[m.bond_type[b]];
+            int32_t dv = vd.diag_vertex[lane_join(m, vd, b, spins)];
             double weight = dv < 0 ? 0.0 : vd.weights[dv >> 1];
             double p_make = pm_raw / (double)(L - n);
             if (rng.next_float() < p_make * weight) {
@@ -59,8 +74,7 @@ __device__ void lane_diagonal(DevWalker &w, const DevModel &m, LaneRng &rng)
             }
         } else {
             uint32_t b = op_bond(op);
-            int2 br = m.bond_rec[b];
-            const DevVD &vd = m.types[br.x >> 20];
+            const DevVD &vd = m.types[m.bond_type[b]];
             uint32_t vc = op_vcode(op);
             if (vc & 1u) {
                 double weight = vd.weights[vc >> 1];
@@ -70,9 +84,10 @@ __device__ void lane_diagonal(DevWalker &w, const DevModel &m, LaneRng &rng)
                     n -= 1;
                 }
             } else {
-                const uint8_t *ls = vd.leg_states + 4 * (vc >> 1);
-                spins[br.x & 0xFFFFF] = ls[2];
-                spins[br.y] = ls[3];
+                const uint8_t *ls = vd.leg_states + 2 * nsb * (vc >> 1);
+                const int32_t *bs = m.bond_sites + (size_t)nsb * b;
+                for (int i = 0; i < nsb; ++i)
+                    spins[bs[i]] = ls[nsb + i];
             }
         }
     }
@@ -81,9 +96,10 @@ __device__ void lane_diagonal(DevWalker &w, const DevModel &m, LaneRng &rng)
 
 __device__ void lane_vertex_list(DevWalker &w, const DevModel &m)
 {
+    const int nsb = m.nsb, legs = 2 * nsb;
     int32_t *links = w.links;
     const uint32_t *ops = w.ops;
-    for (int64_t k = 0; k < 4 * w.L; ++k)
+    for (int64_t k = 0; k < legs * w.L; ++k)
         links[k] = -1;
     for (int i = 0; i < m.n_sites; ++i)
         w.first[i] = w.last[i] = -1;
@@ -91,19 +107,18 @@ __device__ void lane_vertex_list(DevWalker &w, const DevModel &m)
         uint32_t op = ops[p];
         if (op == 0u)
             continue;
-        int2 br = m.bond_rec[op_bond(op)];
-        int sites[2] = {br.x & 0xFFFFF, br.y};
-        for (int s = 0; s < 2; ++s) {
-            int site = sites[s];
+        const int32_t *bs = m.bond_sites + (size_t)nsb * op_bond(op);
+        for (int s = 0; s < nsb; ++s) {
+            int site = bs[s];
             int32_t lst = w.last[site];
-            int32_t me = (int32_t)(4 * p + s);
+            int32_t me = (int32_t)(legs * p + s);
             if (lst != -1) {
                 links[lst] = me;
                 links[me] = lst;
             } else {
                 w.first[site] = me;
             }
-            w.last[site] = (int32_t)(4 * p + 2 + s);
+            w.last[site] = (int32_t)(legs * p + nsb + s);
         }
     }
     for (int i = 0; i < m.n_sites; ++i) {
@@ -116,10 +131,11 @@ __device__ void lane_vertex_list(DevWalker &w, const DevModel &m)
 }
 
 // scatter (vertex_data.jl:106-125); returns false on fall-through
-__device__ __forceinline__ bool lane_scatter(const DevVD &vd, uint32_t vcode, int leg_in, int worm_in,
-                                             double r, int &leg_out, int &worm_out, uint32_t &nv)
+__device__ __forceinline__ bool lane_scatter(const DevVD &vd, int legs, uint32_t vcode, int leg_in,
+                                             int worm_in, double r, int &leg_out, int &worm_out,
+                                             uint32_t &nv)
 {
-    int k = ((int)(vcode >> 1) * vd.max_worm + (worm_in - 1)) * 4 + leg_in;
+    int k = ((int)(vcode >> 1) * vd.max_worm + (worm_in - 1)) * legs + leg_in;
     int32_t off = vd.toff[k];
     int32_t cnt = vd.tcnt[k];
     if (off < 0)
@@ -139,42 +155,41 @@ __device__ int64_t lane_worm_traverse(DevWalker &w, const DevModel &m, LaneRng &
 {
     if (w.n == 0)
         return 0;
+    const int nsb = m.nsb, legs = 2 * nsb;
     int64_t p0;
     int l0;
     for (;;) {
         p0 = (int64_t)rng.range((uint64_t)w.L);
-        l0 = (int)rng.range(4);
-        if (w.links[4 * p0 + l0] >= 0)
+        l0 = (int)rng.range((uint64_t)legs);
+        if (w.links[legs * p0 + l0] >= 0)
             break;
     }
-    int2 b0 = m.bond_rec[op_bond(w.ops[p0])];
-    int site0 = (l0 & 1) ? b0.y : (b0.x & 0xFFFFF);
+    int site0 = m.bond_sites[(size_t)nsb * op_bond(w.ops[p0]) + l0 % nsb];
     int wf0 = 1 + (int)rng.range((uint64_t)(m.site_dims[site0] - 1));
     int leg_in = l0, wf = wf0;
     int64_t p = p0, len = 1;
     for (;;) {
         uint32_t op = w.ops[p];
         uint32_t b = op_bond(op);
-        int2 br = m.bond_rec[b];
-        const DevVD &vd = m.types[br.x >> 20];
+        const DevVD &vd = m.types[m.bond_type[b]];
         int leg_out, wf_out;
         uint32_t nv;
         double r = rng.next_float();
         scat += 1;
-        if (!lane_scatter(vd, op_vcode(op), leg_in, wf, r, leg_out, wf_out, nv)) {
+        if (!lane_scatter(vd, legs, op_vcode(op), leg_in, wf, r, leg_out, wf_out, nv)) {
             w.error = SSE_ERR_SCATTER;
             return len;
         }
         w.ops[p] = op_make(b, nv);
-        int site_out = (leg_out & 1) ? br.y : (br.x & 0xFFFFF);
+        int site_out = m.bond_sites[(size_t)nsb * b + leg_out % nsb];
         int dim_out = m.site_dims[site_out];
         if (p == p0 && leg_out == l0 && wf_out == dim_out - wf0)
             break;
         len += 1;
         wf = wf_out;
-        int32_t nx = w.links[4 * p + leg_out];
-        leg_in = nx & 3;
-        p = nx >> 2;
+        int32_t nx = w.links[legs * p + leg_out];
+        leg_in = nx % legs;
+        p = nx / legs;
         if (p == p0 && leg_in == l0 && wf == wf0)
             break;
     }
@@ -183,6 +198,7 @@ __device__ int64_t lane_worm_traverse(DevWalker &w, const DevModel &m, LaneRng &
 
 __device__ void lane_worm_update(DevWalker &w, const DevModel &m, LaneRng &rng, bool therm)
 {
+    const int legs = 2 * m.nsb;
     double total = 1.0;
     int64_t nworms = (int64_t)ceil(w.num_worms);
     int64_t scat = 0;
@@ -195,9 +211,9 @@ __device__ void lane_worm_update(DevWalker &w, const DevModel &m, LaneRng &rng, 
         if (f < 0) {
             w.spins[i] = (uint8_t)(1 + rng.range(m.site_dims[i]));
         } else {
-            uint32_t op = w.ops[f >> 2];
-            const DevVD &vd = m.types[m.bond_rec[op_bond(op)].x >> 20];
-            w.spins[i] = vd.leg_states[4 * (op_vcode(op) >> 1) + (f & 3)];
+            uint32_t op = w.ops[f / legs];
+            const DevVD &vd = m.types[m.bond_type[op_bond(op)]];
+            w.spins[i] = vd.leg_states[legs * (op_vcode(op) >> 1) + (f % legs)];
         }
     }
 }
@@ -220,7 +236,7 @@ __global__ void lane_sweep_kernel(DevWalker *W, const DevModel *models, DevPool 
     LaneRng rng;
     rng.init(w.key, w.ctr);
     if (phases & 1) {
-        if (!lane_grow(w, pool)) {
+        if (!lane_grow(w, pool, 2 * m.nsb)) {
             W[wi].error = w.error;
             flag_error(pool, w.error);
             return;
@@ -265,7 +281,7 @@ __global__ void lane_init_kernel(DevWalker *W, const DevModel *models, DevPool *
         w.ops[p] = 0u;
     w.n = 0;
     for (int k = 0; k < w.warmup; ++k) {
-        if (!lane_grow(w, pool)) {
+        if (!lane_grow(w, pool, 2 * m.nsb)) {
             flag_error(pool, w.error);
             break;
         }
@@ -280,11 +296,12 @@ __global__ void lane_init_kernel(DevWalker *W, const DevModel *models, DevPool *
 // Carlo.measure! for one walker, sequentially (sse.jl:70-87,305-376).
 __device__ void measure_walker_serial(const DevWalker &w, const DevModel &m, double *out, uint8_t *st)
 {
+    const int nsb = m.nsb;
     int neg = 0;
     for (int64_t p = 0; p < w.L; ++p) {
         uint32_t op = w.ops[p];
         if (op != 0u)
-            neg += m.types[m.bond_rec[op_bond(op)].x >> 20].signs[op_vcode(op) >> 1] < 0;
+            neg += m.types[m.bond_type[op_bond(op)]].signs[op_vcode(op) >> 1] < 0;
     }
     double sign = (neg & 1) ? -1.0 : 1.0;
     double nn = (double)w.n;
@@ -310,13 +327,13 @@ __device__ void measure_walker_serial(const DevWalker &w, const DevModel &m, dou
                 continue;
             uint32_t vc = op_vcode(op);
             if (!(vc & 1u)) {
-                int2 br = m.bond_rec[op_bond(op)];
-                const uint8_t *ls = m.types[br.x >> 20].leg_states + 4 * (vc >> 1);
-                int sites[2] = {br.x & 0xFFFFF, br.y};
-                for (int l = 0; l < 2; ++l) {
-                    int site = sites[l];
-                    tmp += coef[site] * (mv[site * D + ls[2 + l] - 1] - mv[site * D + ls[l] - 1]);
-                    st[site] = ls[2 + l];
+                const uint32_t b = op_bond(op);
+                const uint8_t *ls = m.types[m.bond_type[b]].leg_states + 2 * nsb * (vc >> 1);
+                const int32_t *bs = m.bond_sites + (size_t)nsb * b;
+                for (int l = 0; l < nsb; ++l) {
+                    int site = bs[l];
+                    tmp += coef[site] * (mv[site * D + ls[nsb + l] - 1] - mv[site * D + ls[l] - 1]);
+                    st[site] = ls[nsb + l];
                 }
             }
             mag += tmp;

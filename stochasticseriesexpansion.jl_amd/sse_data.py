@@ -58,15 +58,17 @@ class SSEData:
 
 
 def _vd_arrays(vd):
-    """Flatten one VertexData into the device table layout."""
+    """Flatten one VertexData into the device table layout (legs
+    2 * NSites per vertex)."""
     nv = vd.vertex_count()
     mw = vd.max_worm_count
+    legs = 2 * len(vd.dims)
     diag = np.full(int(np.prod(vd.dims)), -1, dtype=np.int32)
     for k, v in enumerate(vd.diagonal_vertices):
         if v is not None:
             diag[k] = (v & 1) | (((v >> 1) - 1) << 1)
-    toff = np.full((nv, mw, 4), -1, dtype=np.int32)
-    tcnt = np.zeros((nv, mw, 4), dtype=np.int32)
+    toff = np.full((nv, mw, legs), -1, dtype=np.int32)
+    tcnt = np.zeros((nv, mw, legs), dtype=np.int32)
     for (leg, worm, v), (offset, length) in vd.transitions.items():
         toff[v - 1, worm - 1, leg - 1] = offset - 1
         tcnt[v - 1, worm - 1, leg - 1] = length + 1
@@ -76,7 +78,7 @@ def _vd_arrays(vd):
                    dtype=np.int32).reshape(ne)
     eleg = np.array([so[0] - 1 for so in vd.transition_step_outs], dtype=np.uint8).reshape(ne)
     ewrm = np.array([so[1] for so in vd.transition_step_outs], dtype=np.uint8).reshape(ne)
-    ls = np.ascontiguousarray(vd.leg_states.T.astype(np.uint8))  # [nv][4]
+    ls = np.ascontiguousarray(vd.leg_states.T.astype(np.uint8))  # [nv][2 NSites]
     return dict(dims=tuple(vd.dims), energy_offset=float(vd.energy_offset), n_vertices=nv,
                 weights=np.ascontiguousarray(vd.weights, dtype=np.float64),
                 signs=np.ascontiguousarray(vd.signs, dtype=np.int8), leg_states=ls,
@@ -92,16 +94,19 @@ class FlatModel:
 
     def __init__(self, sse_data, estimators=(), normalization_site_count=None,
                  estimator_tables=None):
-        if any(len(b.sites) != 2 for b in sse_data.bonds):
-            raise ValueError("device kernels support two-site bonds (leg_count 4) only")
-        if any(len(vd.dims) != 2 for vd in sse_data.vertex_data):
-            raise ValueError("device kernels support two-site VertexData only")
+        # SSEData{NSites}: every bond and bond type has the same site count
+        ns = {len(b.sites) for b in sse_data.bonds} | {len(vd.dims) for vd in sse_data.vertex_data}
+        if len(ns) != 1:
+            raise ValueError("all bonds and bond types of a model need the same number of sites")
+        self.bond_n_sites = ns.pop()
+        if not 1 <= self.bond_n_sites <= 4:
+            raise ValueError("bonds of 1 to 4 sites are supported")
         self.n_sites = len(sse_data.sites)
         self.site_dims = np.array([s.dim for s in sse_data.sites], dtype=np.uint8)
         self.n_bonds = len(sse_data.bonds)
         self.bond_type = np.array([b.type - 1 for b in sse_data.bonds], dtype=np.int32)
         self.bond_sites = np.array([[s - 1 for s in b.sites] for b in sse_data.bonds],
-                                   dtype=np.int32).reshape(self.n_bonds, 2)
+                                   dtype=np.int32).reshape(self.n_bonds, self.bond_n_sites)
         self.types = [_vd_arrays(vd) for vd in sse_data.vertex_data]
         self.energy_offset = float(sse_data.energy_offset)
         self.normalization_site_count = float(

@@ -41,6 +41,7 @@ def isconsistent(flat, ops, state0):
     state = np.array(state0, dtype=np.int64).copy()
     if not np.all((state >= 1) & (state <= flat.site_dims)):
         return False
+    ns = flat.bond_n_sites
     for op in ops:
         op = int(op)
         if op == 0:
@@ -49,10 +50,11 @@ def isconsistent(flat, ops, state0):
         v = (op >> 2) & 511
         t = flat.types[flat.bond_type[b]]
         ls = t["leg_states"][v]
-        s0, s1 = flat.bond_sites[b]
-        if state[s0] != ls[0] or state[s1] != ls[1]:
+        sites = flat.bond_sites[b]
+        if any(state[s] != ls[i] for i, s in enumerate(sites)):
             return False
-        state[s0], state[s1] = ls[2], ls[3]
+        for i, s in enumerate(sites):
+            state[s] = ls[ns + i]
     return bool(np.array_equal(state, state0))
 
 
@@ -101,3 +103,135 @@ def ladder_dimer(sse, Lx, J_leg, J_rung=1.0):
         "lattice": {"unitcell": sse.UnitCells.ladder, "size": (Lx, 1)},
         "measure_quantum_numbers": [{"name": "", "quantum_number": 2}],
         "parameter_map": {"J": ["Jr", "Jl", "Jl"]}, "Jr": J_rung, "Jl": J_leg})
+
+
+class _Est:
+    """Name prefix of an estimator row of a hand-built FlatModel."""
+
+    def __init__(self, prefix):
+        self.prefix = prefix
+
+
+def _spin_half():
+    sz = np.diag([0.5, -0.5])
+    sp = np.array([[0.0, 1.0], [0.0, 0.0]])
+    return sp, sz
+
+
+def trimer_ring(sse, n, J=-1.0, h=0.15):
+    """Spin-1/2 ring whose bonds are three consecutive sites (SSEBond{3}):
+    H_b = J (S_a.S_b + S_b.S_c + S_a.S_c) - (h/3)(S_a^z + S_b^z + S_c^z), the
+    field split over the three bonds that touch a site.  Ferromagnetic J < 0:
+    every vertex weight is positive.  Returns (flat, bond matrices, bond
+    site tuples 0-based)."""
+    sp, sz = _spin_half()
+    I2 = np.eye(2)
+
+    def lift(op, k):
+        mats = [I2, I2, I2]
+        mats[k] = op
+        return np.kron(np.kron(mats[0], mats[1]), mats[2])
+
+    Hb = np.zeros((8, 8))
+    for a, b in ((0, 1), (1, 2), (0, 2)):
+        Hb += J * (lift(sz, a) @ lift(sz, b) + 0.5 * (lift(sp, a) @ lift(sp.T, b)
+                                                        + lift(sp.T, a) @ lift(sp, b)))
+    for a in range(3):
+        Hb -= (h / 3.0) * lift(sz, a)
+    # VertexData takes the bond matrix in kron order, the first site slowest
+    # (vertex_data.jl:137-169 splits the index over reversed dims)
+    vd = sse.VertexData((2, 2, 2), Hb)
+    bonds = [sse.SSEBond(1, (i + 1, (i + 1) % n + 1, (i + 2) % n + 1)) for i in range(n)]
+    coef = np.ones((1, n))
+    mval = np.tile(np.array([0.5, -0.5]), (1, n, 1))
+    flat = sse.FlatModel(sse.SSEData([vd], bonds), [_Est("")], n, (coef, mval))
+    return flat, Hb, [tuple(x - 1 for x in b.sites) for b in bonds]
+
+
+def field_spins(sse, n, hx=0.8, hz=0.3):
+    """Spin-1/2 sites in a tilted field, one-site bonds (SSEBond{1}, two legs
+    per vertex): H_i = -hx S^x_i - hz S^z_i.  A worm through a two-leg vertex
+    either bounces or passes and flips both legs, so it never turns a
+    diagonal vertex into an off-diagonal one: with one-site bonds alone the
+    abstract-loop chain (reference included) is not ergodic and does not
+    reach -(|h|/2) tanh(beta |h| / 2).  The model exercises the leg-count-2
+    code paths for bit-exact parity only."""
+    sp, sz = _spin_half()
+    sx = 0.5 * (sp + sp.T)
+    Hb = -hx * sx - hz * sz
+    vd = sse.VertexData((2,), Hb)
+    bonds = [sse.SSEBond(1, (i + 1,)) for i in range(n)]
+    coef = np.ones((1, n))
+    mval = np.tile(np.array([0.5, -0.5]), (1, n, 1))
+    return sse.FlatModel(sse.SSEData([vd], bonds), [_Est("")], n, (coef, mval))
+
+
+def bond_model_ed(n, site_dim, bond_sites, Hb, Ts):
+    """Exact energy per site of H = sum over bonds of Hb acting on the bond's
+    sites (Hb in kron order, first bond site slowest); returns E(T) array."""
+    D = site_dim ** n
+    H = np.zeros((D, D))
+    for sites in bond_sites:
+        H += _embed(Hb, sites, n, site_dim)
+    w = np.linalg.eigvalsh(H)
+    out = []
+    for T in Ts:
+        x = np.exp(-(w - w.min()) / T)
+        out.append(float((w * x).sum() / x.sum()) / n)
+    return np.array(out)
+
+
+def _embed(Hb, sites, n, d):
+    """Hb (kron order over `sites`) as an operator on n sites of dim d (site 0
+    slowest)."""
+    k = len(sites)
+    D = d ** n
+    H = np.zeros((D, D))
+    for i in range(D):
+        di = [(i // d ** (n - 1 - s)) % d for s in range(n)]
+        bi = 0
+        for s in sites:
+            bi = bi * d + di[s]
+        for bj in range(d ** k):
+            v = Hb[bi, bj]
+            if v == 0.0:
+                continue
+            dj = list(di)
+            rem = bj
+            for s in reversed(sites):
+                dj[s] = rem % d
+                rem //= d
+            j = 0
+            for s in range(n):
+                j = j * d + dj[s]
+            H[i, j] += v
+    return H
+
+
+def plaquette_ring(sse, n, J=-0.7, h=0.1):
+    """Spin-1/2 ring whose bonds are four consecutive sites (SSEBond{4}):
+    H_b = J sum over the six site pairs of S.S - (h/4) sum S^z, ferromagnetic
+    (all vertex weights positive).  Returns (flat, Hb, bond sites 0-based)."""
+    sp, sz = _spin_half()
+    I2 = np.eye(2)
+
+    def lift(op, k):
+        mats = [I2, I2, I2, I2]
+        mats[k] = op
+        out = mats[0]
+        for mm in mats[1:]:
+            out = np.kron(out, mm)
+        return out
+
+    Hb = np.zeros((16, 16))
+    for a in range(4):
+        for b in range(a + 1, 4):
+            Hb += J * (lift(sz, a) @ lift(sz, b) + 0.5 * (lift(sp, a) @ lift(sp.T, b)
+                                                            + lift(sp.T, a) @ lift(sp, b)))
+        Hb -= (h / 4.0) * lift(sz, a)
+    vd = sse.VertexData((2, 2, 2, 2), Hb)
+    bonds = [sse.SSEBond(1, tuple((i + k) % n + 1 for k in range(4))) for i in range(n)]
+    coef = np.ones((1, n))
+    mval = np.tile(np.array([0.5, -0.5]), (1, n, 1))
+    flat = sse.FlatModel(sse.SSEData([vd], bonds), [_Est("")], n, (coef, mval))
+    return flat, Hb, [tuple(x - 1 for x in b.sites) for b in bonds]

@@ -22,7 +22,7 @@ def test_library_exports_all_symbols(sse_pkg):
     lib = sse_pkg.capi.load_library()
     for name in declared_symbols():
         assert hasattr(lib, name), name
-    assert lib.sse_abi_version() == 1
+    assert lib.sse_abi_version() == 2
 
 
 def test_invalid_arguments_rejected(sse_pkg):

@@ -1,0 +1,134 @@
+"""Bonds of one, three and four sites on the GPU (SSEData{NSites},
+sse_data.jl:15-42).  The lane kernels take any site count: bit-exact with
+the oracle after init and after every sweep, measurements included, in
+single-model and mixed batches; energies agree with exact diagonalisation.
+The wave kernels are built for two-site bonds and reject other models."""
+import sys
+
+import numpy as np
+import pytest
+
+import stats
+from fixtures import (bond_model_ed, field_spins, isconsistent, plaquette_ring,
+                      square_heisenberg, trimer_ring)
+from test_gpu_parity import KEYS, _compare
+
+pytestmark = pytest.mark.gpu
+
+LANE = 1
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _flat(sse, kind):
+    if kind == "trimer":
+        return trimer_ring(sse, 7)[0]
+    if kind == "plaquette":
+        return plaquette_ring(sse, 9)[0]
+    return field_spins(sse, 5)
+
+
+@pytest.mark.parametrize("kind", ["trimer", "plaquette", "field"])
+def test_lane_parity_every_sweep(sse_pkg, oracle_mod, kind):
+    flat = _flat(sse_pkg, kind)
+    walkers = [sse_pkg.walker_desc(T=T, seed=40 + k, thermalization=5)
+               for k, T in enumerate((0.25, 0.5, 1.0, 2.0))]
+    _say(f"multisite lane parity: {kind} bond_n_sites={flat.bond_n_sites}")
+    batch = sse_pkg.Batch([flat], walkers, mode=LANE)
+    oracles = [oracle_mod.OracleWalker(sse_pkg.capi, flat, w) for w in walkers]
+    batch.init()
+    for o in oracles:
+        o.init()
+    _compare(batch, oracles, f"{kind} init")
+    for s in range(25):
+        batch.sweep(1)
+        for o in oracles:
+            o.sweep(1)
+        _compare(batch, oracles, f"{kind} sweep {s}")
+    gm = batch.measure()
+    for k, o in enumerate(oracles):
+        assert np.allclose(gm[k], o.measure(), rtol=1e-12, atol=1e-13)
+    for k in range(len(walkers)):
+        sc, ops, spins = batch.get_walker(k)
+        assert isconsistent(flat, ops, spins)
+    batch.close()
+
+
+def test_lane_vertex_list_legs(sse_pkg, oracle_mod):
+    """sse_get_vertex_list returns 2 NSites legs per slot, equal to the
+    oracle's list of the same operator string."""
+    flat = _flat(sse_pkg, "trimer")
+    w = sse_pkg.walker_desc(T=0.5, seed=3)
+    _say("multisite vertex list")
+    batch = sse_pkg.Batch([flat], [w], mode=LANE)
+    o = oracle_mod.OracleWalker(sse_pkg.capi, flat, w)
+    batch.init()
+    o.init()
+    batch.sweep(6)
+    o.sweep(6)
+    links, first, last = batch.vertex_list(0)
+    olinks, ofirst, olast = o.vertex_list()
+    assert len(links) == 6 * o.scalars()["cutoff"]
+    assert np.array_equal(links, olinks)
+    assert np.array_equal(first, ofirst) and np.array_equal(last, olast)
+    batch.close()
+
+
+def test_lane_mixed_site_counts_run(sse_pkg, oracle_mod):
+    """One lane batch holding one-, two-, three- and four-site models, run
+    by sse_run with fused measurement: accumulators equal the oracle's
+    series summed, per walker."""
+    flats = [_flat(sse_pkg, "field"), square_heisenberg(sse_pkg, 4).flat(),
+             _flat(sse_pkg, "trimer"), _flat(sse_pkg, "plaquette")]
+    walkers = [sse_pkg.walker_desc(model=k, T=0.4 + 0.2 * k, seed=500 + k, thermalization=8)
+               for k in range(4)]
+    _say("multisite mixed batch")
+    batch = sse_pkg.Batch(flats, walkers, mode=LANE)
+    batch.init()
+    batch.run(30, measure=True)
+    acc = batch.accumulators()
+    for k, w in enumerate(walkers):
+        o = oracle_mod.OracleWalker(sse_pkg.capi, flats[k], sse_pkg.walker_desc(
+            T=w.T, seed=w.seed, thermalization=8))
+        o.init()
+        rows = o.run_series(30)
+        s, s2, cnt = acc[k]
+        assert cnt == len(rows) == 30 - 8
+        assert np.allclose(s, rows.sum(axis=0), rtol=1e-12, atol=1e-12)
+        assert np.allclose(s2, (rows ** 2).sum(axis=0), rtol=1e-12, atol=1e-12)
+        gs, gops, gspins = batch.get_walker(k)
+        os_, oops, ospins = o.state()
+        assert np.array_equal(gops, oops) and np.array_equal(gspins, ospins)
+        for key in KEYS:
+            assert gs[key] == os_[key]
+    batch.close()
+
+
+def test_wave_mode_rejects_other_site_counts(sse_pkg):
+    flat = _flat(sse_pkg, "trimer")
+    with pytest.raises(sse_pkg.capi.SSEError, match="two-site"):
+        sse_pkg.Batch([flat], [sse_pkg.walker_desc(T=1.0, seed=1)], mode=0)
+
+
+@pytest.mark.parametrize("kind,n,T", [("trimer", 6, 0.5), ("plaquette", 8, 0.7)])
+def test_lane_energy_against_ed(sse_pkg, kind, n, T):
+    """64 independent lane walkers, 3000 measured sweeps each; jackknife over
+    walkers; |E - E_ED| <= 4 sigma."""
+    make = trimer_ring if kind == "trimer" else plaquette_ring
+    flat, Hb, bs = make(sse_pkg, n)
+    exact = bond_model_ed(n, 2, bs, Hb, [T])[0]
+    walkers = [sse_pkg.walker_desc(T=T, seed=900 + k, thermalization=500) for k in range(64)]
+    _say(f"multisite ED: {kind} n={n} T={T}")
+    batch = sse_pkg.Batch([flat], walkers, mode=LANE)
+    batch.init()
+    batch.run(3500, measure=True)
+    acc = batch.accumulators()
+    means = np.array([a[0] / a[2] for a in acc])
+    res = stats.jackknife(flat.observable_names(), means, flat.normalization_site_count)
+    mean, err = res["Energy"]
+    _say(f"  E = {mean:.5f} +- {err:.5f}  ED {exact:.5f}")
+    assert abs(mean - exact) <= 4 * err
+    batch.close()
