@@ -365,6 +365,62 @@ def serving_stream(sse, seconds=20.0, count_for=60.0, sweep_scale=1.0, slots=102
     return stats
 
 
+def carlo_boundary(sse, flat, wds, batch, tasks=16, steps=30, warm=15):
+    """The batched workload driven through Carlo's AbstractMC methods
+    (carlo_mc): `tasks` MC objects, each carrying len(wds)/tasks runs, in
+    one shared batch, stepped by step_all (sweep! of every MC, then
+    measure! of every MC: one launch per Carlo sweep, measurements handed
+    to each MC's context).  The walkers start from `batch`'s thermalized
+    states.  Also the same walkers as one MC per run (the slot-per-MC
+    pattern), whose host side is one Python object per walker."""
+    cm = sse.carlo_mc
+    n = len(wds)
+    # the direct path at one sweep per call: the per-call floor any
+    # one-sweep-per-Carlo-sweep driver pays
+    batch.run(2, measure=True)
+    Lw = batch.work_counters()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run(1, measure=True)
+    dt = time.perf_counter() - t0
+    Lw1 = batch.work_counters()
+    out = {"direct_one_sweep_per_call": {
+        "vertex_updates_per_s": (Lw1[0] - Lw[0] + Lw1[2] - Lw[2]) / dt,
+        "ms_per_call": dt * 1e3 / steps, "calls": steps}}
+    states = [batch.get_walker(k) for k in range(n)]
+    for name, n_mc in (("mc_with_runs", tasks), ("mc_per_run", n)):
+        R = n // n_mc
+        sh = cm.SharedBatch(flat, n)
+        mcs = []
+        for t in range(n_mc):
+            mc = cm.MC({"model": flat, "T": wds[t * R].T, "seed": 10 ** 6 + t * R,
+                        "thermalization": 0}, sh, runs=R)
+            for r, slot in enumerate(mc.slots):
+                sc, ops, spins = states[t * R + r]
+                sh.batch.set_walker(int(slot), sc, ops, spins)
+            mcs.append(mc)
+        ctxs = [cm.MCContext(0) for _ in mcs]
+        k_warm, k_steps = (warm, steps) if n_mc == tasks else (3, 5)
+        for _ in range(k_warm):
+            cm.step_all(mcs, ctxs)
+        sh.work[:] = 0
+        l0 = sh.launches
+        t0 = time.perf_counter()
+        for _ in range(k_steps):
+            cm.step_all(mcs, ctxs)
+        dt = time.perf_counter() - t0
+        Ls, ns, Ws, wsw = (int(x) for x in sh.work)
+        out[name] = {"vertex_updates_per_s": (Ls + Ws) / dt, "ms_per_carlo_sweep": dt * 1e3 / k_steps,
+                     "mc_objects": n_mc, "runs_per_mc": R, "launches_per_carlo_sweep":
+                     (sh.launches - l0) / k_steps, "walker_sweeps": wsw, "timed_steps": k_steps,
+                     "samples_per_mc": len(ctxs[0].series("Sign"))}
+        sh.close()
+    out["workload"] = (f"{n} thermalized walkers of the batched workload as Carlo MC objects in "
+                       f"one shared batch; step_all per Carlo sweep (every sweep! posted, one "
+                       f"launch, then every measure!)")
+    return out
+
+
 def single_walker(sse, thermalization=10000, sweeps=5000):
     """The plain single walker: 8x8 Heisenberg, beta=2, seed 1, initial cutoff
     64, on the GPU (wave and lane kernels, one walker per batch) and on one
@@ -487,6 +543,8 @@ def main():
                     help="oracle threads of the CPU baseline (0: the process's CPU share, "
                          "min(nproc, 16))")
     ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--no-carlo", action="store_true",
+                    help="skip the Carlo-boundary measurement (MC objects over a shared batch)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="end with three device-to-device copies of this many MiB "
                          "(reference kernel for HBM traffic counters)")
@@ -604,6 +662,16 @@ def main():
         baselines["single_walker"] = single_walker(sse)
     else:
         cpu = None
+    if not args.no_carlo:
+        log("Carlo boundary: MC objects over one shared batch")
+        try:
+            cb = carlo_boundary(sse, flat, wds, batch)
+            for v in cb.values():
+                if isinstance(v, dict):
+                    v["fraction_of_direct"] = v["vertex_updates_per_s"] / value
+            baselines["carlo_boundary"] = cb
+        except Exception as e:  # the headline line is reported either way
+            baselines["carlo_boundary"] = {"error": f"{type(e).__name__}: {e}"}
     batch.close()
 
     out = {

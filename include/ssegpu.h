@@ -186,9 +186,11 @@ int sse_read_accumulators(sse_batch *b, double *sums, double *sumsq,
  * when measure != 0).  A walker's chain and results are those of the same
  * number of Carlo.sweep! calls (src/sse.jl:62-87) however the budgets
  * differ, so a driver can give every Carlo task of a batch its own sweep
- * count, or balance uneven walkers by expected cost.  Wave mode; not with
- * temperature ladders that exchange between sweeps.  No reference
- * counterpart: the reference sweeps one MC per call.                        */
+ * count, or balance uneven walkers by expected cost.  In lane mode the call
+ * is one sweep (+ measure) launch per sweep index, in which the walkers
+ * whose budget is used up stay idle.  Not with temperature ladders that
+ * exchange between sweeps.  No reference counterpart: the reference sweeps
+ * one MC per call.                                                          */
 int sse_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure);
 /* sse_run_budgets without any device allocation: no counterpart in the
  * reference.  For callers that cannot allocate device memory during the

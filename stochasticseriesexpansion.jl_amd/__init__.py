@@ -5,7 +5,7 @@ Carlo's AbstractMC interface, above the C ABI of libssegpu.so (gfx950 HIP
 kernels).  The directory name contains a dot, so load it with
 ``__graft_entry__.load_package()`` (registered as module ``sse_amd``).
 """
-from . import util, lattice, vertex_data, sse_data, models, capi, h5lite, carlo, host_estimators  # noqa: F401
+from . import util, lattice, vertex_data, sse_data, models, capi, h5lite, carlo, carlo_mc, host_estimators  # noqa: F401
 from .lattice import Lattice, UnitCell, UnitCells, UCBond, UCSite, neel_vector  # noqa: F401
 from .vertex_data import VertexData  # noqa: F401
 from .sse_data import SSEBond, SSEData, FlatModel  # noqa: F401

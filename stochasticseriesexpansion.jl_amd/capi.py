@@ -382,6 +382,30 @@ class Batch:
             k += wd
         return out
 
+    def accumulators_raw(self, with_sumsq=True):
+        """(sums, sums of squares or None, counts) as flat arrays: walker w's
+        row starts at slot_offsets()[w]."""
+        total = sum(self._slot_widths)
+        s = np.zeros(total)
+        s2 = np.zeros(total) if with_sumsq else None
+        cnt = np.zeros(len(self.walkers), dtype=np.int64)
+        check(self.lib, self.lib.sse_read_accumulators(
+            self.handle, _ptr(s, C.c_double), _ptr(s2, C.c_double) if with_sumsq else None,
+            _ptr(cnt, C.c_int64)))
+        return s, s2, cnt
+
+    def slot_offsets(self):
+        return np.concatenate([[0], np.cumsum(self._slot_widths)[:-1]]).astype(np.int64)
+
+    def worm_fractions_raw(self):
+        n = len(self.walkers)
+        s, s2 = np.zeros(n), np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int64)
+        check(self.lib, self.lib.sse_read_worm_fraction(self.handle, _ptr(s, C.c_double),
+                                                        _ptr(s2, C.c_double),
+                                                        _ptr(cnt, C.c_int64)))
+        return s, s2, cnt
+
     def worm_fractions(self):
         """Per walker (sum, sum of squares, count) of WormLengthFraction over
         the measured sweeps since the last reset (sse.jl:200-202)."""

@@ -957,7 +957,8 @@ static int launch_sweep(sse_batch *b, int phases, hipEvent_t *ev = nullptr)
         int threads = 64;
         int blocks = (n + threads - 1) / threads;
         hipLaunchKernelGGL(lane_sweep_kernel, dim3(blocks), dim3(threads), 0, b->stream,
-                           b->d_walkers, b->d_models, b->d_pool, 0, n, phases);
+                           b->d_walkers, b->d_models, b->d_pool, 0, n, phases,
+                           (const int32_t *)nullptr, 0);
         if (ev) {
             HIPCHECK(hipEventRecord(ev[1], b->stream));
             HIPCHECK(hipEventRecord(ev[2], b->stream));
@@ -1669,7 +1670,8 @@ static int launch_measure(sse_batch *b, int mode)
     if (b->mode == SSE_MODE_LANE) {
         hipLaunchKernelGGL(lane_measure_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream,
                            b->d_walkers, b->d_models, b->d_out, b->d_sum, b->d_sumsq, b->d_count,
-                           b->d_wlf, b->d_scratch, b->max_sites, n, mode, b->d_done);
+                           b->d_wlf, b->d_scratch, b->max_sites, n, mode, b->d_done,
+                           (const int32_t *)nullptr, 0);
     } else {
         int rc = wave_launch_measure(b->wave, b->stream, b->d_walkers, b->d_models, b->d_out,
                                      b->d_sum, b->d_sumsq, b->d_count, b->d_wlf, 0, n, mode,
@@ -1722,17 +1724,66 @@ int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure)
     return sweep_call(b, n_sweeps, measure);
 }
 
+// sse_run_budgets in lane mode: one lane per walker, so the call is a
+// sequence of sweep (+ measure) launches; launch i runs the walkers whose
+// budget exceeds i.
+static int lane_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure)
+{
+    const int n = b->n_walkers;
+    std::vector<int32_t> bud(n);
+    int64_t most = 0;
+    for (int w = 0; w < n; ++w) {
+        if (sweeps[w] < 0 || sweeps[w] > INT32_MAX)
+            return fail(SSE_ERR_INVALID, "sse_run_budgets: sweep counts must be in 0..2^31-1");
+        bud[w] = (int32_t)sweeps[w];
+        most = std::max(most, sweeps[w]);
+    }
+    if (b->any_pending) {
+        int rc = launch_init_pending(b);
+        if (rc)
+            return rc;
+    }
+    int rc = call_timer(b, 0, 0);
+    if (rc)
+        return rc;
+    HIPCHECK(bmemcpy(b, b->d_budget, bud.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    const int blocks = (n + 63) / 64;
+    for (int64_t i = 0; i < most; ++i) {
+        hipLaunchKernelGGL(lane_sweep_kernel, dim3(blocks), dim3(64), 0, b->stream, b->d_walkers,
+                           b->d_models, b->d_pool, 0, n, 7, (const int32_t *)b->d_budget, (int)i);
+        if (measure)
+            hipLaunchKernelGGL(lane_measure_kernel, dim3(blocks), dim3(64), 0, b->stream,
+                               b->d_walkers, b->d_models, b->d_out, b->d_sum, b->d_sumsq,
+                               b->d_count, b->d_wlf, b->d_scratch, b->max_sites, n, 1, b->d_done,
+                               (const int32_t *)b->d_budget, (int)i);
+        HIPCHECK(hipGetLastError());
+    }
+    if (!measure && b->any_budget) {
+        hipLaunchKernelGGL(retire_kernel, dim3(blocks), dim3(64), 0, b->stream, b->d_walkers, n,
+                           b->d_done);
+        HIPCHECK(hipGetLastError());
+    }
+    rc = call_timer(b, 1, most);
+    if (rc)
+        return rc;
+    rc = check_walker_errors(b);
+    if (!rc)
+        rc = call_timer(b, 2, 0);
+    clear_phase_times(b);
+    return rc;
+}
+
 int sse_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure)
 {
     if (!b || !sweeps)
         return fail(SSE_ERR_INVALID, "sse_run_budgets: bad arguments");
     if (!b->inited)
         return fail(SSE_ERR_STATE, "run before init");
-    if (b->mode != SSE_MODE_WAVE)
-        return fail(SSE_ERR_INVALID, "sse_run_budgets: wave mode only");
     if (b->pt_ladders > 0 && b->pt_interval > 0)
         return fail(SSE_ERR_INVALID, "sse_run_budgets: walkers of a temperature ladder advance "
                                      "together (sse_run)");
+    if (b->mode == SSE_MODE_LANE)
+        return lane_run_budgets(b, sweeps, measure);
     const int n = b->n_walkers;
     std::vector<int32_t> bud(n);
     int64_t most = 0;

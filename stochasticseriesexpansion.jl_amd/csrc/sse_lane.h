@@ -220,11 +220,15 @@ __device__ void lane_worm_update(DevWalker &w, const DevModel &m, LaneRng &rng, 
 
 // phases: 1 diagonal, 2 vertex list, 4 worm; phases==7 is a full sweep and
 // advances the walker's sweep counter.  Walkers with an error flag are skipped.
+// With budget non-null this launch is sweep `step` of a budgeted call: walker
+// wi takes part only while step < budget[wi].
 __global__ void lane_sweep_kernel(DevWalker *W, const DevModel *models, DevPool *pool, int w_begin,
-                                  int w_end, int phases)
+                                  int w_end, int phases, const int32_t *budget, int step)
 {
     int wi = w_begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (wi >= w_end)
+        return;
+    if (budget && step >= budget[wi])
         return;
     DevWalker w = W[wi];
     // a slot whose sweep budget ran out (retire set) waits for its
@@ -370,13 +374,17 @@ __device__ __forceinline__ void accumulate(double *sum, double *sumsq, const dou
 // mode 0: measure every walker (explicit Carlo.measure! call)
 // mode 1: measure thermalized walkers (Carlo measures after a sweep once
 //         sweeps > thermalization) and accumulate into sums
+// budget/step as in lane_sweep_kernel: a budgeted call measures a walker
+// only after the sweeps it took part in
 __global__ void lane_measure_kernel(DevWalker *W, const DevModel *models, double *out, double *sum,
                                     double *sumsq, int64_t *count, double *wlf,
                                     uint8_t *scratch_spins, int max_sites, int n_walkers, int mode,
-                                    int32_t *done)
+                                    int32_t *done, const int32_t *budget, int step)
 {
     int wi = blockIdx.x * blockDim.x + threadIdx.x;
     if (wi >= n_walkers)
+        return;
+    if (budget && step >= budget[wi])
         return;
     const DevWalker &w = W[wi];
     // a pending slot's opstring and spins are not initialised yet

@@ -412,13 +412,14 @@ def test_persistent_many_walkers(sse_pkg):
     b.close()
 
 
-@pytest.mark.parametrize("records", [0, 2, 3])
-def test_run_budgets_against_oracle(sse_pkg, oracle_mod, records):
+@pytest.mark.parametrize("records,mode", [(0, 0), (2, 0), (3, 0), (0, 1)])
+def test_run_budgets_against_oracle(sse_pkg, oracle_mod, records, mode):
     """sse_run_budgets: every walker of a heterogeneous batch advances by its
     own sweep count in one persistent launch (0 for some), twice; each walker
     equals its oracle twin after the same number of sweeps, in the LDS
     records and in the global-memory layouts (RecGV / RecHV, links read by
-    vector loads)."""
+    vector loads), and in lane mode (one launch per sweep index, walkers
+    past their budget idle)."""
     rng = np.random.default_rng(21)
     flats, walkers = [], []
     for k in range(12):
@@ -427,9 +428,10 @@ def test_run_budgets_against_oracle(sse_pkg, oracle_mod, records):
         flats.append(model.flat())
         beta = float(np.exp(rng.uniform(np.log(0.5), np.log(8.0))))
         walkers.append(sse_pkg.walker_desc(model=k, T=1.0 / beta, seed=700 + k, thermalization=5))
-    _say(f"run_budgets: 12 walkers, records={records}")
-    batch = sse_pkg.Batch(flats, walkers)
-    batch.set_option(sse_pkg.capi.SSE_OPT_WORM_RECORDS, records)
+    _say(f"run_budgets: 12 walkers, records={records}, mode={mode}")
+    batch = sse_pkg.Batch(flats, walkers, mode=mode)
+    if mode == 0:
+        batch.set_option(sse_pkg.capi.SSE_OPT_WORM_RECORDS, records)
     batch.init()
     b1 = rng.integers(0, 20, size=12)
     b1[3] = 0
