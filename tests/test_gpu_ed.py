@@ -53,7 +53,10 @@ def test_ed_compare_gpu(sse_pkg, job):
             z = abs(mean - ev[i]) / err
             total += 1
             within2 += z <= 2
+            if z > 2:
+                print(f"  {job} T={T:.2f} {name}: GPU {mean:.6g}+-{err:.2g} ED {ev[i]:.6g} "
+                      f"z={z:.2f}", flush=True)
             assert z <= 4, f"{job} T={T:.2f} {name}: GPU {mean}+-{err} ED {ev[i]} z={z:.2f}"
     print(f"{job}: {within2}/{total} observables within 2 sigma", flush=True)
-    assert within2 >= 0.8 * total
+    assert within2 >= 0.9 * total
     batch.close()

@@ -1,18 +1,33 @@
 """The HIP path against results the reference itself published:
 docs/src/bani2v2o8.results.json (job examples/bani2v2o8.jl: spin-1 honeycomb
-Heisenberg antiferromagnet with single-ion anisotropy Dz, Carlo binning over
-80000 sweeps per task).  Five of its tasks (tests/golden/bani2v2o8_rows.json,
-written by tests/golden/make_bani2v2o8_rows.py) run here as one heterogeneous
-batch of independent walkers; every observable must agree with the published
-mean within 4 combined standard errors, and at least 80 % within 2.
+Heisenberg antiferromagnet with single-ion anisotropy Dz, 10x10 and 20x20,
+20 temperatures each, Carlo binning over 80000 sweeps after 10000
+thermalization sweeps per task; all 40 rows in
+tests/golden/bani2v2o8_rows.json, written by
+tests/golden/make_bani2v2o8_rows.py).  Tasks run as batches of independent
+walkers; every compared observable must agree with the published mean
+within 4 combined standard errors, and at least 90 % within 2.
 
-At T = 0.258 only the energy-type observables are compared.  A walker's total
-magnetization changes only through worms that wind the imaginary-time axis,
-which are rare at that temperature: per-walker means of the magnetization
-moments over 1000 sweeps are heavy-tailed, and three seed sets of 64 oracle
-walkers (0-63, 500-563, 90000-90063; the GPU reproduces the last bit for bit)
-give Mag2 = 1.36, 1.21 and 1.32 e-4 with jackknife errors of ~3e-6 each,
-against the published 1.2055(8) e-4 of one 80000-sweep chain."""
+The magnetization moments need long chains at low T: a walker's total
+magnetization changes only through worms that wind the imaginary-time
+axis, which are rare there, so a chain's mean over a few hundred sweeps
+depends on its starting sector.  Means over 1000 sweeps after 600
+thermalization sweeps came out heavy-tailed (three seed sets of 64 walkers
+gave Mag2 = 1.36, 1.21, 1.32 e-4 against the published 1.2055(8) e-4 at
+T = 0.258); 64 oracle walkers of 2000 + 8000 sweeps give 1.211(20) e-4.
+So the tasks at T >= 0.47 run many short chains (every observable from
+T = 0.67 on, energy-type ones at 0.47), T = 0.258 runs long chains with
+every observable, and T = 0.05 runs a few chains with the energy-type
+observables.  The 20x20 tasks below T = 0.4 (12600 and 65000 operators per
+walker) are left out for run time.
+
+Seeds of the T = 0.05 chains: starting from random spins at beta = 20, the
+reference's worm controller (avg_worm_length 1, sse.jl:35) makes some
+chains take single sweeps of 10^8-10^10 scatter steps while it adapts; one
+wavefront runs such a chain at the latency of its dependent record loads
+(about 0.5 us a step once the records outgrow LDS), i.e. minutes.  The
+block 72000-72015 has at most 1.8e7 steps in its first 100 sweeps (checked
+on the oracle); the measured sweeps come long after that transient."""
 import json
 import os
 import sys
@@ -26,12 +41,8 @@ import stats
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-WALKERS = 64
-THERM = 600
-SWEEPS = 1000
-CHUNK = 50
-SLOW_SECTOR_T = 0.5   # below: skip the magnetization moments (see above)
 MAGNETIZATION = ("Mag2", "AbsMag", "MagChi", "BinderRatio")
+ENERGY_TYPE = ("Energy", "SpecificHeat", "OperatorCount")
 
 
 def _say(msg):
@@ -39,15 +50,52 @@ def _say(msg):
     sys.stdout.flush()
 
 
+def _rows():
+    return json.load(open(os.path.join(HERE, "golden", "bani2v2o8_rows.json")))["rows"]
+
+
+def _model(sse_pkg, r):
+    return sse_pkg.MagnetModel({"lattice": {"unitcell": sse_pkg.UnitCells.honeycomb,
+                                            "size": tuple(r["size"])},
+                                "S": r["S"], "J": r["J"], "Dz": r["Dz"],
+                                "measure": ["magnetization"]})
+
+
+def _compare(flat, r, acc, first, walkers, sweeps, names_to_compare, report):
+    """Jackknife over the walkers' means against the published row; appends
+    (z, line) per observable to report."""
+    names = flat.observable_names()
+    assert all(acc[first + k][2] == sweeps for k in range(walkers))
+    means = np.array([acc[first + k][0] / acc[first + k][2] for k in range(walkers)])
+    res = stats.jackknife(names, means, flat.normalization_site_count)
+    nc = means[:, names.index("OperatorCount")]
+    res["OperatorCount"] = (float(nc.mean()), float(nc.std(ddof=1) / np.sqrt(len(nc))))
+    for name in names_to_compare:
+        ref_mean, ref_err = r["results"][name]
+        mean, err = res[name]
+        sigma = np.hypot(err, ref_err)
+        z = abs(mean - ref_mean) / sigma if sigma > 0 else 0.0
+        report.append((z, f"L={r['size'][0]} T={r['T']:.3f} {name}: {mean:.6g}+-{err:.2g} "
+                          f"published {ref_mean:.6g}+-{ref_err:.2g} z={z:.2f}"))
+
+
+def _gate(report, tag):
+    _say("\n".join(line for _, line in report))
+    worst = max(report)
+    within2 = sum(z <= 2 for z, _ in report)
+    _say(f"{tag}: {within2}/{len(report)} observables within 2 sigma")
+    assert worst[0] <= 4, worst[1]
+    assert within2 >= 0.9 * len(report)
+
+
 def test_bani2v2o8_published_results(sse_pkg):
-    rows = json.load(open(os.path.join(HERE, "golden", "bani2v2o8_rows.json")))["rows"]
+    """Every task at T >= 0.47 (36 of 40), 128 walkers each, 1000 + 2000
+    sweeps, one batch."""
+    WALKERS, THERM, SWEEPS, CHUNK = 128, 1000, 2000, 250
+    rows = [r for r in _rows() if r["T"] > 0.4]
     flats, walkers = [], []
     for i, r in enumerate(rows):
-        model = sse_pkg.MagnetModel({"lattice": {"unitcell": sse_pkg.UnitCells.honeycomb,
-                                                 "size": tuple(r["size"])},
-                                     "S": r["S"], "J": r["J"], "Dz": r["Dz"],
-                                     "measure": ["magnetization"]})
-        flats.append(model.flat())
+        flats.append(_model(sse_pkg, r).flat())
         for k in range(WALKERS):
             walkers.append(sse_pkg.walker_desc(model=i, T=r["T"], seed=90000 + 1000 * i + k,
                                                thermalization=THERM))
@@ -59,31 +107,52 @@ def test_bani2v2o8_published_results(sse_pkg):
         batch.run(CHUNK, measure=True)
         _say(f"bani2v2o8: {done + CHUNK} sweeps, {time.time() - t0:.1f} s")
     acc = batch.accumulators()
-    within2 = total = 0
     report = []
     for i, r in enumerate(rows):
-        flat = flats[i]
-        names = flat.observable_names()
-        means = np.array([acc[i * WALKERS + k][0] / acc[i * WALKERS + k][2]
-                          for k in range(WALKERS)])
-        assert all(acc[i * WALKERS + k][2] == SWEEPS for k in range(WALKERS))
-        res = stats.jackknife(names, means, flat.normalization_site_count)
-        ix = {n: j for j, n in enumerate(names)}
-        nc = means[:, ix["OperatorCount"]]
-        res["OperatorCount"] = (float(nc.mean()), float(nc.std(ddof=1) / np.sqrt(len(nc))))
-        for name, (ref_mean, ref_err) in r["results"].items():
-            if r["T"] < SLOW_SECTOR_T and name in MAGNETIZATION:
-                continue
-            mean, err = res[name]
-            sigma = np.hypot(err, ref_err)
-            z = abs(mean - ref_mean) / sigma if sigma > 0 else 0.0
-            total += 1
-            within2 += z <= 2
-            report.append((z, f"L={r['size'][0]} T={r['T']:.3f} {name}: {mean:.6g}+-{err:.2g} "
-                              f"published {ref_mean:.6g}+-{ref_err:.2g} z={z:.2f}"))
-    _say("\n".join(line for _, line in report))
-    worst = max(report)
-    assert worst[0] <= 4, worst[1]
-    _say(f"bani2v2o8: {within2}/{total} observables within 2 sigma")
-    assert within2 >= 0.8 * total
+        names = ENERGY_TYPE if r["T"] < 0.5 else ENERGY_TYPE + MAGNETIZATION
+        _compare(flats[i], r, acc, i * WALKERS, WALKERS, SWEEPS, names, report)
     batch.close()
+    _gate(report, "bani2v2o8")
+
+
+def test_bani2v2o8_low_temperature(sse_pkg):
+    """10x10 at T = 0.258 with every observable (64 walkers, 2000 + 8000
+    sweeps) and T = 0.05 with the energy-type observables (16 walkers, 300 +
+    1000 sweeps), in one batch: sse_run_budgets gives each task its own
+    sweeps per call, so both finish together."""
+    tasks = {0.05: (16, 300, 1000, 72000), 0.2578947368421053: (64, 2000, 8000, 70000)}
+    CALLS = 20
+
+    def task(r):
+        return tasks[min(tasks, key=lambda t: abs(t - r["T"]))]
+
+    rows = [r for r in _rows() if r["size"][0] == 10 and
+            any(abs(r["T"] - t) < 1e-12 for t in tasks)]
+    assert len(rows) == 2
+    flats, walkers, budget = [], [], []
+    for i, r in enumerate(rows):
+        n_w, therm, sweeps, seed0 = task(r)
+        assert (therm + sweeps) % CALLS == 0
+        flats.append(_model(sse_pkg, r).flat())
+        for k in range(n_w):
+            walkers.append(sse_pkg.walker_desc(model=i, T=r["T"], seed=seed0 + k,
+                                               thermalization=therm))
+            budget.append((therm + sweeps) // CALLS)
+    _say(f"bani2v2o8 low T: {[(round(r['T'], 3), task(r)) for r in rows]}")
+    batch = sse_pkg.Batch(flats, walkers)
+    batch.init()
+    t0 = time.time()
+    budget = np.array(budget, dtype=np.int64)
+    for c in range(CALLS):
+        batch.run_budgets(budget, measure=True)
+        _say(f"bani2v2o8 low T: call {c + 1}/{CALLS}, {time.time() - t0:.1f} s")
+    acc = batch.accumulators()
+    report = []
+    first = 0
+    for i, r in enumerate(rows):
+        n_w, therm, sweeps, _ = task(r)
+        names = ENERGY_TYPE if r["T"] < 0.1 else ENERGY_TYPE + MAGNETIZATION
+        _compare(flats[i], r, acc, first, n_w, sweeps, names, report)
+        first += n_w
+    batch.close()
+    _gate(report, "bani2v2o8 low T")
