@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "libsseoracle.so")
+# SSE_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of run_sanitized.sh)
+LIB = os.environ.get("SSE_ORACLE_LIB") or os.path.join(HERE, "libsseoracle.so")
 
 
 def build():

@@ -5,10 +5,11 @@ Conventions follow the reference: vertex indices and states are 1-based,
 a vertex code is ``diag | idx << 1`` (opercode.jl:18-21) and ``None`` is the
 invalid vertex (opercode.jl:16).  The transition LP (vertex_data.jl:213-416)
 is solved by the library's C++ builder (sse_vertex_transitions: a dense
-two-phase simplex) or, with lp_solver="highs", by the HiGHS dual simplex
-shipped in scipy.  When an LP optimum is degenerate either may return a
-different optimal vertex than the reference's HiGHS.jl build; every returned
-table is checked for detailed balance and normalisation by the tests instead.
+two-phase simplex); `lp_solver` may also be a callable with the signature
+of that builder (the tests pass a HiGHS-based one as a cross-check).  When
+an LP optimum is degenerate the builder may return a different optimal
+vertex than the reference's HiGHS.jl build; every returned table is checked
+for detailed balance and normalisation by the tests instead.
 """
 from __future__ import annotations
 
@@ -105,16 +106,6 @@ def vertex_apply_change(leg_states, dims, vertex, step_in, step_out, index=None)
     return index.find(new)
 
 
-def _solve_lp(cost, A, b, lp_tolerance):
-    from scipy.optimize import linprog  # only the lp_solver="highs" path needs scipy
-    res = linprog(cost, A_eq=A, b_eq=b, bounds=[(0, None)] * len(cost), method="highs-ds",
-                  options={"primal_feasibility_tolerance": lp_tolerance,
-                           "dual_feasibility_tolerance": lp_tolerance})
-    if res.status != 0:
-        raise RuntimeError(f"transition probability optimization failed: {res.message}")
-    return res.x
-
-
 def construct_transitions_native(weights, leg_states, max_worm_count, dims, tolerance,
                                  lp_tolerance):
     """construct_transitions by the library's C++ builder (sse_vertex_transitions:
@@ -135,128 +126,6 @@ def construct_transitions_native(weights, leg_states, max_worm_count, dims, tole
         warnings.warn("had to truncate some probabilities in a possibly inaccurate way!")
     return (transitions, [float(x) for x in cum], [int(t) for t in tg],
             [(int(a), int(b)) for a, b in zip(ol, ow)], flags)
-
-
-def construct_transitions(weights, leg_states, max_worm_count, dims, tolerance, lp_tolerance):
-    """Transition tables from one LP per class of vertices connected by a
-    single worm step (vertex_data.jl:213-416), each LP solved by scipy's HiGHS.
-
-    Returns (transitions, cumprobs, targets, step_outs) where transitions maps
-    (leg_in, worm_in, vertex) -> (offset, length) with a 1-based offset and
-    length = number of exits - 1, as the reference's Transition."""
-    ns = len(dims)
-    leg_count = 2 * ns
-    transitions = {}
-    cumprobs, targets, step_outs = [], [], []
-
-    steps = []
-    inv_steps = {}
-    step_idx = {}
-    for worm in range(1, max_worm_count + 1):
-        for leg in range(1, leg_count + 1):
-            dim = dims[site_of_leg(leg, ns) - 1]
-            if worm <= worm_count(dim):
-                steps.append((leg, worm))
-                step_idx[(leg, worm)] = len(steps)
-                inv_steps[(leg, worm)] = (leg, worm_inverse(worm, dim))
-
-    def inverse(vc):
-        return (inv_steps[vc[1]], inv_steps[vc[0]])
-
-    variables = []
-    var_set = set()
-    for s_in in steps:
-        for s_out in steps:
-            vc = (s_in, s_out)
-            if inverse(vc) not in var_set:
-                variables.append(vc)
-                var_set.add(vc)
-    # variable of (in, out): findfirst over variables equal to it or whose
-    # inverse is (vertex_data.jl:359-365)
-    var_lookup = {}
-    for s_in in steps:
-        for s_out in steps:
-            key = (s_in, s_out)
-            cands = [k for k, vc in enumerate(variables) if vc == key or inverse(vc) == key]
-            var_lookup[key] = cands[0]
-
-    cost = np.array([1.0 if vc == inverse(vc) else 0.0 for vc in variables])
-    A = np.zeros((len(steps), len(variables)))
-    for r, step in enumerate(steps):
-        for k, vc in enumerate(variables):
-            if vc[0] == step or inverse(vc)[0] == step:
-                A[r, k] = 1.0
-
-    index = _LegStateIndex(leg_states)
-    nv = len(weights)
-    used_inaccurate = False
-    while True:
-        v = None
-        step_in = None
-        for empty_v in range(1, nv + 1):
-            found = None
-            for s in steps:
-                if (s[0], s[1], empty_v) not in transitions:
-                    found = s
-                    break
-            if found is not None:
-                v = empty_v
-                step_in = inv_steps[found]
-                break
-        if v is None:
-            break
-
-        tmap = {}
-        constraints = np.zeros(len(steps))
-        for s_out in steps:
-            target = vertex_apply_change(leg_states, dims, v, step_in, s_out, index)
-            tmap[s_out] = target
-            constraints[step_idx[s_out] - 1] = weights[target - 1] if target is not None else 0.0
-
-        solution = _solve_lp(cost, A, constraints, lp_tolerance)
-
-        for s_in in steps:
-            if tmap[s_in] is None:
-                continue
-            c = constraints[step_idx[s_in] - 1]
-            norm = 1.0 if c == 0 else c
-            offset = len(cumprobs) + 1
-            length = -1
-            for s_out in steps:
-                var = var_lookup[(s_in, s_out)]
-                x = solution[var]
-                prob = x / norm
-                if not x >= -lp_tolerance:
-                    raise AssertionError(f"LP solution below bound: {x}")
-                if prob < 0.0:
-                    used_inaccurate = prob < -tolerance
-                    prob = 0.0
-                if not x <= norm + lp_tolerance:
-                    raise AssertionError(f"LP solution above bound: {x} > {norm}")
-                if prob > 1.0:
-                    used_inaccurate = prob > 1 + tolerance
-                    prob = 1.0
-                if prob > tolerance / len(steps):
-                    cumprobs.append(prob)
-                    vc = variables[var]
-                    in_inv = inverse(vc)[0] if vc[0] == s_in else vc[0]
-                    t = wrap_vertex_idx(leg_states, tmap[in_inv])
-                    if t is None:
-                        raise AssertionError("transition to invalid vertex")
-                    targets.append(t)
-                    step_outs.append(s_out)
-                    length += 1
-            transitions[(s_in[0], s_in[1], tmap[s_in])] = (offset, length)
-            acc = 0.0
-            for k in range(offset - 1, offset + length):
-                acc = acc + cumprobs[k]
-                cumprobs[k] = acc
-            if length >= 0 and abs(cumprobs[offset + length - 1] - 1) > tolerance:
-                warnings.warn(f"normalization error: {cumprobs[offset + length - 1]} != 1")
-
-    if used_inaccurate:
-        warnings.warn("had to truncate some probabilities in a possibly inaccurate way!")
-    return transitions, cumprobs, targets, step_outs
 
 
 class VertexData:
@@ -287,11 +156,12 @@ class VertexData:
         if lp_solver == "native":
             (self.transitions, self.transition_cumprobs, self.transition_targets,
              self.transition_step_outs, self.lp_flags) = construct_transitions_native(*args)
-        elif lp_solver == "highs":
+        elif callable(lp_solver):
             (self.transitions, self.transition_cumprobs, self.transition_targets,
-             self.transition_step_outs) = construct_transitions(*args)
+             self.transition_step_outs) = lp_solver(*args)
         else:
-            raise ValueError(f"lp_solver {lp_solver!r}: 'native' or 'highs'")
+            raise ValueError(f"lp_solver {lp_solver!r}: 'native' or a callable "
+                             "(weights, leg_states, max_worm_count, dims, tolerance, lp_tolerance)")
 
     # accessors (vertex_data.jl:91-104)
     def vertex_count(self):
