@@ -129,17 +129,28 @@ def measured_traffic(kernel, walkers_per_launch):
     """HBM bytes per launch of `kernel` over `walkers_per_launch` walkers of
     the batched 16x16 beta=8 workload, from the committed rocprofv3 counter
     summary of the timed configuration (FETCH_SIZE + WRITE_SIZE,
-    copy-calibrated), or None when it does not cover that launch."""
+    copy-calibrated): (bytes, note), or (None, reason).  The summary was
+    taken at one walker count per launch; at another count (the tuner may
+    pick another group count) the bytes are scaled by the walker count, since
+    every byte a launch moves belongs to one of its walkers (records,
+    operator string, spins)."""
     import re
     path = os.path.join(ROOT, "profiles",
                         "hbm_traffic_timed_configuration_heisenberg16x16_beta8_4096walkers.md")
     try:
         text = open(path).read()
     except OSError:
-        return None
-    m = re.search(rf"traffic: kernel={kernel} walkers_per_launch={walkers_per_launch} "
-                  rf"bytes_per_launch=(\d+)", text)
-    return int(m.group(1)) if m else None
+        return None, "no counter summary"
+    m = re.search(rf"traffic: kernel={kernel} walkers_per_launch=(\d+) bytes_per_launch=(\d+)",
+                  text)
+    if not m:
+        return None, f"counter summary does not cover {kernel}"
+    w0, b0 = int(m.group(1)), int(m.group(2))
+    src = os.path.relpath(path, ROOT)
+    if w0 == walkers_per_launch:
+        return b0, f"{src}: measured at {w0} walkers per launch"
+    return (int(round(b0 * walkers_per_launch / w0)),
+            f"{src}: measured at {w0} walkers per launch, scaled to {walkers_per_launch}")
 
 
 def issue_bound(kernel):
@@ -634,6 +645,9 @@ def main():
         walkers_per_launch = args.walkers // max(1, g)
     log("kernels in the timed configuration: " + ", ".join(
         f"{nm} {v['launches']} x {v['avg_launch_ms']:.3f} ms" for nm, v in per_kernel.items()))
+    traffic = (measured_traffic(kernels[dom], walkers_per_launch)
+               if (args.walkers, args.lattice, args.beta) == (4096, 16, 8.0)
+               else (None, "counter summary covers the 4096-walker 16x16 beta=8 workload only"))
     lds = batch.lds_bytes()
     states = [batch.get_walker(k) for k in range(max(args.plain_walkers, 16))]
     baselines = {}
@@ -704,9 +718,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kernels[dom],
                      "achieved": d["achieved_gbs_per_launch"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": d["achieved_gbs_per_launch"] / HBM_PEAK_GBS,
-                     "traffic": (measured_traffic(kernels[dom], walkers_per_launch)
-                                 if (args.walkers, args.lattice, args.beta) == (4096, 16, 8.0)
-                                 else None),
+                     "traffic": traffic[0], "traffic_source": traffic[1],
                      "algorithmic_bytes_per_launch": d["algorithmic_bytes_per_launch"],
                      "avg_launch_ms": d["avg_launch_ms"], "launches": d["launches"],
                      "aggregate_achieved_gbs": region_bytes[dom] / (diag_ms * 1e-3) / 1e9,
