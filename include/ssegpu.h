@@ -373,8 +373,8 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
  *                instruction issue, as on the batched Heisenberg workload).
  *                Results are identical.
  * SSE_OPT_TUNE (default 1): autotuner.  The first measuring sse_run calls of
- *                a wave-mode batch run their sweeps in turn (one sweep each,
- *                twice round-robin) with the candidate launch forms: stream
+ *                a wave-mode batch run their sweeps in turn (segments of four
+ *                sweeps, twice round-robin) with the candidate launch forms: stream
  *                groups at 2, 4 and hardware-queue-count groups; the
  *                persistent work-queue launch with the worm-kernel LDS sized
  *                from the largest walker or capped at 32 KiB; then the
@@ -385,6 +385,13 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
  *                SSE_OPT_STREAMS, SSE_OPT_PERSISTENT, SSE_OPT_K2_LDS or
  *                SSE_OPT_CHASE turns tuning off; setting SSE_OPT_TUNE restarts
  *                it.  Results are identical for every choice.
+ * SSE_OPT_FUSE_MEASURE (default 0): 1 = measuring sweeps of the per-sweep
+ *                kernels run the worm update and the measurement in one kernel
+ *                (each walker's wave measures right after its own worm update,
+ *                inside the launch's worm tail).  Slower on the batched
+ *                Heisenberg workload: the measuring wave keeps the worm
+ *                kernel's LDS, so fewer walkers chase at a time.  Results are
+ *                identical.
  * SSE_OPT_ORDER (default 1): launch schedule.  1 = walkers in order of their
  *                expected operator count (bonds / T), largest first, so the
  *                longest chains start first and each stream group holds walkers
@@ -399,7 +406,8 @@ enum sse_option {
     SSE_OPT_K2_LDS = 6,
     SSE_OPT_ORDER = 7,
     SSE_OPT_CHASE = 8,
-    SSE_OPT_TUNE = 9
+    SSE_OPT_TUNE = 9,
+    SSE_OPT_FUSE_MEASURE = 10
 };
 int sse_set_option(sse_batch *b, int32_t option, int64_t value);
 

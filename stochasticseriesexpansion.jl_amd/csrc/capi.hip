@@ -45,7 +45,7 @@ int fail(int code, const std::string &msg)
 // while every field is unchanged.  Compared bytewise (constructed zeroed).
 struct GraphKey {
     int32_t kind;       // 1 one stream, 2 stream groups, 3 persistent launch
-    int32_t sweeps, measure, groups, rec_mode, chase, use_order, n_tasks;
+    int32_t sweeps, measure, groups, rec_mode, chase, use_order, n_tasks, fuse;
     size_t lds[9];      // worm-kernel LDS (one stream / per group), persistent LDS
     const void *models, *budgets, *order;
     GraphKey() { memset(this, 0, sizeof *this); }
@@ -1037,6 +1037,8 @@ static void collect_kernel_times(sse_batch *b)
             b->kern_launches[3] += 1;
             continue;
         }
+        // kind 0: K1, K2, K3; 1: K1, K2 (no measure); 4: K1, fused K2 + K3
+        // (counted as the worm kernel)
         (void)hipEventElapsedTime(&a, e[0], e[1]);
         (void)hipEventElapsedTime(&c, e[1], e[2]);
         b->kern_ms[0] += a;
@@ -1053,6 +1055,7 @@ static void collect_kernel_times(sse_batch *b)
 }
 
 static int launch_measure(sse_batch *b, int mode);
+static int launch_sweep_measure(sse_batch *b, int measure, hipEvent_t *ev = nullptr);
 
 // Replay the captured launch sequence of key k, capturing issue() into a
 // HIP graph first when the cached one was made for another key.  No memset
@@ -1099,6 +1102,7 @@ static GraphKey graph_key(const sse_batch *b, int kind, int sweeps, int measure)
     k.measure = measure;
     k.rec_mode = b->wave.rec_mode;
     k.chase = b->wave.chase;
+    k.fuse = b->wave.fuse_measure;
     k.use_order = b->use_order;
     k.models = b->d_models;
     k.lds[0] = b->wave.lds_k2;
@@ -1115,11 +1119,8 @@ static int graph_run(sse_batch *b, int sweeps, int measure)
 {
     return graph_replay(b, graph_key(b, 1, sweeps, measure), [&]() {
         int rc = SSE_OK;
-        for (int k = 0; k < sweeps && !rc; ++k) {
-            rc = launch_sweep(b, 7);
-            if (!rc && measure)
-                rc = launch_measure(b, 1);
-        }
+        for (int k = 0; k < sweeps && !rc; ++k)
+            rc = launch_sweep_measure(b, measure);
         return rc;
     });
 }
@@ -1227,18 +1228,26 @@ static int groups_run(sse_batch *b, int n_sweeps, int measure, bool ktime = fals
                 continue;
             const int32_t *ord = b->use_order ? b->d_order : nullptr;
             hipEvent_t *e = nullptr;
+            const bool fused = measure && b->wave.fuse_measure;
             if (ktime) {
                 const size_t i = ev0 + 4 * ((size_t)k * G + g);
                 e = &b->kevpool[i];
-                b->kern_ev.push_back({measure ? 0 : 1, i});
+                b->kern_ev.push_back({fused ? 4 : measure ? 0 : 1, i});
                 HIPCHECK(hipEventRecord(e[0], gs));
             }
-            int rc = wave_launch_sweep(b->wave, gs, b->d_walkers, b->d_models, b->d_pool, w0, w1, 7,
+            int rc;
+            if (fused) {
+                rc = wave_launch_sweep_measure(b->wave, gs, b->d_walkers, b->d_models, b->d_pool, w0,
+                                               w1, e, ord, g, b->d_out, b->d_sum, b->d_sumsq,
+                                               b->d_count, b->d_wlf, b->d_done);
+            } else {
+                rc = wave_launch_sweep(b->wave, gs, b->d_walkers, b->d_models, b->d_pool, w0, w1, 7,
                                        e, ord, g);
-            if (!rc && measure)
-                rc = wave_launch_measure(b->wave, gs, b->d_walkers, b->d_models, b->d_out, b->d_sum,
-                                         b->d_sumsq, b->d_count, b->d_wlf, w0, w1, 1, b->d_done,
-                                         ord);
+                if (!rc && measure)
+                    rc = wave_launch_measure(b->wave, gs, b->d_walkers, b->d_models, b->d_out,
+                                             b->d_sum, b->d_sumsq, b->d_count, b->d_wlf, w0, w1, 1,
+                                             b->d_done, ord);
+            }
             if (!rc && ktime)
                 rc = hipEventRecord(e[3], gs) == hipSuccess ? SSE_OK : SSE_ERR_HIP;
             if (rc)
@@ -1308,9 +1317,7 @@ static int graph_run_many(sse_batch *b, int n_sweeps, int measure)
         done += chunk;
     }
     for (; done < n_sweeps; ++done) {
-        int rc = launch_sweep(b, 7);
-        if (!rc && measure)
-            rc = launch_measure(b, 1);
+        int rc = launch_sweep_measure(b, measure);
         if (rc)
             return rc;
     }
@@ -1387,9 +1394,7 @@ static int issue_sweeps(sse_batch *b, int n_sweeps, int measure, bool grouped, b
         for (int k = 0; k < n_sweeps; ++k) {
             hipEvent_t *e = &b->evpool[4 * (size_t)(k0 + k)];
             HIPCHECK(hipEventRecord(e[0], b->stream));
-            int rc = launch_sweep(b, 7, e);
-            if (!rc && measure)
-                rc = launch_measure(b, 1);
+            int rc = launch_sweep_measure(b, measure, e);
             if (rc)
                 return rc;
             HIPCHECK(hipEventRecord(e[3], b->stream));
@@ -1399,9 +1404,7 @@ static int issue_sweeps(sse_batch *b, int n_sweeps, int measure, bool grouped, b
     if (b->use_graph)
         return graph_run_many(b, n_sweeps, measure);
     for (int k = 0; k < n_sweeps; ++k) {
-        int rc = launch_sweep(b, 7);
-        if (!rc && measure)
-            rc = launch_measure(b, 1);
+        int rc = launch_sweep_measure(b, measure);
         if (rc)
             return rc;
     }
@@ -1509,8 +1512,8 @@ static void tune_record(sse_batch::Tuner &t, int i, double updates, double secon
 static int issue_sweeps(sse_batch *b, int n_sweeps, int measure, bool grouped, bool timed, int k0);
 static int sweep_call_rest(sse_batch *b, int32_t n_sweeps, int32_t measure, int tuned_sweeps);
 
-// Tuning inside an sse_run call: candidate segments of two sweeps each (one
-// when only one is left; direct launches: no graph capture is timed), the
+// Tuning inside an sse_run call: candidate segments of four sweeps each (fewer
+// when fewer are left; direct launches: no graph capture is timed), the
 // worm-kernel LDS resized from the operator counts between segments, until a
 // choice is made or the call's sweeps run out.  Returns the sweeps it ran, or
 // -(error code).
@@ -1531,7 +1534,9 @@ static int tune_in_call(sse_batch *b, int n_sweeps, int measure)
         rc = work_sum(b, &w0);
         if (rc)
             break;
-        const int k = std::min(2, n_sweeps - used);
+        // four sweeps per segment: a stream-group pipeline needs a few sweeps
+        // past its fill before a segment shows its steady-state rate
+        const int k = std::min(4, n_sweeps - used);
         const auto t0 = std::chrono::steady_clock::now();
         rc = issue_sweeps(b, k, measure, grouped, false, 0);
         if (!rc && hipStreamSynchronize(b->stream) != hipSuccess)
@@ -1681,6 +1686,26 @@ static int launch_measure(sse_batch *b, int mode)
     }
     HIPCHECK(hipGetLastError());
     return SSE_OK;
+}
+
+// One sweep and, with measure, the accumulated measurement of every walker on
+// the batch stream: wave mode with fuse_measure runs K1 and the fused worm +
+// measurement kernel; otherwise the sweep kernels, then the measurement
+// kernel.  ev (optional) as in launch_sweep.
+static int launch_sweep_measure(sse_batch *b, int measure, hipEvent_t *ev)
+{
+    if (measure && b->mode == SSE_MODE_WAVE && b->wave.fuse_measure) {
+        int rc = wave_launch_sweep_measure(b->wave, b->stream, b->d_walkers, b->d_models, b->d_pool, 0,
+                                           b->n_walkers, ev, b->use_order ? b->d_order : nullptr, -1,
+                                           b->d_out, b->d_sum, b->d_sumsq, b->d_count, b->d_wlf,
+                                           b->d_done);
+        HIPCHECK(hipGetLastError());
+        return rc;
+    }
+    int rc = launch_sweep(b, 7, ev);
+    if (!rc && measure)
+        rc = launch_measure(b, 1);
+    return rc;
 }
 
 int32_t sse_measure_width(const sse_batch *b, int32_t walker)
@@ -2450,6 +2475,12 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         HIPCHECK(hipStreamSynchronize(b->stream));
         b->tune = false;
         b->wave.chase = (int)value;
+        return SSE_OK;
+    case SSE_OPT_FUSE_MEASURE:
+        if (value != 0 && value != 1)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_FUSE_MEASURE takes 0 or 1");
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        b->wave.fuse_measure = (int)value;
         return SSE_OK;
     case SSE_OPT_WORM_RECORDS:
         if (value < 0 || value > 3)

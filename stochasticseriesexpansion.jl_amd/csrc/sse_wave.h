@@ -40,6 +40,7 @@ struct WaveConfig {
     int chase = 0;             // LDS-record chase: 0 plain, 1 with record lookahead
     size_t lds_k2_fixed = 0;   // worm-kernel LDS per walker set by the caller (0 = sized here)
     size_t lds_k2_group[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per stream group (0 = lds_k2)
+    int fuse_measure = 0;      // measuring sweeps: worm update + measurement in one kernel
 };
 
 __device__ __forceinline__ uint64_t below_mask(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
@@ -1135,7 +1136,7 @@ __device__ __forceinline__ uint32_t chase_exact(const DevModel &m, const_char *C
 // is selected on the scalar unit, and the closing tests compare packed keys
 // (leg id << 14 | worm-1).  Returns an error code (0 = ok).
 template <class R, bool kOneWorm>
-__device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R &rec,
+__device__ __forceinline__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R &rec,
                           const gmem<uint64_t> *MASK, const gmem<uint32_t> *BASE, double &total,
                           int64_t &scat)
 {
@@ -1243,7 +1244,7 @@ __device__ int worm_chase(DevWalker &w, const DevModel &m, K2Stream &rs, const R
 // with one operator, linked to itself through the periodic closure) patches
 // the code into the picked record.
 template <class R, bool kOneWorm>
-__device__ int worm_chase_la(DevWalker &w, const DevModel &m, K2Stream &rs, const R &rec,
+__device__ __forceinline__ int worm_chase_la(DevWalker &w, const DevModel &m, K2Stream &rs, const R &rec,
                              const gmem<uint64_t> *MASK, const gmem<uint32_t> *BASE, double &total,
                              int64_t &scat)
 {
@@ -1336,7 +1337,7 @@ __device__ int worm_chase_la(DevWalker &w, const DevModel &m, K2Stream &rs, cons
 
 // state refresh (sse.jl:219-228) and operator write-back after the worms
 template <class R, class RS>
-__device__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &rec,
+__device__ __forceinline__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &rec,
                             const gmem<uint64_t> *MASK, const gmem<uint32_t> *BASE)
 {
     const int lane = threadIdx.x;
@@ -1927,6 +1928,27 @@ __global__ __launch_bounds__(64) void wave_measure_kernel(DevWalker *W, const De
         k3_walker(W, models, out, sum, sumsq, count, wlf, walker_at(order, pos), mode, done);
 }
 
+// K2 and the accumulated measurement of the same sweep in one kernel: each
+// walker's wave measures as soon as its own worm update is written back, so the
+// measurement runs inside the launch's worm tail instead of after it.  The
+// measurement reads the operator string, spins and DevWalker that the worm
+// phase of the same wave has just stored, through vector loads after the
+// barrier (nothing of it goes through the scalar cache).
+__global__ __launch_bounds__(64) void wave_worm_measure_kernel(
+    DevWalker *W, const DevModel *models, DevPool *pool, int w_begin, int w_end, int lds_bytes,
+    int rec_mode, const int32_t *order, int chase, double *out, double *sum, double *sumsq,
+    int64_t *count, double *wlf, int32_t *done)
+{
+    extern __shared__ __align__(16) char lds[];
+    const int pos = w_begin + (int)blockIdx.x;
+    if (pos < w_end) {
+        const int wi = walker_at(order, pos);
+        k2_walker<false>(W, models, pool, wi, 1, lds_bytes, rec_mode, lds, chase);
+        __syncthreads();
+        k3_walker(W, models, out, sum, sumsq, count, wlf, wi, 1, done);
+    }
+}
+
 // pending walkers become active once initialised
 __global__ void activate_pending_kernel(DevWalker *W, int n_walkers)
 {
@@ -2042,6 +2064,29 @@ static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, 
     if (phases & 4)
         hipLaunchKernelGGL(wave_worm_kernel, dim3(n), dim3(64), lds2, s, W, M, pool, w0, w1,
                            (phases & 7) == 7 ? 1 : 0, (int)lds2, c.rec_mode, order, c.chase);
+    if (ev)
+        (void)hipEventRecord(ev[2], s);
+    return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
+}
+
+// one measuring sweep of the walkers at schedule positions [w0, w1): K1, then
+// the worm update and measurement in one kernel (ev[1] after K1, ev[2] after
+// the fused kernel)
+static inline int wave_launch_sweep_measure(WaveConfig &c, hipStream_t s, DevWalker *W,
+                                            const DevModel *M, DevPool *pool, int w0, int w1,
+                                            hipEvent_t *ev, const int32_t *order, int grp,
+                                            double *out, double *sum, double *sumsq,
+                                            int64_t *count, double *wlf, int32_t *done)
+{
+    const int n = w1 - w0;
+    const int g = grp >= 0 && grp < 7 ? grp : 7;
+    const size_t lds2 = g < 7 && c.lds_k2_group[g] ? c.lds_k2_group[g] : c.lds_k2;
+    hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, w0, w1, 3,
+                       -1, (int)c.lds_k1, order, g);
+    if (ev)
+        (void)hipEventRecord(ev[1], s);
+    hipLaunchKernelGGL(wave_worm_measure_kernel, dim3(n), dim3(64), lds2, s, W, M, pool, w0, w1,
+                       (int)lds2, c.rec_mode, order, c.chase, out, sum, sumsq, count, wlf, done);
     if (ev)
         (void)hipEventRecord(ev[2], s);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;

@@ -391,6 +391,46 @@ def test_persistent_equals_per_sweep_kernels(sse_pkg, oracle_mod, graph):
     b.close()
 
 
+@pytest.mark.parametrize("streams,graph", [(1, 0), (1, 1), (3, 0)])
+def test_fused_worm_measure_equals_separate(sse_pkg, oracle_mod, streams, graph):
+    """SSE_OPT_FUSE_MEASURE: the worm update and the measurement in one
+    kernel give the separate kernels' walkers and accumulators bit for bit
+    (one stream eager and through graphs, three stream groups), on a batch
+    mixing two models, with sweeps before and after thermalization and a
+    call without measurement in between."""
+    sq = square_heisenberg(sse_pkg, 6).flat()
+    hc = job_honeycomb(sse_pkg, 0.5).flat()
+    wd = [sse_pkg.walker_desc(model=s % 2, T=0.3 + 0.05 * (s % 3), seed=500 + s, thermalization=12)
+          for s in range(12)]
+    _say(f"fused worm+measure: streams={streams} graph={graph}")
+    a = sse_pkg.Batch([sq, hc], wd)
+    b = sse_pkg.Batch([sq, hc], wd)
+    for x in (a, b):
+        x.set_option(sse_pkg.capi.SSE_OPT_STREAMS, streams)
+        x.set_option(sse_pkg.capi.SSE_OPT_GRAPH, graph)
+    a.set_option(sse_pkg.capi.SSE_OPT_FUSE_MEASURE, 1)
+    a.init()
+    b.init()
+    for rep, (k, meas) in enumerate(((20, True), (5, False), (33, True))):
+        a.run(k, measure=meas)
+        b.run(k, measure=meas)
+        _assert_same_walkers(a, b, len(wd), f"call {rep}")
+    for (s1, q1, c1), (s2, q2, c2) in zip(a.accumulators(), b.accumulators()):
+        assert c1 == c2 and np.array_equal(s1, s2) and np.array_equal(q1, q2)
+    for (s1, q1, c1), (s2, q2, c2) in zip(a.worm_fractions(), b.worm_fractions()):
+        assert c1 == c2 and s1 == s2 and q1 == q2
+    o = oracle_mod.OracleWalker(sse_pkg.capi, hc, sse_pkg.walker_desc(
+        T=wd[1].T, seed=wd[1].seed, thermalization=12))
+    o.init()
+    rows = o.run_series(58)  # sweeps 13..58; the call without measurement ran 21..25
+    kept = np.concatenate([rows[:8], rows[13:]])
+    s1, _, c1 = a.accumulators()[1]
+    assert c1 == len(kept) == 41
+    assert np.allclose(s1, kept.sum(axis=0), rtol=1e-12, atol=1e-12)
+    a.close()
+    b.close()
+
+
 def test_persistent_many_walkers(sse_pkg):
     """More walkers than the launch has workgroups: each workgroup takes
     several walkers from the task counter."""
