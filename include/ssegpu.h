@@ -392,6 +392,14 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
  *                Heisenberg workload: the measuring wave keeps the worm
  *                kernel's LDS, so fewer walkers chase at a time.  Results are
  *                identical.
+ * SSE_OPT_LANE_WIDTH: lane mode (SSE_MODE_LANE) runs this many walkers per
+ *                wavefront (1..64; one lane each).  Default: 1 until the batch
+ *                has 16 wavefronts per compute unit, then the power of two
+ *                that keeps it near that.  Fewer walkers
+ *                per wavefront means more wavefronts for the same batch, so
+ *                the compute units hide more of each lane's memory latency,
+ *                and less divergence between the chains of one wavefront.
+ *                Results are identical.
  * SSE_OPT_ORDER (default 1): launch schedule.  1 = walkers in order of their
  *                expected operator count (bonds / T), largest first, so the
  *                longest chains start first and each stream group holds walkers
@@ -407,7 +415,8 @@ enum sse_option {
     SSE_OPT_ORDER = 7,
     SSE_OPT_CHASE = 8,
     SSE_OPT_TUNE = 9,
-    SSE_OPT_FUSE_MEASURE = 10
+    SSE_OPT_FUSE_MEASURE = 10,
+    SSE_OPT_LANE_WIDTH = 11
 };
 int sse_set_option(sse_batch *b, int32_t option, int64_t value);
 

@@ -45,7 +45,7 @@ int fail(int code, const std::string &msg)
 // while every field is unchanged.  Compared bytewise (constructed zeroed).
 struct GraphKey {
     int32_t kind;       // 1 one stream, 2 stream groups, 3 persistent launch
-    int32_t sweeps, measure, groups, rec_mode, chase, use_order, n_tasks, fuse;
+    int32_t sweeps, measure, groups, rec_mode, chase, use_order, n_tasks, fuse, lane_width;
     size_t lds[9];      // worm-kernel LDS (one stream / per group), persistent LDS
     const void *models, *budgets, *order;
     GraphKey() { memset(this, 0, sizeof *this); }
@@ -137,6 +137,7 @@ struct sse_batch {
         int best = -1;             // chosen candidate (-1: still measuring)
     };
     bool tune = true;         // tuning enabled (an explicit launch option turns it off)
+    int lane_width = 64;      // lane mode: walkers per wavefront (threads per workgroup)
     Tuner tune_run, tune_bud; // sse_run/sse_sweep calls; sse_run_budgets calls
     unsigned long long *d_work = nullptr;  // vertex-update sum of a tuning measurement
     double phase_ms[3] = {0, 0, 0};
@@ -905,6 +906,15 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
                 hipSuccess ||
             b->n_cu <= 0)
             b->n_cu = 256;
+        // lane mode: one walker per wavefront until the batch fills 16
+        // wavefronts per compute unit, then more per wavefront (powers of two
+        // up to 64).  On 4096 walkers one per wavefront measured 1.5-1.7x the
+        // rate of 64 per wavefront: a lane's chain waits on its own memory
+        // loads, and more wavefronts per unit hide more of that wait.
+        int lw = 1;
+        while (lw < 64 && (int64_t)n_walkers >= (int64_t)2 * lw * 16 * b->n_cu)
+            lw *= 2;
+        b->lane_width = lw;
     }
     if (rc) {
         std::string keep = g_err;
@@ -954,8 +964,8 @@ static int launch_sweep(sse_batch *b, int phases, hipEvent_t *ev = nullptr)
 {
     int n = b->n_walkers;
     if (b->mode == SSE_MODE_LANE) {
-        int threads = 64;
-        int blocks = (n + threads - 1) / threads;
+        const int threads = b->lane_width;
+        const int blocks = (n + threads - 1) / threads;
         hipLaunchKernelGGL(lane_sweep_kernel, dim3(blocks), dim3(threads), 0, b->stream,
                            b->d_walkers, b->d_models, b->d_pool, 0, n, phases,
                            (const int32_t *)nullptr, 0);
@@ -1103,6 +1113,7 @@ static GraphKey graph_key(const sse_batch *b, int kind, int sweeps, int measure)
     k.rec_mode = b->wave.rec_mode;
     k.chase = b->wave.chase;
     k.fuse = b->wave.fuse_measure;
+    k.lane_width = b->lane_width;
     k.use_order = b->use_order;
     k.models = b->d_models;
     k.lds[0] = b->wave.lds_k2;
@@ -1330,7 +1341,8 @@ static int launch_init_pending(sse_batch *b)
 {
     int n = b->n_walkers;
     if (b->mode == SSE_MODE_LANE) {
-        hipLaunchKernelGGL(lane_init_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream,
+        const int lw = b->lane_width;
+        hipLaunchKernelGGL(lane_init_kernel, dim3((n + lw - 1) / lw), dim3(lw), 0, b->stream,
                            b->d_walkers, b->d_models, b->d_pool, 0, n, 1);
         HIPCHECK(hipGetLastError());
     } else {
@@ -1673,7 +1685,8 @@ static int launch_measure(sse_batch *b, int mode)
 {
     int n = b->n_walkers;
     if (b->mode == SSE_MODE_LANE) {
-        hipLaunchKernelGGL(lane_measure_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream,
+        const int lw = b->lane_width;
+        hipLaunchKernelGGL(lane_measure_kernel, dim3((n + lw - 1) / lw), dim3(lw), 0, b->stream,
                            b->d_walkers, b->d_models, b->d_out, b->d_sum, b->d_sumsq, b->d_count,
                            b->d_wlf, b->d_scratch, b->max_sites, n, mode, b->d_done,
                            (const int32_t *)nullptr, 0);
@@ -1772,19 +1785,20 @@ static int lane_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure
     if (rc)
         return rc;
     HIPCHECK(bmemcpy(b, b->d_budget, bud.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-    const int blocks = (n + 63) / 64;
+    const int lw = b->lane_width;
+    const int blocks = (n + lw - 1) / lw;
     for (int64_t i = 0; i < most; ++i) {
-        hipLaunchKernelGGL(lane_sweep_kernel, dim3(blocks), dim3(64), 0, b->stream, b->d_walkers,
+        hipLaunchKernelGGL(lane_sweep_kernel, dim3(blocks), dim3(lw), 0, b->stream, b->d_walkers,
                            b->d_models, b->d_pool, 0, n, 7, (const int32_t *)b->d_budget, (int)i);
         if (measure)
-            hipLaunchKernelGGL(lane_measure_kernel, dim3(blocks), dim3(64), 0, b->stream,
+            hipLaunchKernelGGL(lane_measure_kernel, dim3(blocks), dim3(lw), 0, b->stream,
                                b->d_walkers, b->d_models, b->d_out, b->d_sum, b->d_sumsq,
                                b->d_count, b->d_wlf, b->d_scratch, b->max_sites, n, 1, b->d_done,
                                (const int32_t *)b->d_budget, (int)i);
         HIPCHECK(hipGetLastError());
     }
     if (!measure && b->any_budget) {
-        hipLaunchKernelGGL(retire_kernel, dim3(blocks), dim3(64), 0, b->stream, b->d_walkers, n,
+        hipLaunchKernelGGL(retire_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_walkers, n,
                            b->d_done);
         HIPCHECK(hipGetLastError());
     }
@@ -2475,6 +2489,12 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         HIPCHECK(hipStreamSynchronize(b->stream));
         b->tune = false;
         b->wave.chase = (int)value;
+        return SSE_OK;
+    case SSE_OPT_LANE_WIDTH:
+        if (value < 1 || value > 64)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_LANE_WIDTH takes 1..64 walkers per wavefront");
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        b->lane_width = (int)value;
         return SSE_OK;
     case SSE_OPT_FUSE_MEASURE:
         if (value != 0 && value != 1)

@@ -107,6 +107,36 @@ def test_lane_mixed_site_counts_run(sse_pkg, oracle_mod):
     batch.close()
 
 
+@pytest.mark.parametrize("width", [1, 7, 64])
+def test_lane_width_equal_results(sse_pkg, width):
+    """SSE_OPT_LANE_WIDTH (walkers per wavefront in lane mode) changes the
+    launch shape only: 40 walkers of two models, 30 measured sweeps, walkers
+    and accumulators equal those of the default shape bit for bit."""
+    flats = [_flat(sse_pkg, "trimer"), square_heisenberg(sse_pkg, 4).flat()]
+    walkers = [sse_pkg.walker_desc(model=k % 2, T=0.3 + 0.1 * (k % 5), seed=800 + k,
+                                   thermalization=6) for k in range(40)]
+    _say(f"lane width {width}")
+    a = sse_pkg.Batch(flats, walkers, mode=LANE)
+    b = sse_pkg.Batch(flats, walkers, mode=LANE)
+    a.set_option(sse_pkg.capi.SSE_OPT_LANE_WIDTH, width)
+    for x in (a, b):
+        x.init()
+        x.run(30, measure=True)
+        x.run_budgets(np.arange(40) % 4, measure=True)
+    for k in range(40):
+        sa, oa, pa = a.get_walker(k)
+        sb, ob, pb = b.get_walker(k)
+        assert np.array_equal(oa, ob) and np.array_equal(pa, pb), k
+        for key in KEYS:
+            assert sa[key] == sb[key], (k, key)
+    for (s1, q1, c1), (s2, q2, c2) in zip(a.accumulators(), b.accumulators()):
+        assert c1 == c2 and np.array_equal(s1, s2) and np.array_equal(q1, q2)
+    with pytest.raises(sse_pkg.capi.SSEError):
+        a.set_option(sse_pkg.capi.SSE_OPT_LANE_WIDTH, 65)
+    a.close()
+    b.close()
+
+
 def test_wave_mode_rejects_other_site_counts(sse_pkg):
     flat = _flat(sse_pkg, "trimer")
     with pytest.raises(sse_pkg.capi.SSEError, match="two-site"):
