@@ -50,6 +50,8 @@ def lib(capi):
     L.so_walker_destroy.argtypes = [vp]
     L.so_init.argtypes = [vp]
     L.so_sweep.argtypes = [vp]
+    L.so_worm_traverse_from.argtypes = [vp, C.c_int64, C.c_int, C.c_int]
+    L.so_worm_traverse_from.restype = C.c_int64
     L.so_sweep_phases.argtypes = [vp, C.c_int]
     L.so_measure.argtypes = [vp, C.POINTER(C.c_double)]
     L.so_measure_width.argtypes = [vp]
@@ -114,6 +116,11 @@ class OracleWalker:
             err = self.L.so_sweep(self.h)
             if err:
                 raise RuntimeError(f"oracle sweep error {err}")
+
+    def worm_traverse_from(self, p0, l0, wf0):
+        """One worm from (slot p0, leg l0, worm wf0), all 0-based but the
+        1-based worm, over a freshly built vertex list; returns its length."""
+        return self.L.so_worm_traverse_from(self.h, p0, l0, wf0)
 
     def sweep_phases(self, phases):
         return self.L.so_sweep_phases(self.h, phases)

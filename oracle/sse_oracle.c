@@ -358,6 +358,8 @@ static int scatter(const sse_vertex_data_desc *vd, uint32_t vcode, int leg_in, i
 }
 
 /* worm_traverse! (sse.jl:233-303); returns worm_length, counts scatter calls */
+static int64_t traverse_from(so_walker *w, int64_t p0, int l0, int wf0, int64_t *scatters);
+
 static int64_t worm_traverse(so_walker *w, int64_t *scatters)
 {
     const sse_model_desc *m = w->m;
@@ -375,7 +377,15 @@ static int64_t worm_traverse(so_walker *w, int64_t *scatters)
     uint32_t b0 = op_bond(w->ops[p0]);
     int site0 = bond_sites_of(m, b0)[l0 % ns];
     int wf0 = 1 + (int)so_range(w->key, &w->ctr, (uint64_t)(m->site_dims[site0] - 1)); /* :251 */
+    return traverse_from(w, p0, l0, wf0, scatters);
+}
 
+/* worm_traverse! from the start (leg l0, worm wf0, operator p0), 0-based leg
+ * and slot (sse.jl:253-303): one scatter per step until the worm closes. */
+static int64_t traverse_from(so_walker *w, int64_t p0, int l0, int wf0, int64_t *scatters)
+{
+    const sse_model_desc *m = w->m;
+    const int ns = nsb_of(m), legs = 2 * ns;
     int leg_in = l0, wf = wf0;
     int64_t p = p0;
     int64_t len = 1;
@@ -484,6 +494,22 @@ int so_sweep_phases(so_walker *w, int phases)
 }
 
 int so_sweep(so_walker *w) { return so_sweep_phases(w, 7); }
+
+/* The reference's worm_traverse! test (test/test_sse.jl:30-60): build the
+ * vertex list of the current operator string, then run one worm from (leg
+ * l0, worm wf0, slot p0), scatters drawn from the walker's stream.  Returns
+ * the worm length, or -1 for a start on an identity slot or an error. */
+int64_t so_worm_traverse_from(so_walker *w, int64_t p0, int l0, int wf0)
+{
+    const int legs = 2 * nsb_of(w->m);
+    if (p0 < 0 || p0 >= w->L || l0 < 0 || l0 >= legs || w->ops[p0] == 0)
+        return -1;
+    so_make_vertex_list(w->m->n_sites, nsb_of(w->m), w->m->bond_sites, w->ops, w->L, w->vert,
+                        w->v_first, w->v_last);
+    int64_t scat = 0;
+    int64_t len = traverse_from(w, p0, l0, wf0, &scat);
+    return w->error ? -1 : len;
+}
 
 int so_measure_width(const so_walker *w) { return 5 + 5 * w->m->n_estimators; }
 

@@ -32,6 +32,7 @@ so_walker *so_walker_create(const sse_model_desc *model, const sse_walker_desc *
 void so_walker_destroy(so_walker *w);
 void so_init(so_walker *w);
 int so_sweep(so_walker *w);
+int64_t so_worm_traverse_from(so_walker *w, int64_t p0, int l0, int wf0);
 int so_sweep_phases(so_walker *w, int phases);
 void so_measure(so_walker *w, double *out);
 int so_measure_width(const so_walker *w);
