@@ -45,7 +45,7 @@ int fail(int code, const std::string &msg)
 // while every field is unchanged.  Compared bytewise (constructed zeroed).
 struct GraphKey {
     int32_t kind;       // 1 one stream, 2 stream groups, 3 persistent launch
-    int32_t sweeps, measure, groups, rec_mode, chase, use_order, n_tasks, fuse, lane_width;
+    int32_t sweeps, measure, groups, rec_mode, chase, use_order, n_tasks, fuse, lane_width, generic;
     size_t lds[9];      // worm-kernel LDS (one stream / per group), persistent LDS
     const void *models, *budgets, *order;
     GraphKey() { memset(this, 0, sizeof *this); }
@@ -720,9 +720,6 @@ int sse_batch_create(const sse_model_desc *models, int32_t n_models, const sse_w
         int rc = validate_model(models[k], k);
         if (rc)
             return rc;
-        if (mode == SSE_MODE_WAVE && models[k].bond_n_sites != 2)
-            return fail(SSE_ERR_INVALID, "sse_batch_create: the wave kernels take two-site bonds; "
-                                         "run models with other bonds in SSE_MODE_LANE");
     }
     for (int w = 0; w < n_walkers; ++w) {
         const sse_walker_desc &wd = walkers[w];
@@ -1114,6 +1111,7 @@ static GraphKey graph_key(const sse_batch *b, int kind, int sweeps, int measure)
     k.chase = b->wave.chase;
     k.fuse = b->wave.fuse_measure;
     k.lane_width = b->lane_width;
+    k.generic = b->wave.generic;
     k.use_order = b->use_order;
     k.models = b->d_models;
     k.lds[0] = b->wave.lds_k2;
@@ -1445,7 +1443,7 @@ static int work_sum(sse_batch *b, unsigned long long *v)
 
 static void tune_apply(sse_batch *b, const sse_batch::TuneCfg &c)
 {
-    b->persistent = c.persistent != 0;
+    b->persistent = c.persistent != 0 && !b->wave.generic;
     b->n_groups = c.groups;
     b->wave.chase = c.chase;
     b->wave.lds_k2_fixed = c.k2_lds;
@@ -1474,8 +1472,12 @@ static void tune_init(sse_batch *b, sse_batch::Tuner &t, bool budgets)
         if (t.cands.empty())
             t.cands.push_back({0, 1, 0, 0});
     }
-    t.cands.push_back({1, 0, 0, 0});
-    t.cands.push_back({1, 0, 0, 32768});
+    if (!b->wave.generic) {  // the generic kernels have no work-queue launch
+        t.cands.push_back({1, 0, 0, 0});
+        t.cands.push_back({1, 0, 0, 32768});
+    }
+    if (t.cands.empty())
+        t.cands.push_back({0, 1, 0, 0});
     t.rate.assign(t.cands.size(), 0.0);
     t.runs.assign(t.cands.size(), 0);
     t.best = -1;
@@ -1762,9 +1764,9 @@ int sse_run(sse_batch *b, int32_t n_sweeps, int32_t measure)
     return sweep_call(b, n_sweeps, measure);
 }
 
-// sse_run_budgets in lane mode: one lane per walker, so the call is a
-// sequence of sweep (+ measure) launches; launch i runs the walkers whose
-// budget exceeds i.
+// sse_run_budgets in lane mode, and in wave mode with bonds of other than two
+// sites (no work-queue launch there): a sequence of sweep (+ measure)
+// launches; launch i runs the walkers whose budget exceeds i.
 static int lane_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure)
 {
     const int n = b->n_walkers;
@@ -1787,14 +1789,27 @@ static int lane_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure
     HIPCHECK(bmemcpy(b, b->d_budget, bud.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     const int lw = b->lane_width;
     const int blocks = (n + lw - 1) / lw;
+    const int32_t *bp = (const int32_t *)b->d_budget;
+    const int32_t *ord = b->use_order ? b->d_order : nullptr;
     for (int64_t i = 0; i < most; ++i) {
-        hipLaunchKernelGGL(lane_sweep_kernel, dim3(blocks), dim3(lw), 0, b->stream, b->d_walkers,
-                           b->d_models, b->d_pool, 0, n, 7, (const int32_t *)b->d_budget, (int)i);
-        if (measure)
-            hipLaunchKernelGGL(lane_measure_kernel, dim3(blocks), dim3(lw), 0, b->stream,
-                               b->d_walkers, b->d_models, b->d_out, b->d_sum, b->d_sumsq,
-                               b->d_count, b->d_wlf, b->d_scratch, b->max_sites, n, 1, b->d_done,
-                               (const int32_t *)b->d_budget, (int)i);
+        if (b->mode == SSE_MODE_LANE) {
+            hipLaunchKernelGGL(lane_sweep_kernel, dim3(blocks), dim3(lw), 0, b->stream, b->d_walkers,
+                               b->d_models, b->d_pool, 0, n, 7, bp, (int)i);
+            if (measure)
+                hipLaunchKernelGGL(lane_measure_kernel, dim3(blocks), dim3(lw), 0, b->stream,
+                                   b->d_walkers, b->d_models, b->d_out, b->d_sum, b->d_sumsq,
+                                   b->d_count, b->d_wlf, b->d_scratch, b->max_sites, n, 1,
+                                   b->d_done, bp, (int)i);
+        } else {  // the generic wave kernels, one walker per workgroup
+            hipLaunchKernelGGL(waven_diag_vlist_kernel, dim3(n), dim3(64), b->wave.lds_k1n, b->stream,
+                               b->d_walkers, b->d_models, b->d_pool, 0, n, 3, -1, ord, 7, bp, (int)i);
+            hipLaunchKernelGGL(waven_worm_kernel, dim3(n), dim3(64), 0, b->stream, b->d_walkers,
+                               b->d_models, b->d_pool, 0, n, 1, ord, bp, (int)i);
+            if (measure)
+                hipLaunchKernelGGL(waven_measure_kernel, dim3(n), dim3(64), 0, b->stream,
+                                   b->d_walkers, b->d_models, b->d_out, b->d_sum, b->d_sumsq,
+                                   b->d_count, b->d_wlf, 0, n, 1, b->d_done, ord, bp, (int)i);
+        }
         HIPCHECK(hipGetLastError());
     }
     if (!measure && b->any_budget) {
@@ -1821,7 +1836,7 @@ int sse_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure)
     if (b->pt_ladders > 0 && b->pt_interval > 0)
         return fail(SSE_ERR_INVALID, "sse_run_budgets: walkers of a temperature ladder advance "
                                      "together (sse_run)");
-    if (b->mode == SSE_MODE_LANE)
+    if (b->mode == SSE_MODE_LANE || b->wave.generic)
         return lane_run_budgets(b, sweeps, measure);
     const int n = b->n_walkers;
     std::vector<int32_t> bud(n);
@@ -1958,8 +1973,8 @@ int sse_run_budgets_noalloc(sse_batch *b, const int64_t *sweeps, int32_t measure
         return fail(SSE_ERR_INVALID, "sse_run_budgets_noalloc: bad arguments");
     if (!b->inited)
         return fail(SSE_ERR_STATE, "run before init");
-    if (b->mode != SSE_MODE_WAVE)
-        return fail(SSE_ERR_INVALID, "sse_run_budgets_noalloc: wave mode only");
+    if (b->mode != SSE_MODE_WAVE || b->wave.generic)
+        return fail(SSE_ERR_INVALID, "sse_run_budgets_noalloc: wave mode with two-site bonds only");
     if (b->n_walkers > SSE_NOALLOC_MAX_WALKERS)
         return fail(SSE_ERR_INVALID, "sse_run_budgets_noalloc: the batch has more walkers than the "
                                      "static scratch holds (SSE_NOALLOC_MAX_WALKERS)");
@@ -2151,16 +2166,19 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links, int64_t li
     }
     std::vector<int64_t> pos;  // wave path: operator rank -> slot
     if (b->mode != SSE_MODE_LANE) {
-        // compact records (rank-indexed) -> the reference's dense layout
+        // compact records (rank-indexed, leg ids PL*rank + leg with PL the leg
+        // count rounded up to a power of two) -> the reference's dense layout
+        const int legs = 2 * b->h_models[w.model].nsb;
+        const int lg = legs <= 2 ? 1 : legs <= 4 ? 2 : 3, pl = 1 << lg;
         int64_t nch = (w.L + 63) / 64;
         const uint32_t *vt = (const uint32_t *)w.links;
         std::vector<uint64_t> mask(nch > 0 ? nch : 1);
-        std::vector<uint32_t> lk(4 * (w.n > 0 ? w.n : 1));
+        std::vector<uint32_t> lk((size_t)pl * (w.n > 0 ? w.n : 1));
         if (nch > 0)
-            HIPCHECK(bmemcpy(b, mask.data(), (const uint64_t *)(vt + 5 * w.cap), 8 * nch,
+            HIPCHECK(bmemcpy(b, mask.data(), (const uint64_t *)(vt + (1 + pl) * w.cap), 8 * nch,
                                hipMemcpyDeviceToHost));
         if (w.n > 0)
-            HIPCHECK(bmemcpy(b, lk.data(), vt + w.cap, 16 * w.n, hipMemcpyDeviceToHost));
+            HIPCHECK(bmemcpy(b, lk.data(), vt + w.cap, 4 * (size_t)pl * w.n, hipMemcpyDeviceToHost));
         for (int64_t c = 0; c < nch; ++c)
             for (int i = 0; i < 64; ++i)
                 if ((mask[c] >> i) & 1ull)
@@ -2168,14 +2186,14 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links, int64_t li
         if ((int64_t)pos.size() != w.n)
             return fail(SSE_ERR_STATE, "vertex list not built for the current operator string");
         auto dense = [&](int32_t leg) -> int32_t {
-            return leg < 0 ? leg : (int32_t)(4 * pos[leg >> 2] + (leg & 3));
+            return leg < 0 ? leg : (int32_t)(legs * pos[leg >> lg] + (leg & (pl - 1)));
         };
-        if (links && links_capacity >= 4 * w.L) {
-            for (int64_t k = 0; k < 4 * w.L; ++k)
+        if (links && links_capacity >= legs * w.L) {
+            for (int64_t k = 0; k < legs * w.L; ++k)
                 links[k] = -1;
             for (int64_t r = 0; r < w.n; ++r)
-                for (int k = 0; k < 4; ++k)
-                    links[4 * pos[r] + k] = dense((int32_t)lk[4 * r + k]);
+                for (int k = 0; k < legs; ++k)
+                    links[legs * pos[r] + k] = dense((int32_t)lk[(size_t)pl * r + k]);
         }
         if (first || last) {
             std::vector<int32_t> f(w.n_sites), l(w.n_sites);
@@ -2253,8 +2271,6 @@ int sse_add_model(sse_batch *b, const sse_model_desc *model, int32_t *index)
     int rc = validate_model(*model, b->n_models);
     if (rc)
         return rc;
-    if (b->mode == SSE_MODE_WAVE && model->bond_n_sites != 2)
-        return fail(SSE_ERR_INVALID, "sse_add_model: the wave kernels take two-site bonds");
     if (model->n_sites > b->max_sites)
         return fail(SSE_ERR_INVALID, "sse_add_model: more sites than the batch was created for");
     HIPCHECK(hipStreamSynchronize(b->stream));
@@ -2279,6 +2295,17 @@ int sse_add_model(sse_batch *b, const sse_model_desc *model, int32_t *index)
     if (((int64_t)dm.n_types << dm.dt_lgp) > 256 && b->wave.small_codes) {
         b->wave.small_codes = false;  // 10-byte worm records from now on: size LDS for them
         wave_resize_lds(b->wave, b->seen_max_n, b->seen_max_L);
+    }
+    if (dm.nsb != 2 || b->wave.generic) {
+        // bonds of other than two sites: the generic wave kernels from now on
+        // (they take two-site models too), never the work-queue launch
+        b->wave.generic = 1;
+        b->wave.lds_k1n = std::max(b->wave.lds_k1n, k1n_lds_bytes(b->max_sites, dm.nsb));
+        if (b->wave.lds_k1n > 65536)
+            (void)hipFuncSetAttribute((const void *)waven_diag_vlist_kernel,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)b->wave.lds_k1n);
+        b->persistent = false;
     }
     *index = b->n_models++;
     return SSE_OK;
@@ -2443,6 +2470,9 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
             return fail(SSE_ERR_INVALID, "SSE_OPT_PERSISTENT takes 0 or 1");
         if (value && b->mode != SSE_MODE_WAVE)
             return fail(SSE_ERR_INVALID, "SSE_OPT_PERSISTENT needs wave mode");
+        if (value && b->wave.generic)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_PERSISTENT: the work-queue launch takes two-site "
+                                         "bonds only");
         HIPCHECK(hipStreamSynchronize(b->stream));
         b->persistent = value != 0;
         b->tune = false;  // the caller chose the launch form

@@ -399,12 +399,16 @@ __device__ __host__ __forceinline__ int64_t slab_capacity(int64_t L)
 }
 
 // Slab of a walker with leg_count legs per operator: ops[cap], then a record
-// region that holds either the compact records of the wave kernels (20 bytes
-// per slot plus the chunk masks and bases) or the dense links of the lane
+// region that holds either the compact records of the wave kernels (4 bytes
+// of vertex code and 4 * PL bytes of links per slot, plus the chunk masks and
+// bases; 20 bytes per slot for two-site bonds) or the dense links of the lane
 // kernels (4 * legs bytes per slot), whichever is larger.
 __device__ __host__ __forceinline__ uint64_t slab_bytes(int64_t cap, int legs = 4)
 {
-    const uint64_t wave = (uint64_t)cap * 20u + ((uint64_t)cap / 64u + 1u) * 16u;
+    // wave layout: vt[cap] and links[PL*cap] (PL = legs rounded up to a power
+    // of two), then the chunk masks and bases
+    const uint64_t pl = legs <= 2 ? 2u : legs <= 4 ? 4u : 8u;
+    const uint64_t wave = (uint64_t)cap * 4u * (1u + pl) + ((uint64_t)cap / 64u + 1u) * 16u;
     const uint64_t lane = (uint64_t)cap * 4u * (uint64_t)legs;
     uint64_t b = (uint64_t)cap * 4u + (wave > lane ? wave : lane);
     return (b + 255u) & ~(uint64_t)255u;

@@ -41,6 +41,8 @@ struct WaveConfig {
     size_t lds_k2_fixed = 0;   // worm-kernel LDS per walker set by the caller (0 = sized here)
     size_t lds_k2_group[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per stream group (0 = lds_k2)
     int fuse_measure = 0;      // measuring sweeps: worm update + measurement in one kernel
+    int generic = 0;           // some model has bonds of other than two sites: sse_wave_n.h kernels
+    size_t lds_k1n = 0;        // LDS of the generic diagonal-update kernel
 };
 
 __device__ __forceinline__ uint64_t below_mask(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
@@ -1962,10 +1964,23 @@ __global__ void activate_pending_kernel(DevWalker *W, int n_walkers)
 }
 
 // ------------------------------------------------------------------ host side
+#include "sse_wave_n.h"
+
 static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &models, int max_sites)
 {
     c.max_sites = max_sites;
     c.lds_k1 = k1_lds_bytes(max_sites);
+    int max_ns = 2;
+    c.generic = 0;
+    for (const DevModel &m : models) {
+        max_ns = std::max(max_ns, (int)m.nsb);
+        if (m.nsb != 2)
+            c.generic = 1;
+    }
+    c.lds_k1n = k1n_lds_bytes(max_sites, max_ns);
+    if (c.generic && c.lds_k1n > 65536)
+        (void)hipFuncSetAttribute((const void *)waven_diag_vlist_kernel,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_k1n);
     // stage each model's lookups in LDS if that keeps K1 within 32 KiB (high
     // occupancy matters more than the lookups of larger models)
     for (const DevModel &m : models) {
@@ -1978,7 +1993,7 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
         hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
         lim > 0)
         c.lds_limit = (size_t)lim;
-    if (c.lds_k1 > c.lds_limit)
+    if (c.lds_k1 > c.lds_limit || (c.generic && c.lds_k1n > c.lds_limit))
         return SSE_ERR_INVALID;
     c.small_codes = true;
     for (const DevModel &m : models) {
@@ -2037,9 +2052,14 @@ static inline int wave_launch_init(WaveConfig &c, hipStream_t s, DevWalker *W, c
                                    DevPool *pool, int n, int max_warmup)
 {
     hipLaunchKernelGGL(wave_init_kernel, dim3(n), dim3(64), 0, s, W, M, n);
-    for (int r = 0; r < max_warmup; ++r)
-        hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, 0, n, 1, r,
-                           (int)c.lds_k1, (const int32_t *)nullptr, 0);
+    for (int r = 0; r < max_warmup; ++r) {
+        if (c.generic)
+            hipLaunchKernelGGL(waven_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1n, s, W, M, pool, 0,
+                               n, 1, r, (const int32_t *)nullptr, 0, (const int32_t *)nullptr, 0);
+        else
+            hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, 0, n,
+                               1, r, (int)c.lds_k1, (const int32_t *)nullptr, 0);
+    }
     hipLaunchKernelGGL(activate_pending_kernel, dim3((n + 63) / 64), dim3(64), 0, s, W, n);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
@@ -2054,6 +2074,19 @@ static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, 
     const int n = w1 - w0;
     const int k1 = phases & (1 | 2 | 8);
     const int g = grp >= 0 && grp < 7 ? grp : 7;  // slot 7 collects the rest
+    if (c.generic) {
+        if (k1 & 3)
+            hipLaunchKernelGGL(waven_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1n, s, W, M, pool, w0,
+                               w1, k1, -1, order, g, (const int32_t *)nullptr, 0);
+        if (ev)
+            (void)hipEventRecord(ev[1], s);
+        if (phases & 4)
+            hipLaunchKernelGGL(waven_worm_kernel, dim3(n), dim3(64), 0, s, W, M, pool, w0, w1,
+                               (phases & 7) == 7 ? 1 : 0, order, (const int32_t *)nullptr, 0);
+        if (ev)
+            (void)hipEventRecord(ev[2], s);
+        return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
+    }
     const size_t lds2 = g < 7 && c.lds_k2_group[g] ? c.lds_k2_group[g] : c.lds_k2;
     if (k1 & 3) {
         hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, w0, w1, k1,
@@ -2080,6 +2113,13 @@ static inline int wave_launch_sweep_measure(WaveConfig &c, hipStream_t s, DevWal
 {
     const int n = w1 - w0;
     const int g = grp >= 0 && grp < 7 ? grp : 7;
+    if (c.generic) {  // no fused form: sweep, then measure
+        int rc = wave_launch_sweep(c, s, W, M, pool, w0, w1, 7, ev, order, grp);
+        if (!rc)
+            hipLaunchKernelGGL(waven_measure_kernel, dim3(n), dim3(64), 0, s, W, M, out, sum, sumsq,
+                               count, wlf, w0, w1, 1, done, order, (const int32_t *)nullptr, 0);
+        return rc ? rc : (hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP);
+    }
     const size_t lds2 = g < 7 && c.lds_k2_group[g] ? c.lds_k2_group[g] : c.lds_k2;
     hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, w0, w1, 3,
                        -1, (int)c.lds_k1, order, g);
@@ -2092,13 +2132,17 @@ static inline int wave_launch_sweep_measure(WaveConfig &c, hipStream_t s, DevWal
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
 
-static inline int wave_launch_measure(WaveConfig &, hipStream_t s, DevWalker *W, const DevModel *M,
+static inline int wave_launch_measure(WaveConfig &c, hipStream_t s, DevWalker *W, const DevModel *M,
                                       double *out, double *sum, double *sumsq, int64_t *count,
                                       double *wlf, int w0, int w1, int mode, int32_t *done,
                                       const int32_t *order = nullptr)
 {
-    hipLaunchKernelGGL(wave_measure_kernel, dim3(w1 - w0), dim3(64), 0, s, W, M, out, sum, sumsq, count,
-                       wlf, w0, w1, mode, done, order);
+    if (c.generic)
+        hipLaunchKernelGGL(waven_measure_kernel, dim3(w1 - w0), dim3(64), 0, s, W, M, out, sum, sumsq,
+                           count, wlf, w0, w1, mode, done, order, (const int32_t *)nullptr, 0);
+    else
+        hipLaunchKernelGGL(wave_measure_kernel, dim3(w1 - w0), dim3(64), 0, s, W, M, out, sum, sumsq,
+                           count, wlf, w0, w1, mode, done, order);
     return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;
 }
 
