@@ -63,9 +63,13 @@ enum sse_status {
  * A bond type acts on n_sites sites (VertexData{NSites}, 1 <= NSites <= 4),
  * so its vertices have leg_count = 2 * n_sites legs, 0-based: legs
  * 0..n_sites-1 are the lower legs of bond sites 0..n_sites-1 (state before
- * the operator), n_sites..2*n_sites-1 the upper legs (state after).  The
- * wave kernels (SSE_MODE_WAVE) take two-site bonds; the lane kernels
- * (SSE_MODE_LANE) any n_sites.                                              */
+ * the operator), n_sites..2*n_sites-1 the upper legs (state after).  Both
+ * kernel families take any n_sites.  In wave mode a batch whose models all
+ * have two-site bonds runs the two-site kernels (LDS records, the chase
+ * table, the work-queue launch); a batch with any other model runs every
+ * walker through the generic wave kernels, which have no work-queue launch
+ * (SSE_OPT_PERSISTENT and sse_run_budgets_noalloc are refused;
+ * sse_run_budgets issues one launch per sweep index).                      */
 #define SSE_MAX_BOND_SITES 4
 typedef struct sse_vertex_data_desc {
     int32_t n_sites;                 /* NSites, 1..SSE_MAX_BOND_SITES           */
