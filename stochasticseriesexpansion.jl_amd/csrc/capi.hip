@@ -461,6 +461,58 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
         }
     }
     size_t o_dtab = put(dt.data(), sizeof(DirectEntry) * dt.size());
+    // the generic wave kernels' chase table (every bond size): the same
+    // entries with PL = 2^glg legs per (ecode, worm) and gen_out exit words
+    const int glg = legs <= 2 ? 1 : legs <= 4 ? 2 : 3;
+    std::vector<DirectEntry> gt((size_t)m.n_types * P * dmw << glg);
+    for (auto &x : gt) {
+        for (int j = 0; j < 4; ++j) {
+            x.K[j] = 0;
+            x.out[j] = kChaseSlow;
+        }
+        x.count = 0;
+        x.trans = 0xFFFFFFFFu;
+        x.pad[0] = x.pad[1] = 0;
+    }
+    {
+        auto thr = [](double c) {
+            return c <= 0.0 ? 0ull : c >= 1.0 ? (1ull << 53) : (uint64_t)std::ceil(std::ldexp(c, 53));
+        };
+        for (int t = 0; t < m.n_types; ++t) {
+            const sse_vertex_data_desc &vd = m.types[t];
+            for (int v = 0; v < vd.n_vertices; ++v)
+                for (int wv = 0; wv < vd.max_worm_count; ++wv)
+                    for (int l = 0; l < legs; ++l) {
+                        int src = (v * vd.max_worm_count + wv) * legs + l;
+                        int off = vd.transition_offset[src], cnt = vd.transition_count[src];
+                        if (off < 0 || cnt <= 0)
+                            continue;
+                        DirectEntry x;
+                        x.count = (uint32_t)cnt;
+                        x.trans = (uint32_t)off;  // first exit in the type's own exit list
+                        x.pad[0] = x.pad[1] = 0;
+                        auto word = [&](int j) {
+                            const uint32_t ec = (uint32_t)t << lgp | (uint32_t)vd.exit_target[off + j];
+                            return gen_out(ec, vd.exit_leg[off + j], vd.exit_worm[off + j]);
+                        };
+                        for (int j = 0; j < 4; ++j) {
+                            x.K[j] = j < cnt ? thr(vd.exit_cumprob[off + j]) : (1ull << 53);
+                            x.out[j] = j < cnt ? word(j) : kChaseSlow;
+                        }
+                        if (cnt > 4) {
+                            x.K[3] = 1ull << 53;
+                            x.out[3] = kChaseSlow;
+                        } else if (cnt == 4) {
+                            x.pad[0] = x.out[3];
+                            if (x.K[3] < (1ull << 53))
+                                x.out[3] = kChaseSlow;
+                        }
+                        for (int dg = 0; dg < 2; ++dg)
+                            gt[(((((size_t)t << lgp) + (2 * v + dg)) * dmw + wv) << glg) + l] = x;
+                    }
+        }
+    }
+    size_t o_gtab = put(gt.data(), sizeof(DirectEntry) * gt.size());
     // diagonal-update table (layout: DevModel::k1tab)
     std::vector<char> k1((size_t)16 * m.n_types);
     for (int t = 0; t < m.n_types && ns == 2; ++t) {
@@ -547,6 +599,9 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
     dm.k1tab = (const uint8_t *)(d + o_k1);
     dm.k1tab_bytes = (int32_t)k1.size();
     dm.pad_k1 = 0;
+    dm.gtab = (const DirectEntry *)(d + o_gtab);
+    dm.gtab_bytes = (int32_t)(sizeof(DirectEntry) * gt.size());
+    dm.g_lg = glg;
     return SSE_OK;
 }
 
@@ -1803,8 +1858,9 @@ static int lane_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure
         } else {  // the generic wave kernels, one walker per workgroup
             hipLaunchKernelGGL(waven_diag_vlist_kernel, dim3(n), dim3(64), b->wave.lds_k1n, b->stream,
                                b->d_walkers, b->d_models, b->d_pool, 0, n, 3, -1, ord, 7, bp, (int)i);
-            hipLaunchKernelGGL(waven_worm_kernel, dim3(n), dim3(64), 0, b->stream, b->d_walkers,
-                               b->d_models, b->d_pool, 0, n, 1, ord, bp, (int)i);
+            hipLaunchKernelGGL(waven_worm_kernel, dim3(n), dim3(64), b->wave.lds_k2n, b->stream,
+                               b->d_walkers, b->d_models, b->d_pool, 0, n, 1, ord, bp, (int)i,
+                               (int)b->wave.lds_k2n);
             if (measure)
                 hipLaunchKernelGGL(waven_measure_kernel, dim3(n), dim3(64), 0, b->stream,
                                    b->d_walkers, b->d_models, b->d_out, b->d_sum, b->d_sumsq,

@@ -71,6 +71,11 @@ struct DevModel {
     //   [dims0*dims1] (or -1), u8 leg states [nv][4]; offsets from the table start
     const uint8_t *k1tab;
     int32_t k1tab_bytes, pad_k1;
+    // chase table of the generic wave kernels (any bond size): one entry per
+    // (ecode, worm-1, leg_in) at ((ecode*dt_mw + worm-1) << g_lg) + leg_in,
+    // 2^g_lg = leg count rounded up to a power of two; exit words gen_out()
+    const struct DirectEntry *gtab;
+    int32_t gtab_bytes, g_lg;
 };
 
 // Exit j is taken by the first j with r < cum[j], r = (word >> 11) * 2^-53.
@@ -98,6 +103,15 @@ struct __align__(16) DirectEntry {
 // model check keeps every worm count below 8192).
 #define kChaseSlow 0xFFFFFFFFu
 #define kChaseSlowBit 0x2000u
+
+// Exit word of the generic chase table: worm-1 in bits 0..11, exit leg in
+// bits 12..14, target ecode in bits 16..31; bit 15 is set only in the slow
+// marker kChaseSlow (worm counts stay below 4096, legs below 8).
+__host__ __device__ inline uint32_t gen_out(uint32_t ecode, uint32_t leg, uint32_t worm)
+{
+    return (worm - 1u) | (leg << 12) | (ecode << 16);
+}
+#define kGenSlowBit 0x8000u
 
 // exit word of the chase: worm-1 in bits 0..13, exit leg in 14..15, target
 // ecode (type << dt_lgp | vcode) in 16..31

@@ -43,6 +43,7 @@ struct WaveConfig {
     int fuse_measure = 0;      // measuring sweeps: worm update + measurement in one kernel
     int generic = 0;           // some model has bonds of other than two sites: sse_wave_n.h kernels
     size_t lds_k1n = 0;        // LDS of the generic diagonal-update kernel
+    size_t lds_k2n = 0;        // LDS of the generic worm kernel (chase codes, 2 bytes per operator)
 };
 
 __device__ __forceinline__ uint64_t below_mask(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
@@ -2002,6 +2003,7 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
             c.small_codes = false;
     }
     c.lds_k2 = 4096;
+    c.lds_k2n = 4096;
     if (c.lds_limit > 65536) {
         (void)hipFuncSetAttribute((const void *)wave_worm_kernel,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_limit);
@@ -2046,6 +2048,10 @@ static inline size_t wave_k2_lds_for(const WaveConfig &c, unsigned long long max
 static inline void wave_resize_lds(WaveConfig &c, unsigned long long max_n, unsigned long long max_L)
 {
     c.lds_k2 = wave_k2_lds_for(c, max_n, max_L);
+    // generic worm kernel: the chase codes of the largest walker and some
+    // headroom, at most 64 KiB (larger walkers chase the global records)
+    size_t g = (2 * ((size_t)max_n + 64) + 255) & ~(size_t)255;
+    c.lds_k2n = g > 65536 ? 65536 : g;
 }
 
 static inline int wave_launch_init(WaveConfig &c, hipStream_t s, DevWalker *W, const DevModel *M,
@@ -2081,8 +2087,9 @@ static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, 
         if (ev)
             (void)hipEventRecord(ev[1], s);
         if (phases & 4)
-            hipLaunchKernelGGL(waven_worm_kernel, dim3(n), dim3(64), 0, s, W, M, pool, w0, w1,
-                               (phases & 7) == 7 ? 1 : 0, order, (const int32_t *)nullptr, 0);
+            hipLaunchKernelGGL(waven_worm_kernel, dim3(n), dim3(64), c.lds_k2n, s, W, M, pool, w0, w1,
+                               (phases & 7) == 7 ? 1 : 0, order, (const int32_t *)nullptr, 0,
+                               (int)c.lds_k2n);
         if (ev)
             (void)hipEventRecord(ev[2], s);
         return hipGetLastError() == hipSuccess ? SSE_OK : SSE_ERR_HIP;

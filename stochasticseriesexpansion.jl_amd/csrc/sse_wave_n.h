@@ -513,6 +513,115 @@ template <int NS>
 __device__ __forceinline__ int64_t k2n_worm(const DevWalker &w, const DevModel &m, K2Stream &rs,
                                             gmem<uint32_t> *gvt, const gmem<uint32_t> *glk,
                                             const gmem<uint64_t> *gmask,
+                                            const gmem<uint32_t> *gbase, int64_t &scat);
+
+// The same worm with the chase codes (ecode = type << lgp | vcode) in LDS and
+// one chase-table entry per step (DevModel::gtab): the exit is picked by
+// comparing the step's 53-bit mantissa with the entry's integer thresholds
+// (r < cum exactly when (word >> 11) < ceil(cum * 2^53)); entries marked slow
+// (a fourth exit with a threshold below 2^53, more than four exits) take the
+// exact search.
+template <int NS>
+__device__ __forceinline__ int64_t k2n_worm_lds(const DevWalker &w, const DevModel &m, K2Stream &rs,
+                                                lds_u16 *lec, const gmem<uint32_t> *glk,
+                                                const gmem<uint64_t> *gmask,
+                                                const gmem<uint32_t> *gbase, int64_t &scat)
+{
+    constexpr int LEGS = LegsOf<NS>::LEGS, PL = LegsOf<NS>::PL, LG = LegsOf<NS>::LG;
+    if (w.n == 0)
+        return 0;
+    const int64_t L = w.L;
+    const uint32_t lgp = uni((uint32_t)m.dt_lgp), dmw = uni((uint32_t)m.dt_mw);
+    const_char *GT = (const_char *)(const void *)uni64((uint64_t)(const void *)m.gtab);
+    int64_t p0;
+    uint32_t l0, r0;
+    for (;;) {  // sse.jl:241-247
+        p0 = (int64_t)uni64(rs.range((uint64_t)L));
+        l0 = uni((uint32_t)rs.range((uint64_t)LEGS));
+        const uint64_t mk = uni64(gmask[p0 >> 6]);
+        const int bit = (int)(p0 & 63);
+        if ((mk >> bit) & 1ull) {
+            r0 = uni(gbase[p0 >> 6] + (uint32_t)popc(mk & below_mask(bit)));
+            break;
+        }
+    }
+    const uint32_t b0 = uni(op_bond(w.ops[p0]));
+    const int site0 = (int)uni((uint32_t)m.bond_sites[(size_t)NS * b0 + l0 % NS]);
+    const uint32_t dim0 = uni((uint32_t)m.site_dims[site0]);
+    const uint32_t wf0 = 1u + uni((uint32_t)rs.range((uint64_t)(dim0 - 1)));  // sse.jl:251
+    uint32_t r = r0, leg_in = l0, wf = wf0;
+    int64_t len = 1;
+    for (;;) {  // sse.jl:274-300
+        // the links of all legs of this operator, lane j holding leg j's
+        // (one vector load, in flight while the exit is picked)
+        const uint32_t lkv = threadIdx.x < (unsigned)LEGS ? glk[PL * r + threadIdx.x] : 0u;
+        const uint32_t e = uni((uint32_t)lec[r]);
+        const uint32_t eix = ((e * dmw + (wf - 1u)) << LG) + leg_in;
+        ConstEntry *E = (ConstEntry *)(GT + (size_t)eix * sizeof(DirectEntry));
+        const uint64_t mm = rs.next() >> 11;
+        scat += 1;
+        uint32_t eo;
+        if (mm < E->K[0])
+            eo = E->out[0];
+        else if (mm < E->K[1])
+            eo = E->out[1];
+        else if (mm < E->K[2])
+            eo = E->out[2];
+        else
+            eo = E->out[3];
+        eo = uni(eo);
+        if (eo & kGenSlowBit) {
+            const uint32_t cnt = uni(E->count);
+            if (cnt == 0u)
+                return -SSE_ERR_SCATTER;
+            if (cnt <= 4u) {
+                uint32_t j = 0;
+                for (; j < cnt; ++j)
+                    if (mm < uni64(E->K[j]))
+                        break;
+                if (j >= cnt)
+                    return -SSE_ERR_SCATTER;
+                eo = uni(j < 3u ? E->out[j] : E->pad[0]);
+            } else {
+                const uint32_t type = e >> lgp;
+                const DevVD &vd = m.types[type];
+                const double rr = (double)mm * 0x1.0p-53;
+                const int32_t off = (int32_t)uni(E->trans);
+                int32_t x = off;
+                for (; x < off + (int32_t)cnt; ++x) {
+                    const double c = __longlong_as_double((long long)uni64(
+                        (uint64_t)__double_as_longlong(vd.cum[x])));
+                    if (rr < c)
+                        break;
+                }
+                if (x >= off + (int32_t)cnt)
+                    return -SSE_ERR_SCATTER;
+                eo = gen_out((type << lgp) | uni((uint32_t)vd.tgt[x]), uni((uint32_t)vd.eleg[x]),
+                             uni((uint32_t)vd.ewrm[x]));
+            }
+        }
+        const uint32_t leg_out = (eo >> 12) & 7u;
+        const uint32_t wf_out = (eo & 0xFFFu) + 1u;
+        lec[r] = (uint16_t)(eo >> 16);  // every lane writes the same value
+        if (r == r0 && leg_out == l0 && wf_out == dim0 - wf0)
+            break;
+        len += 1;
+        wf = wf_out;
+        const uint32_t nx = (uint32_t)__builtin_amdgcn_readlane((int)lkv, (int)leg_out);
+        leg_in = nx & (uint32_t)(PL - 1);
+        r = nx >> LG;
+        if (r >= (uint32_t)w.n || leg_in >= (uint32_t)LEGS)
+            return -SSE_ERR_STATE;
+        if (r == r0 && leg_in == l0 && wf == wf0)
+            break;
+    }
+    return len;
+}
+
+template <int NS>
+__device__ __forceinline__ int64_t k2n_worm(const DevWalker &w, const DevModel &m, K2Stream &rs,
+                                            gmem<uint32_t> *gvt, const gmem<uint32_t> *glk,
+                                            const gmem<uint64_t> *gmask,
                                             const gmem<uint32_t> *gbase, int64_t &scat)
 {
     constexpr int LEGS = LegsOf<NS>::LEGS, PL = LegsOf<NS>::PL, LG = LegsOf<NS>::LG;
@@ -583,7 +692,7 @@ __device__ __forceinline__ int64_t k2n_worm(const DevWalker &w, const DevModel &
 // write-back and the controller.
 template <int NS>
 __device__ void k2n_walker(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
-                           int full_sweep)
+                           int full_sweep, char *lds, int lds_bytes)
 {
     constexpr int LEGS = LegsOf<NS>::LEGS, PL = LegsOf<NS>::PL, LG = LegsOf<NS>::LG;
     const int lane = threadIdx.x;
@@ -602,13 +711,36 @@ __device__ void k2n_walker(DevWalker *W, const DevModel *models, DevPool *pool, 
     int64_t scat = 0;
     int err = 0;
     const int64_t nworms = (int64_t)ceil(w.num_worms);
-    for (int64_t k = 0; k < nworms; ++k) {
-        const int64_t len = k2n_worm<NS>(w, m, rs, gvt, glk, gmask, gbase, scat);
-        if (len < 0) {
-            err = (int)-len;
-            break;
+    // chase codes in LDS when they fit and the codes fit 16 bits
+    const bool in_lds = (int64_t)2 * w.n <= (int64_t)lds_bytes &&
+                        ((int64_t)m.n_types << m.dt_lgp) <= 65536;
+    if (in_lds) {
+        lds_u16 *lec = (lds_u16 *)(uint16_t *)lds;
+        const uint32_t lgp = (uint32_t)m.dt_lgp;
+        for (int64_t r = lane; r < w.n; r += 64)
+            lec[r] = (uint16_t)ecode_of_vt(gvt[r], lgp);
+        __syncthreads();
+        for (int64_t k = 0; k < nworms; ++k) {
+            const int64_t len = k2n_worm_lds<NS>(w, m, rs, lec, glk, gmask, gbase, scat);
+            if (len < 0) {
+                err = (int)-len;
+                break;
+            }
+            total += (double)len;
         }
-        total += (double)len;
+        __syncthreads();
+        if (!err)
+            for (int64_t r = lane; r < w.n; r += 64)
+                gvt[r] = vt_of_ecode(lec[r], lgp);
+    } else {
+        for (int64_t k = 0; k < nworms; ++k) {
+            const int64_t len = k2n_worm<NS>(w, m, rs, gvt, glk, gmask, gbase, scat);
+            if (len < 0) {
+                err = (int)-len;
+                break;
+            }
+            total += (double)len;
+        }
     }
     __syncthreads();
     if (!err) {
@@ -869,8 +1001,10 @@ __global__ __launch_bounds__(64) void waven_diag_vlist_kernel(DevWalker *W, cons
 __global__ __launch_bounds__(64) void waven_worm_kernel(DevWalker *W, const DevModel *models,
                                                         DevPool *pool, int w_begin, int w_end,
                                                         int full_sweep, const int32_t *order,
-                                                        const int32_t *budget, int step)
+                                                        const int32_t *budget, int step,
+                                                        int lds_bytes)
 {
+    extern __shared__ __align__(16) char lds[];
     const int pos = w_begin + (int)blockIdx.x;
     if (pos >= w_end)
         return;
@@ -878,7 +1012,7 @@ __global__ __launch_bounds__(64) void waven_worm_kernel(DevWalker *W, const DevM
     if (budget && step >= budget[wi])
         return;
     const int ns = models[W[wi].model].nsb;
-#define K2N_CALL(K) k2n_walker<K>(W, models, pool, wi, full_sweep)
+#define K2N_CALL(K) k2n_walker<K>(W, models, pool, wi, full_sweep, lds, lds_bytes)
     SSE_NS_DISPATCH(ns, K2N_CALL)
 #undef K2N_CALL
 }
