@@ -8,9 +8,13 @@
 //   K1n  diagonal update + vertex list, 64 slots per step (as K1: the same
 //        draws, acceptance windows and slot-by-slot fallback), with one
 //        per-site lane mask per bond position;
-//   K2n  worm update, wave-uniform chase over the compact records in global
-//        memory, each scatter the exact exit search of vertex_data.jl:106-125
-//        over the VertexData tables;
+//   K2n  worm update, a wave-uniform chase: the chase codes in LDS (2 bytes
+//        per operator), one chase-table entry per step (DevModel::gtab,
+//        integer thresholds; the exact exit search of vertex_data.jl:106-125
+//        where the entry says so), the operator's links read from global
+//        memory by one vector load while the exit is picked; walkers whose
+//        codes do not fit the launch's LDS chase with every table read from
+//        global memory;
 //   K3n  measurement (sign and the magnetization estimators), 64 slots per step.
 // Compact records of a walker with PL = 2*NS rounded up to a power of two:
 //   vt[cap]      vertex code | type << 10 of the operator of each rank
