@@ -350,6 +350,47 @@ def extra_ladder(sse, walkers=1024, thermalization=500, steps=20, seed=4):
     return r
 
 
+def _trimer_ring_flat(sse, n, J=-1.0, h=0.15):
+    """Spin-1/2 ring whose bonds are three consecutive sites (SSEBond{3}):
+    H_b = J (S_a.S_b + S_b.S_c + S_a.S_c) - (h/3) sum S^z, ferromagnetic, with
+    a magnetization estimator."""
+    from sse_amd.sse_data import FlatModel, SSEBond, SSEData
+    sz = np.diag([0.5, -0.5])
+    sp = np.array([[0.0, 1.0], [0.0, 0.0]])
+    eye = np.eye(2)
+
+    def lift(op, k):
+        mats = [eye, eye, eye]
+        mats[k] = op
+        return np.kron(np.kron(mats[0], mats[1]), mats[2])
+
+    Hb = np.zeros((8, 8))
+    for a, b in ((0, 1), (1, 2), (0, 2)):
+        Hb += J * (lift(sz, a) @ lift(sz, b)
+                   + 0.5 * (lift(sp, a) @ lift(sp.T, b) + lift(sp.T, a) @ lift(sp, b)))
+    for a in range(3):
+        Hb -= (h / 3.0) * lift(sz, a)
+    vd = sse.VertexData((2, 2, 2), Hb)
+    bonds = [SSEBond(1, (i + 1, (i + 1) % n + 1, (i + 2) % n + 1)) for i in range(n)]
+
+    class _Est:
+        prefix = ""
+    mval = np.tile(np.array([0.5, -0.5]), (1, n, 1))
+    return FlatModel(SSEData([vd], bonds), [_Est()], n, (np.ones((1, n)), mval))
+
+
+def extra_trimer(sse, walkers=4096, thermalization=1000, steps=20):
+    """Bonds of three sites through the generic wave kernels: 4096 walkers of
+    a 256-site trimer ring at beta = 8."""
+    flat = _trimer_ring_flat(sse, 256)
+    wds = [sse.walker_desc(T=0.125, seed=300000 + k, thermalization=thermalization)
+           for k in range(walkers)]
+    r = run_batch(sse, [flat], wds, thermalization, steps, "trimer")
+    r["workload"] = ("three-site bonds (SSEBond{3}): 256-site spin-1/2 trimer ring, J=-1, h=0.15, "
+                     "4096 walkers, beta=8, generic wave kernels")
+    return r
+
+
 def serving_stream(sse, seconds=20.0, count_for=60.0, sweep_scale=1.0, slots=1024):
     """Open-loop Poisson stream (200 jobs/s, seed 7) of Heisenberg/XXZ jobs on
     L in 8..48, beta log-uniform [1,32], sweeps log-uniform [1e3,1e5] (times
@@ -563,7 +604,7 @@ def main():
                     help="arrival window of the serving stream (0 skips it); results are "
                          "counted for three times as long")
     ap.add_argument("--extra", action="store_true",
-                    help="also run the heterogeneous and ladder workloads")
+                    help="also run the heterogeneous, ladder and three-site-bond workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); walkers are independent "
@@ -740,7 +781,8 @@ def main():
             out["serving"] = {"error": f"{type(e).__name__}: {e}"}
     if args.extra:
         extras = {}
-        for name, fn in (("heterogeneous", extra_heterogeneous), ("ladder", extra_ladder)):
+        for name, fn in (("heterogeneous", extra_heterogeneous), ("ladder", extra_ladder),
+                         ("trimer", extra_trimer)):
             log(f"extra workload: {name}")
             extras[name] = fn(sse)
         out["extra_workloads"] = extras
