@@ -235,12 +235,15 @@ def tune(batch, what, max_calls=12):
     return None
 
 
-def parallelism(choice, walkers):
+def parallelism(choice, walkers, generic=False):
     if not choice:
         return "one wavefront per walker (untuned defaults)"
     chase = "record-lookahead chase" if choice["chase"] else "scalar chase"
     lds = ("worm LDS sized from the largest walker" if not choice["k2_lds"]
            else f"worm LDS {choice['k2_lds']} B")
+    if generic:
+        chase = "chase codes in LDS"
+        lds = "worm LDS sized from the largest walker"
     if choice["persistent"]:
         return (f"autotuned: persistent work-queue launch (one workgroup per walker task, "
                 f"{walkers} tasks per call, HIP-graph replay), {lds}, {chase}")
@@ -282,12 +285,13 @@ def run_batch(sse, flats, wds, thermalization, steps, what, budgets=None):
     dt = time.perf_counter() - t1
     Ls, ns, Ws, wsw = batch.work_counters()
     batch.close()
+    generic = any(f.bond_n_sites != 2 for f in flats)
     return {"vertex_updates_per_s": (Ls + Ws) / dt, "sweeps_per_s": wsw / dt,
             "walkers": len(wds), "thermalization_sweeps": thermalization,
             "timed": (f"{steps} sweeps of every walker" if budgets is None else
                       f"{steps} budgeted calls, {int(np.sum(budgets))} walker-sweeps each"),
             "mean_L": Ls / wsw, "mean_n": ns / wsw, "mean_W": Ws / wsw,
-            "parallelism": parallelism(choice, len(wds)) if budgets is None else
+            "parallelism": parallelism(choice, len(wds), generic) if budgets is None else
             (f"persistent work-queue launch with per-walker budgets (largest walker "
              f"{int(np.min(budgets))} sweep(s) per call, smallest {int(np.max(budgets))}), tuned "
              f"{choice}")}

@@ -408,7 +408,14 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
  *                expected operator count (bonds / T), largest first, so the
  *                longest chains start first and each stream group holds walkers
  *                of similar size (and gets a worm-kernel LDS size to match);
- *                0 = walker index order.  Results are identical.             */
+ *                0 = walker index order.  Results are identical.
+ * SSE_OPT_GENERIC (wave mode): 1 = run every model, two-site bonds included,
+ *                through the generic wave kernels (the ones that bonds of one,
+ *                three and four sites always take: chase codes of two bytes
+ *                per operator in LDS, no work-queue launch); 0 = the two-site
+ *                kernels (an error while a model has bonds of other sizes).
+ *                Default: 0 unless such a model is present.  Restarts tuning.
+ *                Results are identical.                                     */
 enum sse_option {
     SSE_OPT_GRAPH = 1,
     SSE_OPT_TIMING = 2,
@@ -420,7 +427,8 @@ enum sse_option {
     SSE_OPT_CHASE = 8,
     SSE_OPT_TUNE = 9,
     SSE_OPT_FUSE_MEASURE = 10,
-    SSE_OPT_LANE_WIDTH = 11
+    SSE_OPT_LANE_WIDTH = 11,
+    SSE_OPT_GENERIC = 12
 };
 int sse_set_option(sse_batch *b, int32_t option, int64_t value);
 

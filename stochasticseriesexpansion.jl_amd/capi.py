@@ -29,6 +29,7 @@ SSE_OPT_CHASE = 8
 SSE_OPT_TUNE = 9
 SSE_OPT_FUSE_MEASURE = 10
 SSE_OPT_LANE_WIDTH = 11
+SSE_OPT_GENERIC = 12
 
 STATUS = {0: "SSE_OK", 1: "SSE_ERR_INVALID", 2: "SSE_ERR_HIP", 3: "SSE_ERR_OOM",
           4: "SSE_ERR_SCATTER", 5: "SSE_ERR_NO_DEVICE", 6: "SSE_ERR_STATE"}

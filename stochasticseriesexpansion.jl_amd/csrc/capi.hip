@@ -135,6 +135,7 @@ struct sse_batch {
         std::vector<double> rate;  // best vertex updates per second seen per candidate
         std::vector<int> runs;
         int best = -1;             // chosen candidate (-1: still measuring)
+        bool twin = true;          // add the lookahead-chase twin of the fastest
     };
     bool tune = true;         // tuning enabled (an explicit launch option turns it off)
     int lane_width = 64;      // lane mode: walkers per wavefront (threads per workgroup)
@@ -1533,6 +1534,7 @@ static void tune_init(sse_batch *b, sse_batch::Tuner &t, bool budgets)
     }
     if (t.cands.empty())
         t.cands.push_back({0, 1, 0, 0});
+    t.twin = !b->wave.generic;  // the generic worm kernel has one chase form
     t.rate.assign(t.cands.size(), 0.0);
     t.runs.assign(t.cands.size(), 0);
     t.best = -1;
@@ -1562,7 +1564,7 @@ static void tune_record(sse_batch::Tuner &t, int i, double updates, double secon
         if (t.rate[k] > t.rate[best])
             best = (int)k;
     sse_batch::TuneCfg tw = t.cands[best];
-    if (tw.chase == 0) {
+    if (tw.chase == 0 && t.twin) {
         tw.chase = 1;
         bool have = false;
         for (auto &c : t.cands)
@@ -2588,6 +2590,31 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         HIPCHECK(hipStreamSynchronize(b->stream));
         b->wave.fuse_measure = (int)value;
         return SSE_OK;
+    case SSE_OPT_GENERIC: {
+        if (value != 0 && value != 1)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_GENERIC takes 0 or 1");
+        if (b->mode != SSE_MODE_WAVE)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_GENERIC needs wave mode");
+        int max_ns = 2;
+        for (const DevModel &m : b->h_models)
+            max_ns = std::max(max_ns, (int)m.nsb);
+        if (!value && max_ns != 2)
+            return fail(SSE_ERR_INVALID, "SSE_OPT_GENERIC: bonds of other than two sites need the "
+                                         "generic kernels");
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        b->wave.generic = (int)value;
+        if (value) {
+            b->wave.lds_k1n = std::max(b->wave.lds_k1n, k1n_lds_bytes(b->max_sites, max_ns));
+            if (b->wave.lds_k1n > 65536)
+                (void)hipFuncSetAttribute((const void *)waven_diag_vlist_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)b->wave.lds_k1n);
+            b->persistent = false;
+        }
+        b->tune_run = sse_batch::Tuner();  // the candidates depend on the kernel family
+        b->tune_bud = sse_batch::Tuner();
+        return SSE_OK;
+    }
     case SSE_OPT_WORM_RECORDS:
         if (value < 0 || value > 3)
             return fail(SSE_ERR_INVALID, "SSE_OPT_WORM_RECORDS takes 0, 1, 2 or 3");

@@ -38,16 +38,18 @@ def _compare(batch, oracles, tag):
 
 
 def _run_parity(sse, oracle_mod, model, T, n_walkers, sweeps, mode, seed0=0, records=None,
-                chase=None, **kw):
+                chase=None, generic=None, **kw):
     flat = model.flat()
     walkers = [sse.walker_desc(T=T, seed=seed0 + s, **kw) for s in range(n_walkers)]
     _say(f"parity: mode={mode} N={flat.n_sites} walkers={n_walkers} sweeps={sweeps} "
-         f"records={records} chase={chase}")
+         f"records={records} chase={chase} generic={generic}")
     batch = sse.Batch([flat], walkers, mode=mode)
     if records is not None:
         batch.set_option(sse.capi.SSE_OPT_WORM_RECORDS, records)
     if chase is not None:
         batch.set_option(sse.capi.SSE_OPT_CHASE, chase)
+    if generic is not None:
+        batch.set_option(sse.capi.SSE_OPT_GENERIC, generic)
     oracles = [oracle_mod.OracleWalker(sse.capi, flat, w) for w in walkers]
     batch.init()
     for o in oracles:
@@ -125,6 +127,53 @@ def test_parity_lookahead_chase(sse_pkg, oracle_mod, records):
                 thermalization=10, records=records, chase=1)
     _run_parity(sse_pkg, oracle_mod, ladder_dimer(sse_pkg, 8, 0.6), 0.2, 4, 20, 0, seed0=80,
                 thermalization=10, records=records, chase=1)
+
+
+@pytest.mark.parametrize("kind", ["square", "honeycomb", "xxz", "ladder", "bilayer"])
+def test_parity_generic_kernels_two_site(sse_pkg, oracle_mod, kind):
+    """SSE_OPT_GENERIC: two-site models through the generic wave kernels
+    (chase codes in LDS, the generic chase table) equal the oracle after
+    every sweep: spin-1 exit lists beyond four entries (honeycomb), a field
+    (xxz), four-state sites (ladder, bilayer)."""
+    model, T = {"square": (square_heisenberg(sse_pkg, 6), 0.4),
+                "honeycomb": (job_honeycomb(sse_pkg, 0.4), 0.4),
+                "xxz": (square_heisenberg(sse_pkg, 8, d=-0.3, hz=0.1), 0.25),
+                "ladder": (ladder_dimer(sse_pkg, 16, 0.6), 0.2),
+                "bilayer": (job_bilayer(sse_pkg, 0.2), 0.2)}[kind]
+    _run_parity(sse_pkg, oracle_mod, model, T, 4, 20, 0, seed0=90, generic=1, thermalization=6)
+
+
+def test_generic_switch_mid_chain(sse_pkg):
+    """Switching SSE_OPT_GENERIC between calls continues the same chains:
+    a batch alternating the kernel families equals one that never switches,
+    with and without measurement; the option's errors."""
+    hc = job_honeycomb(sse_pkg, 0.5).flat()
+    sq = square_heisenberg(sse_pkg, 6).flat()
+    wd = [sse_pkg.walker_desc(model=s % 2, T=0.3 + 0.05 * (s % 3), seed=500 + s, thermalization=8)
+          for s in range(12)]
+    _say("generic kernels switched mid-chain: 12 walkers, 2 models")
+    a = sse_pkg.Batch([sq, hc], wd)
+    b = sse_pkg.Batch([sq, hc], wd)
+    a.init()
+    b.init()
+    for rep, (k, meas) in enumerate(((15, True), (4, False), (21, True), (3, True))):
+        a.set_option(sse_pkg.capi.SSE_OPT_GENERIC, rep % 2)
+        a.run(k, measure=meas)
+        b.run(k, measure=meas)
+        _assert_same_walkers(a, b, len(wd), f"call {rep}")
+    for (s1, q1, c1), (s2, q2, c2) in zip(a.accumulators(), b.accumulators()):
+        assert c1 == c2 and np.allclose(s1, s2, rtol=1e-12) and np.allclose(q1, q2, rtol=1e-12)
+    with pytest.raises(RuntimeError):
+        a.set_option(sse_pkg.capi.SSE_OPT_GENERIC, 2)
+    a.set_option(sse_pkg.capi.SSE_OPT_GENERIC, 1)
+    with pytest.raises(RuntimeError):  # no work-queue launch in the generic family
+        a.set_option(sse_pkg.capi.SSE_OPT_PERSISTENT, 1)
+    a.close()
+    b.close()
+    lane = sse_pkg.Batch([sq], [wd[0]], mode=1)
+    with pytest.raises(RuntimeError):
+        lane.set_option(sse_pkg.capi.SSE_OPT_GENERIC, 1)
+    lane.close()
 
 
 @pytest.mark.parametrize("mode", MODES)
