@@ -271,6 +271,17 @@ static int validate_model(const sse_model_desc &m, int k)
                 return fail(SSE_ERR_INVALID, buf);
             }
         }
+        // leg states are 1-based local states of the leg's site (leg j sits on
+        // bond site j % ns); the kernels index per-state tables with them
+        for (int v = 0; v < vd.n_vertices; ++v)
+            for (int j = 0; j < 2 * ns; ++j) {
+                const int st = vd.leg_states[(size_t)2 * ns * v + j];
+                if (st < 1 || st > vd.dims[j % ns] || (m.n_estimators > 0 && st > m.est_max_dim)) {
+                    snprintf(buf, sizeof buf, "model %d type %d: vertex %d leg %d state out of range",
+                             k, t, v, j);
+                    return fail(SSE_ERR_INVALID, buf);
+                }
+            }
         int nt = vd.n_vertices * vd.max_worm_count * 2 * ns;
         for (int i = 0; i < nt; ++i) {
             int off = vd.transition_offset[i], cnt = vd.transition_count[i];
@@ -514,6 +525,40 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
         }
     }
     size_t o_gtab = put(gt.data(), sizeof(DirectEntry) * gt.size());
+    // measurement tables (two-site bonds): per (bond, vertex) the change of
+    // each estimator across the vertex, the same expression in the same order
+    // as the measurement's own (cf[a] (mv[a][u0] - mv[a][l0]) + cf[b] (mv[b][u1]
+    // - mv[b][l1])), and the vertex's sign
+    int any_neg = 0;
+    for (int t = 0; t < m.n_types; ++t)
+        for (int v = 0; v < m.types[t].n_vertices; ++v)
+            any_neg |= m.types[t].signs[v] < 0;
+    const size_t edt_n = (size_t)m.n_bonds * dnv;
+    const bool edt = ns == 2 && edt_n * (ne + 1) * sizeof(double) <= ((size_t)16 << 20);
+    size_t o_edt = 0, o_esg = 0;
+    if (edt) {
+        std::vector<double> dtb(edt_n * ne + 1, 0.0);
+        std::vector<int8_t> sgb(edt_n, 0);
+        const int D = m.n_estimators ? m.est_max_dim : 1;
+        for (int bd = 0; bd < m.n_bonds; ++bd) {
+            const sse_vertex_data_desc &vd = m.types[m.bond_type[bd]];
+            const int sa = m.bond_sites[2 * bd], sb = m.bond_sites[2 * bd + 1];
+            for (int v = 0; v < vd.n_vertices; ++v) {
+                const size_t ix = (size_t)bd * dnv + v;
+                sgb[ix] = vd.signs[v];
+                const uint8_t *ls = vd.leg_states + 4 * (size_t)v;
+                for (size_t k = 0; k < ne; ++k) {
+                    const double *cf = m.est_coef + k * m.n_sites;
+                    const double *mk = m.est_mval + k * m.n_sites * D;
+                    double delta = cf[sa] * (mk[sa * D + ls[2] - 1] - mk[sa * D + ls[0] - 1]);
+                    delta += cf[sb] * (mk[sb * D + ls[3] - 1] - mk[sb * D + ls[1] - 1]);
+                    dtb[ix * ne + k] = delta;
+                }
+            }
+        }
+        o_edt = put(dtb.data(), sizeof(double) * dtb.size());
+        o_esg = put(sgb.data(), sgb.size());
+    }
     // diagonal-update table (layout: DevModel::k1tab)
     std::vector<char> k1((size_t)16 * m.n_types);
     for (int t = 0; t < m.n_types && ns == 2; ++t) {
@@ -603,6 +648,10 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
     dm.gtab = (const DirectEntry *)(d + o_gtab);
     dm.gtab_bytes = (int32_t)(sizeof(DirectEntry) * gt.size());
     dm.g_lg = glg;
+    dm.est_dtab = edt ? (const double *)(d + o_edt) : nullptr;
+    dm.est_dsgn = edt ? (const int8_t *)(d + o_esg) : nullptr;
+    dm.est_dnv = dnv;
+    dm.any_neg = any_neg;
     return SSE_OK;
 }
 

@@ -1707,15 +1707,28 @@ __device__ __forceinline__ double wave_sum(double x)
     return x;
 }
 
+// x moved across lanes by one DPP pattern, both 32-bit halves; lanes without
+// a source (or outside the row mask) read 0
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ double dpp_f64(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, kCtrl, kRowMask, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), kCtrl, kRowMask, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Inclusive prefix sum over the wave on the VALU's cross-lane (DPP) paths:
+// shifts by 1, 2, 4 and 8 within each row of 16 lanes, then lane 15 of rows
+// 0 and 2 added to rows 1 and 3, then lane 31 added to rows 2 and 3.
 __device__ __forceinline__ double wave_incl_scan(double x)
 {
-    const int lane = threadIdx.x;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        double y = __shfl_up(x, o);
-        if (lane >= o)
-            x += y;
-    }
+    x += dpp_f64<0x111, 0xf>(x);  // row_shr:1
+    x += dpp_f64<0x112, 0xf>(x);  // row_shr:2
+    x += dpp_f64<0x114, 0xf>(x);  // row_shr:4
+    x += dpp_f64<0x118, 0xf>(x);  // row_shr:8
+    x += dpp_f64<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+    x += dpp_f64<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
     return x;
 }
 
@@ -1727,6 +1740,11 @@ __device__ __forceinline__ double wave_incl_scan(double x)
 // running sums over the operator positions are wave prefix sums.  Writes the
 // five outputs of each of its estimators.
 constexpr int K3_MAXE = 4;
+__device__ __forceinline__ void k3_finish(const DevWalker &w, const DevModel &m, int e0, int ne,
+                                          bool want_sign, int neg, int64_t cnt, const double *tmp0,
+                                          const double *mag, const double *absmag, const double *mag2,
+                                          const double *mag4, double &sign, double *o);
+template <bool kTable = true>
 __device__ __forceinline__ void k3_pass(const DevWalker &w, const DevModel &m, int e0, int ne,
                                         bool want_sign, double &sign, double *o)
 {
@@ -1750,6 +1768,60 @@ __device__ __forceinline__ void k3_pass(const DevWalker &w, const DevModel &m, i
     int neg = 0;
     int64_t cnt = 0;
     const gmem<uint32_t> *ops = as_global((const uint32_t *)w.ops);
+    if (kTable && m.est_dtab) {
+        // per-(bond, vertex) tables: an operator's estimator changes and sign
+        // are one gather each; the operator is loaded two chunks ahead, its
+        // table values one ahead
+        const gmem<double> *DT = as_global(m.est_dtab);
+        const gmem<int8_t> *SG = as_global(m.est_dsgn);
+        const uint32_t dnv = (uint32_t)m.est_dnv, NE = (uint32_t)m.n_est;
+        const bool sgn = want_sign && m.any_neg;
+        auto op_at = [&](int64_t q) -> uint32_t { return q < L ? ops[q] : 0u; };
+        auto vals_of = [&](uint32_t op, double *d, int8_t &sg) {
+            const uint32_t ix = op_bond(op) * dnv + (op_vcode(op) >> 1);
+            const bool offd = op != 0u && !(op_vcode(op) & 1u);
+#pragma unroll
+            for (int k = 0; k < K3_MAXE; ++k)
+                d[k] = k < ne && offd ? DT[(size_t)ix * NE + e0 + k] : 0.0;
+            sg = sgn && op != 0u ? SG[ix] : (int8_t)0;
+        };
+        uint32_t op0 = op_at(lane), op1 = op_at(64 + lane);
+        double d0[K3_MAXE];
+        int8_t sg0;
+        vals_of(op0, d0, sg0);
+        for (int64_t base = 0; base < L; base += 64) {
+            const uint32_t op2 = op_at(base + 128 + lane);
+            double d1[K3_MAXE];
+            int8_t sg1;
+            vals_of(op1, d1, sg1);
+            if (sgn)
+                neg += popc(__ballot(op0 != 0u && sg0 < 0));
+            cnt += popc(__ballot(op0 != 0u));
+#pragma unroll
+            for (int k = 0; k < K3_MAXE; ++k) {
+                if (k < ne) {
+                    const double inc = wave_incl_scan(d0[k]);
+                    if (op0 != 0u) {
+                        const double t = run[k] + inc;
+                        const double t2 = t * t;
+                        mag[k] += t;
+                        absmag[k] += fabs(t);
+                        mag2[k] += t2;
+                        mag4[k] += t2 * t2;
+                    }
+                    run[k] += __shfl(inc, 63);
+                }
+            }
+            op0 = op1;
+            op1 = op2;
+#pragma unroll
+            for (int k = 0; k < K3_MAXE; ++k)
+                d0[k] = d1[k];
+            sg0 = sg1;
+        }
+        k3_finish(w, m, e0, ne, want_sign, neg, cnt, tmp0, mag, absmag, mag2, mag4, sign, o);
+        return;
+    }
     const gmem<uint64_t> *brec = as_global((const uint64_t *)m.bond_rec);  // int2 pairs
     // Software pipeline over the 64-slot chunks: the operator is loaded three
     // chunks ahead, its bond record two ahead, the vertex's sign and leg states
@@ -1833,6 +1905,16 @@ __device__ __forceinline__ void k3_pass(const DevWalker &w, const DevModel &m, i
         sg0 = sg1;
         lg0 = lg1;
     }
+    k3_finish(w, m, e0, ne, want_sign, neg, cnt, tmp0, mag, absmag, mag2, mag4, sign, o);
+}
+
+// The five outputs of each estimator of a pass (and the sign of the first).
+__device__ __forceinline__ void k3_finish(const DevWalker &w, const DevModel &m, int e0, int ne,
+                                          bool want_sign, int neg, int64_t cnt, const double *tmp0,
+                                          const double *mag, const double *absmag, const double *mag2,
+                                          const double *mag4, double &sign, double *o)
+{
+    const int lane = threadIdx.x;
     if (want_sign)
         sign = (neg & 1) ? -1.0 : 1.0;
     const double Nn = m.norm;
@@ -1867,6 +1949,7 @@ __device__ __forceinline__ void k3_pass(const DevWalker &w, const DevModel &m, i
 // mode 1: thermalized walkers only, accumulated into the per-walker sums
 // (fused sweep+measure loop).  Estimators are taken K3_MAXE per pass over the
 // operator string; the first pass also computes the sign.
+template <bool kTable = true>
 __device__ __forceinline__ void k3_walker(DevWalker *W, const DevModel *models, double *out, double *sum,
                                           double *sumsq, int64_t *count, double *wlf, int wi, int mode,
                                           int32_t *done)
@@ -1890,7 +1973,7 @@ __device__ __forceinline__ void k3_walker(DevWalker *W, const DevModel *models, 
     int e0 = 0;
     do {
         const int ne = m.n_est - e0 < K3_MAXE ? m.n_est - e0 : K3_MAXE;
-        k3_pass(w, m, e0, ne, e0 == 0, sign, o);
+        k3_pass<kTable>(w, m, e0, ne, e0 == 0, sign, o);
         e0 += K3_MAXE;
     } while (e0 < m.n_est);
     const double nn = (double)w.n;
@@ -1936,7 +2019,9 @@ __global__ __launch_bounds__(64) void wave_measure_kernel(DevWalker *W, const De
 // measurement runs inside the launch's worm tail instead of after it.  The
 // measurement reads the operator string, spins and DevWalker that the worm
 // phase of the same wave has just stored, through vector loads after the
-// barrier (nothing of it goes through the scalar cache).
+// barrier (nothing of it goes through the scalar cache).  Its measurement
+// gathers the estimator values (no per-(bond, vertex) tables): with those the
+// compiler cannot keep the chase's operands in scalar registers.
 __global__ __launch_bounds__(64) void wave_worm_measure_kernel(
     DevWalker *W, const DevModel *models, DevPool *pool, int w_begin, int w_end, int lds_bytes,
     int rec_mode, const int32_t *order, int chase, double *out, double *sum, double *sumsq,
@@ -1948,7 +2033,7 @@ __global__ __launch_bounds__(64) void wave_worm_measure_kernel(
         const int wi = walker_at(order, pos);
         k2_walker<false>(W, models, pool, wi, 1, lds_bytes, rec_mode, lds, chase);
         __syncthreads();
-        k3_walker(W, models, out, sum, sumsq, count, wlf, wi, 1, done);
+        k3_walker<false>(W, models, out, sum, sumsq, count, wlf, wi, 1, done);
     }
 }
 
