@@ -525,16 +525,17 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
         }
     }
     size_t o_gtab = put(gt.data(), sizeof(DirectEntry) * gt.size());
-    // measurement tables (two-site bonds): per (bond, vertex) the change of
-    // each estimator across the vertex, the same expression in the same order
-    // as the measurement's own (cf[a] (mv[a][u0] - mv[a][l0]) + cf[b] (mv[b][u1]
-    // - mv[b][l1])), and the vertex's sign
+    // measurement tables: per (bond, vertex) the change of each estimator
+    // across the vertex, the same expression in the same order as the
+    // measurement kernels' own (two sites: cf[a] (mv[a][u0] - mv[a][l0]) +
+    // cf[b] (mv[b][u1] - mv[b][l1]); other sizes: 0 + the site terms in bond
+    // order), and the vertex's sign
     int any_neg = 0;
     for (int t = 0; t < m.n_types; ++t)
         for (int v = 0; v < m.types[t].n_vertices; ++v)
             any_neg |= m.types[t].signs[v] < 0;
     const size_t edt_n = (size_t)m.n_bonds * dnv;
-    const bool edt = ns == 2 && edt_n * (ne + 1) * sizeof(double) <= ((size_t)16 << 20);
+    const bool edt = edt_n * (ne + 1) * sizeof(double) <= ((size_t)16 << 20);
     size_t o_edt = 0, o_esg = 0;
     if (edt) {
         std::vector<double> dtb(edt_n * ne + 1, 0.0);
@@ -542,16 +543,27 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
         const int D = m.n_estimators ? m.est_max_dim : 1;
         for (int bd = 0; bd < m.n_bonds; ++bd) {
             const sse_vertex_data_desc &vd = m.types[m.bond_type[bd]];
-            const int sa = m.bond_sites[2 * bd], sb = m.bond_sites[2 * bd + 1];
+            const int32_t *bs = m.bond_sites + (size_t)ns * bd;
             for (int v = 0; v < vd.n_vertices; ++v) {
                 const size_t ix = (size_t)bd * dnv + v;
                 sgb[ix] = vd.signs[v];
-                const uint8_t *ls = vd.leg_states + 4 * (size_t)v;
+                const uint8_t *ls = vd.leg_states + (size_t)legs * v;
                 for (size_t k = 0; k < ne; ++k) {
                     const double *cf = m.est_coef + k * m.n_sites;
                     const double *mk = m.est_mval + k * m.n_sites * D;
-                    double delta = cf[sa] * (mk[sa * D + ls[2] - 1] - mk[sa * D + ls[0] - 1]);
-                    delta += cf[sb] * (mk[sb * D + ls[3] - 1] - mk[sb * D + ls[1] - 1]);
+                    auto term = [&](int j) {
+                        const size_t st = (size_t)bs[j] * D;
+                        return cf[bs[j]] * (mk[st + ls[ns + j] - 1] - mk[st + ls[j] - 1]);
+                    };
+                    double delta;
+                    if (ns == 2) {
+                        delta = term(0);
+                        delta += term(1);
+                    } else {
+                        delta = 0.0;
+                        for (int j = 0; j < ns; ++j)
+                            delta += term(j);
+                    }
                     dtb[ix * ne + k] = delta;
                 }
             }

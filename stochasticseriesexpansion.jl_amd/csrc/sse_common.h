@@ -76,7 +76,7 @@ struct DevModel {
     // 2^g_lg = leg count rounded up to a power of two; exit words gen_out()
     const struct DirectEntry *gtab;
     int32_t gtab_bytes, g_lg;
-    // measurement tables of two-site models (null when too large): per
+    // measurement tables (null when too large): per
     // (bond, vertex) at bond * est_dnv + vertex, the change of each estimator
     // across the vertex, f64 [n_est] (the measurement adds it for off-diagonal
     // vertices), and the vertex's sign, i8

@@ -845,9 +845,11 @@ __device__ void k2n_walker(DevWalker *W, const DevModel *models, DevPool *pool, 
     }
 }
 
-// Measurement (sse.jl:70-87, 305-376) of walker wi, 64 slots per step: the
-// sign and every magnetization estimator in one pass over the operator
-// string.  mode as in k3_walker.
+// Measurement (sse.jl:70-87, 305-376) of walker wi, 64 slots per step.  With
+// the model's per-(bond, vertex) tables it is k3_walker's (their values do not
+// depend on the bond size); otherwise the sign and operator count in one pass
+// over the operator string, then one pass per magnetization estimator.  mode
+// as in k3_walker.
 template <int NS>
 __device__ void k3n_walker(DevWalker *W, const DevModel *models, double *out, double *sum,
                            double *sumsq, int64_t *count, double *wlf, int wi, int mode,
@@ -855,6 +857,10 @@ __device__ void k3n_walker(DevWalker *W, const DevModel *models, double *out, do
 {
     constexpr int LEGS = LegsOf<NS>::LEGS;
     const int lane = threadIdx.x;
+    if (models[W[wi].model].est_dtab) {
+        k3_walker<true>(W, models, out, sum, sumsq, count, wlf, wi, mode, done);
+        return;
+    }
     const DevWalker w = W[wi];
     if (w.error || w.pending || (mode == 1 && !w.active))
         return;
