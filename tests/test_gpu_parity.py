@@ -176,6 +176,25 @@ def test_generic_switch_mid_chain(sse_pkg):
     lane.close()
 
 
+@pytest.mark.parametrize("state", [0, 3])
+def test_leg_state_out_of_range_rejected(sse_pkg, state):
+    """A vertex leg state outside 1..dim (the kernels index per-state tables
+    with it) is rejected when the batch is created; the valid model is
+    accepted."""
+    import copy
+    flat = square_heisenberg(sse_pkg, 4).flat()
+    bad = copy.copy(flat)
+    bad.types = [dict(t) for t in flat.types]
+    ls = bad.types[0]["leg_states"].copy()
+    ls[1, 2] = state
+    bad.types[0]["leg_states"] = ls
+    wd = [sse_pkg.walker_desc(T=0.5, seed=1)]
+    _say(f"leg state {state} out of range")
+    with pytest.raises(sse_pkg.capi.SSEError, match="state out of range"):
+        sse_pkg.Batch([bad], wd)
+    sse_pkg.Batch([flat], wd).close()
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_vertex_list_golden_gpu(sse_pkg, mode):
     """Reference golden vertex list, built by the device vertex-list phase."""
