@@ -245,6 +245,8 @@ __host__ __device__ inline size_t k1_lds_bytes(int N)
     return (b + 15) & ~(size_t)15;
 }
 
+constexpr int K1_BLOCK = 16;  // 64-slot chunks per block of operator-string loads
+
 // Orders the LDS accesses of the lanes of a one-wave workgroup.  The LDS runs
 // a wave's instructions in order, so only code motion must be prevented; unlike
 // __syncthreads this does not wait for global loads still in flight.
@@ -480,13 +482,44 @@ __device__ __forceinline__ void k1_walker_t(DevWalker *W, const DevModel *models
     gmem<uint32_t> *gbase = as_global(rec_base(w));
     gmem<uint32_t> *ops = as_global(w.ops);
 
-    // the next chunk's operators are loaded while this chunk is processed
-    uint32_t op_next = (int64_t)lane < L ? ops[lane] : 0u;
+    // The operator string reaches the chunks in blocks of K1_BLOCK chunks held
+    // in registers (lane l: slot l of each chunk): a block's loads are issued a
+    // whole block ahead and moved into place when the block starts, so the loop
+    // waits on global memory once per block.  (A load consumed in the next
+    // chunk would be waited on together with the chunk's global stores: the
+    // memory counter does not order loads against stores.)  K1 writes only the
+    // current chunk's slots, so the loaded values of later chunks stay current.
+    // The chunk index within the block is wave-uniform, so cur[slot] is a
+    // register-indexed move.  Model scalars are read once, before the loop,
+    // for the same reason (a read inside it would wait for the stores).
+    const uint32_t nbonds = (uint32_t)m.n_bonds;
+    uint32_t cur[K1_BLOCK], pf[K1_BLOCK];
+#pragma unroll
+    for (int u = 0; u < K1_BLOCK; ++u) {
+        const int64_t q = 64 * u + lane;
+        cur[u] = q < L ? ops[q] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < K1_BLOCK; ++u) {
+        const int64_t q = 64 * (K1_BLOCK + u) + lane;
+        pf[u] = q < L ? ops[q] : 0u;
+    }
     for (int64_t base = 0; base < L; base += 64) {
         const int64_t p = base + lane;
         const bool valid = p < L;
-        const uint32_t op = op_next;
-        op_next = p + 64 < L ? ops[p + 64] : 0u;
+        const int64_t chunk = base >> 6;
+        const int slot = (int)uni((uint32_t)(chunk % K1_BLOCK));
+        if (slot == 0 && chunk > 0) {
+#pragma unroll
+            for (int u = 0; u < K1_BLOCK; ++u)
+                cur[u] = pf[u];
+#pragma unroll
+            for (int u = 0; u < K1_BLOCK; ++u) {
+                const int64_t q = base + 64 * (K1_BLOCK + u) + lane;
+                pf[u] = q < L ? ops[q] : 0u;
+            }
+        }
+        const uint32_t op = cur[slot];
         const uint32_t vc = op_vcode(op);
         const bool ident = valid && op == 0u;
         const bool isop = valid && op != 0u;
@@ -498,7 +531,7 @@ __device__ __forceinline__ void k1_walker_t(DevWalker *W, const DevModel *models
         {
             bool bad = false;
             if (isop) {
-                bad = b >= (uint32_t)m.n_bonds;
+                bad = b >= nbonds;
                 if (!bad)
                     bad = (int)(vc >> 1) >= tab.nv(tab.bond(b).x >> 20);
             }
