@@ -1372,6 +1372,10 @@ __device__ __forceinline__ int worm_chase_la(DevWalker &w, const DevModel &m, K2
 }
 
 // state refresh (sse.jl:219-228) and operator write-back after the worms
+template <class R>
+__device__ __noinline__ void worm_writeback(R rec, uint32_t lgp, uint32_t *ops_p, int64_t L,
+                                            const gmem<uint64_t> *MASK, const gmem<uint32_t> *BASE);
+
 template <class R, class RS>
 __device__ __forceinline__ void worm_finish(DevWalker &w, const DevModel &m, RS &rs, const R &rec,
                             const gmem<uint64_t> *MASK, const gmem<uint32_t> *BASE)
@@ -1423,33 +1427,40 @@ __device__ __forceinline__ void worm_finish(DevWalker &w, const DevModel &m, RS 
         if (vs)
             w.spins[s] = val;
     }
-    // write the worm-updated vertex codes back into the operator string, four
-    // 64-slot chunks per pass so their loads are in flight together
-    const int64_t L = w.L;
-    gmem<uint32_t> *ops = as_global(w.ops);
-    for (int64_t base = 0; base < L; base += 256) {
-        uint32_t op[4], r[4];
-        bool act[4];
+    worm_writeback<R>(rec, lgp, w.ops, w.L, MASK, BASE);
+}
+
+// Write the worm-updated vertex codes back into the operator string (the
+// last step of the worm update), K 64-slot chunks per pass so their loads are
+// in flight together: lane u loads chunk u's mask and base (vector loads, read
+// back by v_readlane), every lane its slot of each chunk.  A function of its
+// own: inlined, its registers would crowd the chase loop's scalar operands.
+template <class R>
+__device__ __noinline__ void worm_writeback(R rec, uint32_t lgp, uint32_t *ops_p, int64_t L,
+                                            const gmem<uint64_t> *MASK, const gmem<uint32_t> *BASE)
+{
+    const int lane = threadIdx.x;
+    constexpr int K = 16;
+    gmem<uint32_t> *ops = as_global(ops_p);
+    for (int64_t base = 0; base < L; base += 64 * K) {
+        const int64_t cl = (base >> 6) + lane;
+        const bool lin = lane < K && base + 64 * lane < L;
+        const uint64_t mkl = lin ? MASK[cl] : 0ull;
+        const uint32_t bsl = lin ? BASE[cl] : 0u;
+        uint32_t op[K];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int64_t c = (base >> 6) + u;
+        for (int u = 0; u < K; ++u) {
             const int64_t p = base + 64 * u + lane;
-            act[u] = false;
-            op[u] = 0;
-            r[u] = 0;
-            if (base + 64 * u < L) {
-                const uint64_t mk = MASK[c];
-                if ((mk >> lane) & 1ull) {
-                    act[u] = true;
-                    r[u] = BASE[c] + (uint32_t)popc(mk & below_mask(lane));
-                    op[u] = ops[p];
-                }
-            }
+            op[u] = p < L ? ops[p] : 0u;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (act[u]) {
-                const uint32_t vc = vt_of_ecode(rec.get_ecode(r[u]), lgp) & 1023u;
+        for (int u = 0; u < K; ++u) {
+            const uint64_t mk = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(mkl >> 32), u) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mkl, u);
+            if ((mk >> lane) & 1ull) {
+                const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)bsl, u) +
+                                   (uint32_t)popc(mk & below_mask(lane));
+                const uint32_t vc = vt_of_ecode(rec.get_ecode(r), lgp) & 1023u;
                 const uint32_t nop = (op[u] & ~(1023u << 1)) | (vc << 1);
                 if (nop != op[u])
                     ops[base + 64 * u + lane] = nop;
