@@ -1485,20 +1485,23 @@ constexpr size_t K2_LDS_CAP = 65536;
 
 // Stage the compact records of K1 (global) into the LDS layout R, converting
 // vt to ecodes and checking every ecode and link; returns false on a bad record.
+// K 64-record chunks per pass, their loads in flight together; a function of
+// its own (arguments by value) so that its registers stay out of the chase's.
 template <int FMT>
-__device__ bool k2_stage(const DevWalker &w, const DevModel &m, lds_u64 *lk, lds_u16 *ec)
+__device__ __noinline__ bool k2_stage_records(uint32_t n, uint32_t lgp, uint32_t nt, const uint32_t *vt_p,
+                                              const U4 *lk_p, lds_u64 *lk, lds_u16 *ec)
 {
+    constexpr int K = 16;
     const int lane = threadIdx.x;
-    const uint32_t n = (uint32_t)w.n, lgp = (uint32_t)m.dt_lgp, nt = (uint32_t)m.n_types;
-    const gmem<uint32_t> *gvt = as_global((const uint32_t *)rec_vt(w));
-    const gmem<U4> *glk = as_global((const U4 *)rec_link(w));
+    const gmem<uint32_t> *gvt = as_global(vt_p);
+    const gmem<U4> *glk = as_global(lk_p);
     const uint32_t lmax = 4u * n;
     bool bad = false;
-    for (uint32_t i0 = 0; i0 < n; i0 += 256) {
-        uint32_t vt[4];
-        uint4 l4[4];
+    for (uint32_t i0 = 0; i0 < n; i0 += 64 * K) {
+        uint32_t vt[K];
+        uint4 l4[K];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < K; ++u) {
             const uint32_t i = i0 + 64u * u + lane;
             if (i < n) {
                 vt[u] = gvt[i];
@@ -1507,7 +1510,7 @@ __device__ bool k2_stage(const DevWalker &w, const DevModel &m, lds_u64 *lk, lds
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < K; ++u) {
             const uint32_t i = i0 + 64u * u + lane;
             if (i < n) {
                 const uint32_t e = ecode_of_vt(vt[u], lgp);
@@ -1525,6 +1528,13 @@ __device__ bool k2_stage(const DevWalker &w, const DevModel &m, lds_u64 *lk, lds
         }
     }
     return __ballot(bad) == 0;
+}
+
+template <int FMT>
+__device__ __forceinline__ bool k2_stage(const DevWalker &w, const DevModel &m, lds_u64 *lk, lds_u16 *ec)
+{
+    return k2_stage_records<FMT>((uint32_t)w.n, (uint32_t)m.dt_lgp, (uint32_t)m.n_types,
+                                 (const uint32_t *)rec_vt(w), (const U4 *)rec_link(w), lk, ec);
 }
 
 // vt -> ecode into an LDS u16 array (RecH path), checking every code.
