@@ -398,6 +398,23 @@ __device__ void k1_serial_chunk(DevWalker &w, const DevModel &m, WaveStream &rs,
     __syncthreads();
 }
 
+// k1_serial_chunk out of line, everything by value (the caller's walker copy
+// and stream stay in registers; the rare path gets a call instead of inlined
+// code in the chunk loop).  Returns the counts and the stream position.
+struct K1SerialOut {
+    int64_t n, nr;
+    uint64_t bbase;
+    uint32_t off;
+};
+__device__ __noinline__ K1SerialOut k1_serial_chunk_ool(DevWalker w, const DevModel *m, WaveStream rs,
+                                                        int64_t base, int64_t n, int64_t nr, double pm_raw,
+                                                        double pr_raw, bool do_diag, bool do_vl, uint8_t *sp,
+                                                        int32_t *vfirst, int32_t *vlast)
+{
+    k1_serial_chunk(w, *m, rs, base, n, nr, pm_raw, pr_raw, do_diag, do_vl, sp, vfirst, vlast);
+    return K1SerialOut{n, nr, rs.bbase, rs.off};
+}
+
 // phases: 1 diagonal update, 2 vertex list, 8 force the slot-by-slot path.
 // warm >= 0: init warm-up round (diagonal only; walkers with fewer warm-up
 // sweeps are skipped).
@@ -562,14 +579,22 @@ __device__ __forceinline__ void k1_walker_t(DevWalker *W, const DevModel *models
                 }
             }
             if (__ballot(reject)) {
-                k1_serial_chunk(w, m, rs, base, n0, nr, pm_raw, pr_raw, do_diag, do_vl, sp, vfirst,
-                                vlast);
+                const K1SerialOut so = k1_serial_chunk_ool(w, &m, rs, base, n0, nr, pm_raw, pr_raw, do_diag,
+                                                           do_vl, sp, vfirst, vlast);
+                n0 = so.n;
+                nr = so.nr;
+                rs.bbase = so.bbase;
+                rs.off = so.off;
                 continue;
             }
             rs.advance((uint32_t)D);
         } else if (force_serial) {
-            k1_serial_chunk(w, m, rs, base, n0, nr, pm_raw, pr_raw, do_diag, do_vl, sp, vfirst,
-                            vlast);
+            const K1SerialOut so = k1_serial_chunk_ool(w, &m, rs, base, n0, nr, pm_raw, pr_raw, do_diag,
+                                                       do_vl, sp, vfirst, vlast);
+            n0 = so.n;
+            nr = so.nr;
+            rs.bbase = so.bbase;
+            rs.off = so.off;
             continue;
         }
 
