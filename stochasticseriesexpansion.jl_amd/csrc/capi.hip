@@ -596,6 +596,43 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
     }
     k1.resize((k1.size() + 15) & ~(size_t)15);
     size_t o_k1 = put(k1.data(), k1.size());
+    // the generic diagonal-update kernel's lookups (layout: DevModel::k1ntab)
+    std::vector<char> kn((size_t)32 * m.n_types + (size_t)4 * (ns + 1) * m.n_bonds);
+    const uint32_t kn_bond_off = (uint32_t)(32 * (size_t)m.n_types);
+    for (int bd = 0; bd < m.n_bonds; ++bd) {
+        int32_t rec[5] = {m.bond_type[bd], 0, 0, 0, 0};
+        for (int k = 0; k < ns; ++k)
+            rec[1 + k] = m.bond_sites[(size_t)ns * bd + k];
+        memcpy(kn.data() + kn_bond_off + (size_t)4 * (ns + 1) * bd, rec, (size_t)4 * (ns + 1));
+    }
+    for (int t = 0; t < m.n_types; ++t) {
+        const sse_vertex_data_desc &vd = m.types[t];
+        size_t states = 1;
+        uint32_t dimspack = 0;
+        for (int i = 0; i < ns; ++i) {
+            states *= (size_t)vd.dims[i];
+            dimspack |= (uint32_t)vd.dims[i] << (8 * i);
+        }
+        kn.resize((kn.size() + 7) & ~(size_t)7);
+        const uint32_t w_off = (uint32_t)kn.size();
+        kn.resize(kn.size() + 8 * (size_t)vd.n_vertices);
+        memcpy(kn.data() + w_off, vd.weights, 8 * (size_t)vd.n_vertices);
+        const uint32_t dv_off = (uint32_t)kn.size();
+        kn.resize(kn.size() + 4 * states);
+        memcpy(kn.data() + dv_off, vd.diagonal_vertices, 4 * states);
+        const uint32_t up_off = (uint32_t)kn.size();
+        kn.resize(kn.size() + 4 * (size_t)vd.n_vertices);
+        for (int v = 0; v < vd.n_vertices; ++v) {
+            uint32_t u = 0;
+            for (int k = 0; k < ns; ++k)
+                u |= (uint32_t)vd.leg_states[(size_t)2 * ns * v + ns + k] << (8 * k);
+            memcpy(kn.data() + up_off + 4 * (size_t)v, &u, 4);
+        }
+        const uint32_t h[8] = {w_off, dv_off, up_off, (uint32_t)vd.n_vertices, dimspack, 0, 0, 0};
+        memcpy(kn.data() + 32 * (size_t)t, h, 32);
+    }
+    kn.resize((kn.size() + 15) & ~(size_t)15);
+    size_t o_kn = put(kn.data(), kn.size());
 
     char *d = nullptr;
     int rc = dev_alloc(b, &d, blob.size() + 16);
@@ -657,6 +694,9 @@ static int upload_model(sse_batch *b, const sse_model_desc &m, DevModel &dm, Hos
     dm.k1tab = (const uint8_t *)(d + o_k1);
     dm.k1tab_bytes = (int32_t)k1.size();
     dm.pad_k1 = 0;
+    dm.k1ntab = (const uint8_t *)(d + o_kn);
+    dm.k1ntab_bytes = (int32_t)kn.size();
+    dm.k1n_bond_off = (int32_t)kn_bond_off;
     dm.gtab = (const DirectEntry *)(d + o_gtab);
     dm.gtab_bytes = (int32_t)(sizeof(DirectEntry) * gt.size());
     dm.g_lg = glg;
@@ -1920,7 +1960,8 @@ static int lane_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure
                                    b->d_done, bp, (int)i);
         } else {  // the generic wave kernels, one walker per workgroup
             hipLaunchKernelGGL(waven_diag_vlist_kernel, dim3(n), dim3(64), b->wave.lds_k1n, b->stream,
-                               b->d_walkers, b->d_models, b->d_pool, 0, n, 3, -1, ord, 7, bp, (int)i);
+                               b->d_walkers, b->d_models, b->d_pool, 0, n, 3, -1, ord, 7, bp, (int)i,
+                               (int)b->wave.lds_k1n);
             hipLaunchKernelGGL(waven_worm_kernel, dim3(n), dim3(64), b->wave.lds_k2n, b->stream,
                                b->d_walkers, b->d_models, b->d_pool, 0, n, 1, ord, bp, (int)i,
                                (int)b->wave.lds_k2n);
@@ -2419,7 +2460,12 @@ int sse_add_model(sse_batch *b, const sse_model_desc *model, int32_t *index)
         // bonds of other than two sites: the generic wave kernels from now on
         // (they take two-site models too), never the work-queue launch
         b->wave.generic = 1;
-        b->wave.lds_k1n = std::max(b->wave.lds_k1n, k1n_lds_bytes(b->max_sites, dm.nsb));
+        {
+            int max_ns = 2;
+            for (const DevModel &x : b->h_models)
+                max_ns = std::max(max_ns, (int)x.nsb);
+            b->wave.lds_k1n = std::max(b->wave.lds_k1n, k1n_lds_for(b->h_models, b->max_sites, max_ns));
+        }
         if (b->wave.lds_k1n > 65536)
             (void)hipFuncSetAttribute((const void *)waven_diag_vlist_kernel,
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2665,7 +2711,7 @@ int sse_set_option(sse_batch *b, int32_t option, int64_t value)
         HIPCHECK(hipStreamSynchronize(b->stream));
         b->wave.generic = (int)value;
         if (value) {
-            b->wave.lds_k1n = std::max(b->wave.lds_k1n, k1n_lds_bytes(b->max_sites, max_ns));
+            b->wave.lds_k1n = std::max(b->wave.lds_k1n, k1n_lds_for(b->h_models, b->max_sites, max_ns));
             if (b->wave.lds_k1n > 65536)
                 (void)hipFuncSetAttribute((const void *)waven_diag_vlist_kernel,
                                           hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -71,6 +71,14 @@ struct DevModel {
     //   [dims0*dims1] (or -1), u8 leg states [nv][4]; offsets from the table start
     const uint8_t *k1tab;
     int32_t k1tab_bytes, pad_k1;
+    // lookups of the generic diagonal-update kernel, copied to LDS when they
+    // fit (any bond size): per type a 32-byte header {weights off, diagonal-
+    // vertex off, upper-states off, nv, dims packed one byte each}, then per
+    // bond {type, sites[nsb]} (i32 each, from k1n_bond_off), then per type f64
+    // weights[nv], i32 diagonal vertex codes [prod dims] (or -1) and u32 upper
+    // leg states [nv] (one byte per bond position); offsets from the start
+    const uint8_t *k1ntab;
+    int32_t k1ntab_bytes, k1n_bond_off;
     // chase table of the generic wave kernels (any bond size): one entry per
     // (ecode, worm-1, leg_in) at ((ecode*dt_mw + worm-1) << g_lg) + leg_in,
     // 2^g_lg = leg count rounded up to a power of two; exit words gen_out()

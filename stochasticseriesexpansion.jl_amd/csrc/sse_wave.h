@@ -2131,6 +2131,20 @@ __global__ void activate_pending_kernel(DevWalker *W, int n_walkers)
 // ------------------------------------------------------------------ host side
 #include "sse_wave_n.h"
 
+// LDS of the generic diagonal-update kernel: the base layout for the largest
+// site count and bond size, raised to stage a model's lookups in LDS where that
+// keeps it within 32 KiB (other models read them from global memory)
+static inline size_t k1n_lds_for(const std::vector<DevModel> &models, int max_sites, int max_ns)
+{
+    size_t b = k1n_lds_bytes(max_sites, max_ns);
+    for (const DevModel &m : models) {
+        const size_t st = k1n_lds_staged(m.n_sites, m.nsb, m.k1ntab_bytes);
+        if (st <= 32768 && st > b)
+            b = st;
+    }
+    return b;
+}
+
 static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &models, int max_sites)
 {
     c.max_sites = max_sites;
@@ -2142,7 +2156,7 @@ static inline int wave_configure(WaveConfig &c, const std::vector<DevModel> &mod
         if (m.nsb != 2)
             c.generic = 1;
     }
-    c.lds_k1n = k1n_lds_bytes(max_sites, max_ns);
+    c.lds_k1n = k1n_lds_for(models, max_sites, max_ns);
     if (c.generic && c.lds_k1n > 65536)
         (void)hipFuncSetAttribute((const void *)waven_diag_vlist_kernel,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_k1n);
@@ -2225,7 +2239,8 @@ static inline int wave_launch_init(WaveConfig &c, hipStream_t s, DevWalker *W, c
     for (int r = 0; r < max_warmup; ++r) {
         if (c.generic)
             hipLaunchKernelGGL(waven_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1n, s, W, M, pool, 0,
-                               n, 1, r, (const int32_t *)nullptr, 0, (const int32_t *)nullptr, 0);
+                               n, 1, r, (const int32_t *)nullptr, 0, (const int32_t *)nullptr, 0,
+                               (int)c.lds_k1n);
         else
             hipLaunchKernelGGL(wave_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1, s, W, M, pool, 0, n,
                                1, r, (int)c.lds_k1, (const int32_t *)nullptr, 0);
@@ -2247,7 +2262,7 @@ static inline int wave_launch_sweep(WaveConfig &c, hipStream_t s, DevWalker *W, 
     if (c.generic) {
         if (k1 & 3)
             hipLaunchKernelGGL(waven_diag_vlist_kernel, dim3(n), dim3(64), c.lds_k1n, s, W, M, pool, w0,
-                               w1, k1, -1, order, g, (const int32_t *)nullptr, 0);
+                               w1, k1, -1, order, g, (const int32_t *)nullptr, 0, (int)c.lds_k1n);
         if (ev)
             (void)hipEventRecord(ev[1], s);
         if (phases & 4)

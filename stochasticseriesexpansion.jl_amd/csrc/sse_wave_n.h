@@ -158,13 +158,65 @@ __device__ void k1n_serial_chunk(DevWalker &w, const DevModel &m, WaveStream &rs
     __syncthreads();
 }
 
+// Model lookups of K1n: from the LDS copy of DevModel::k1ntab (N1Staged) or
+// from the model's global arrays (N1Global).  Same values either way.
+template <int NS>
+struct N1Global {
+    const DevModel *m;
+    __device__ __forceinline__ void stage(char *) const {}
+    __device__ __forceinline__ int type(uint32_t b) const { return m->bond_type[b]; }
+    __device__ __forceinline__ int site(uint32_t b, int k) const { return m->bond_sites[(size_t)NS * b + k]; }
+    __device__ __forceinline__ int nv(int t) const { return m->types[t].nv; }
+    __device__ __forceinline__ int dim(int t, int j) const { return m->types[t].dims[j]; }
+    __device__ __forceinline__ int diag(int t, int i) const { return m->types[t].diag_vertex[i]; }
+    __device__ __forceinline__ double weight(int t, int v) const { return m->types[t].weights[v]; }
+    __device__ __forceinline__ uint32_t upper(int t, int v) const { return k1n_upper<NS>(m->types[t], (uint32_t)v); }
+};
+template <int NS>
+struct N1Staged {
+    const DevModel *m;
+    const char *t;  // the LDS copy of m->k1ntab
+    // copy the table into LDS (whole wave; the caller's barrier follows)
+    __device__ __forceinline__ void stage(char *dst) const
+    {
+        const uint32_t *g = (const uint32_t *)m->k1ntab;
+        uint32_t *d = (uint32_t *)dst;
+        for (int i = threadIdx.x; i < m->k1ntab_bytes / 4; i += 64)
+            d[i] = g[i];
+    }
+    __device__ __forceinline__ const uint32_t *hdr(int ty) const { return (const uint32_t *)(t + 32 * ty); }
+    __device__ __forceinline__ const int32_t *bond(uint32_t b) const
+    {
+        return (const int32_t *)(t + m->k1n_bond_off) + (size_t)(NS + 1) * b;
+    }
+    __device__ __forceinline__ int type(uint32_t b) const { return bond(b)[0]; }
+    __device__ __forceinline__ int site(uint32_t b, int k) const { return bond(b)[1 + k]; }
+    __device__ __forceinline__ int nv(int ty) const { return (int)hdr(ty)[3]; }
+    __device__ __forceinline__ int dim(int ty, int j) const { return (int)((hdr(ty)[4] >> (8 * j)) & 255u); }
+    __device__ __forceinline__ int diag(int ty, int i) const { return ((const int32_t *)(t + hdr(ty)[1]))[i]; }
+    __device__ __forceinline__ double weight(int ty, int v) const
+    {
+        return ((const double *)(t + hdr(ty)[0]))[v];
+    }
+    __device__ __forceinline__ uint32_t upper(int ty, int v) const
+    {
+        return ((const uint32_t *)(t + hdr(ty)[2]))[v];
+    }
+};
+
+// LDS bytes of K1n with the model's lookups staged
+__host__ __device__ inline size_t k1n_lds_staged(int N, int NS, int k1ntab_bytes)
+{
+    return k1n_lds_bytes(N, NS) + (((size_t)k1ntab_bytes + 15) & ~(size_t)15);
+}
+
 // K1n of walker wi: diagonal update (sse.jl:137-191) and vertex list
 // (vertex_list.jl:15-54), 64 slots per step.  phases: 1 diagonal update,
 // 2 vertex list, 8 force the slot-by-slot path; warm >= 0: init warm-up
 // round r (diagonal only, pending walkers with more warm-up sweeps).
-template <int NS>
+template <int NS, class Tab>
 __device__ void k1n_walker(DevWalker *W, const DevModel *models, DevPool *pool, int wi, int phases,
-                           int warm, char *lds, int grp)
+                           int warm, char *lds, int grp, Tab tab)
 {
     constexpr int LEGS = LegsOf<NS>::LEGS, PL = LegsOf<NS>::PL;
     const int lane = threadIdx.x;
@@ -187,6 +239,7 @@ __device__ void k1n_walker(DevWalker *W, const DevModel *models, DevPool *pool, 
     const bool do_vl = (phases & 2) && warm < 0;
     const bool force_serial = phases & 8;
 
+    tab.stage(lds + k1n_lds_bytes(N, NS));
     for (int i = lane; i < NS * N; i += 64)
         tm[i] = 0ull;
     for (int i = lane; i < N; i += 64) {
@@ -239,6 +292,9 @@ __device__ void k1n_walker(DevWalker *W, const DevModel *models, DevPool *pool, 
     gmem<uint32_t> *gbase = as_global(recn_base<PL>(w));
     gmem<uint32_t> *ops = as_global(w.ops);
 
+    // read once: a read of the model inside the loop would wait for the
+    // chunk's global stores
+    const uint32_t nbonds = (uint32_t)m.n_bonds;
     uint32_t op_next = (int64_t)lane < L ? ops[lane] : 0u;
     for (int64_t base = 0; base < L; base += 64) {
         const int64_t p = base + lane;
@@ -256,9 +312,9 @@ __device__ void k1n_walker(DevWalker *W, const DevModel *models, DevPool *pool, 
         {
             bool bad = false;
             if (isop) {
-                bad = b >= (uint32_t)m.n_bonds;
+                bad = b >= nbonds;
                 if (!bad)
-                    bad = (int)(vc >> 1) >= m.types[m.bond_type[b]].nv;
+                    bad = (int)(vc >> 1) >= tab.nv(tab.type(b));
             }
             if (__ballot(bad)) {
                 if (lane == 0) {
@@ -305,12 +361,11 @@ __device__ void k1n_walker(DevWalker *W, const DevModel *models, DevPool *pool, 
         for (int k = 0; k < NS; ++k)
             st[k] = 0;
         if (has) {
-            type = m.bond_type[b];
+            type = tab.type(b);
 #pragma unroll
             for (int k = 0; k < NS; ++k)
-                st[k] = m.bond_sites[(size_t)NS * b + k];
+                st[k] = tab.site(b, k);
         }
-        const DevVD &vd = m.types[type];
         // per-site lane masks, one per bond position
         if (has) {
 #pragma unroll
@@ -318,7 +373,7 @@ __device__ void k1n_walker(DevWalker *W, const DevModel *models, DevPool *pool, 
                 atomicOr(&tmw[2 * ((size_t)k * N + st[k]) + (lane >> 5)], 1u << (lane & 31));
         }
         if (offd)
-            cups[lane] = k1n_upper<NS>(vd, vc >> 1);
+            cups[lane] = tab.upper(type, (int)(vc >> 1));
         wave_lds_sync();
         // M[j][k]: lanes whose bond has the site of this lane's position j at
         // position k; A[j]: lanes whose bond touches that site at all
@@ -360,12 +415,12 @@ __device__ void k1n_walker(DevWalker *W, const DevModel *models, DevPool *pool, 
                         v = sp[st[j]];
                     }
                     idx += (v - 1) * stride;
-                    stride *= vd.dims[j];
+                    stride *= tab.dim(type, j);
                 }
-                dv = vd.diag_vertex[idx];
-                wgt = dv < 0 ? 0.0 : vd.weights[dv >> 1];
+                dv = tab.diag(type, idx);
+                wgt = dv < 0 ? 0.0 : tab.weight(type, dv >> 1);
             } else if (dia) {
-                wgt = vd.weights[vc >> 1];
+                wgt = tab.weight(type, (int)(vc >> 1));
             }
             bool amb = false, dec = false;
             if (ident) {
@@ -989,11 +1044,26 @@ __device__ void k3n_walker(DevWalker *W, const DevModel *models, double *out, do
 
 // budget non-null: sweep `step` of a budgeted call; walker wi takes part
 // only while step < budget[wi]
+// K1n of walker wi with the model's lookups in LDS when the launch's LDS
+// (lds_bytes) holds them
+template <int NS>
+__device__ __forceinline__ void k1n_walker_sel(DevWalker *W, const DevModel *models, DevPool *pool, int wi,
+                                               int phases, int warm, char *lds, int grp, int lds_bytes)
+{
+    const DevModel *m = &models[W[wi].model];
+    if (k1n_lds_staged(m->n_sites, NS, m->k1ntab_bytes) <= (size_t)lds_bytes)
+        k1n_walker<NS>(W, models, pool, wi, phases, warm, lds, grp,
+                       N1Staged<NS>{m, lds + k1n_lds_bytes(m->n_sites, NS)});
+    else
+        k1n_walker<NS>(W, models, pool, wi, phases, warm, lds, grp, N1Global<NS>{m});
+}
+
 __global__ __launch_bounds__(64) void waven_diag_vlist_kernel(DevWalker *W, const DevModel *models,
                                                               DevPool *pool, int w_begin, int w_end,
                                                               int phases, int warm,
                                                               const int32_t *order, int grp,
-                                                              const int32_t *budget, int step)
+                                                              const int32_t *budget, int step,
+                                                              int lds_bytes)
 {
     extern __shared__ __align__(16) char lds[];
     const int pos = w_begin + (int)blockIdx.x;
@@ -1003,7 +1073,7 @@ __global__ __launch_bounds__(64) void waven_diag_vlist_kernel(DevWalker *W, cons
     if (budget && step >= budget[wi])
         return;
     const int ns = models[W[wi].model].nsb;
-#define K1N_CALL(K) k1n_walker<K>(W, models, pool, wi, phases, warm, lds, grp)
+#define K1N_CALL(K) k1n_walker_sel<K>(W, models, pool, wi, phases, warm, lds, grp, lds_bytes)
     SSE_NS_DISPATCH(ns, K1N_CALL)
 #undef K1N_CALL
 }
