@@ -93,6 +93,28 @@ def test_parity_every_sweep(sse_pkg, oracle_mod, kind, mode):
     batch.close()
 
 
+def test_wave_lookups_from_global_memory(sse_pkg, oracle_mod):
+    """A model too large for the generic diagonal-update kernel to stage its
+    lookups in LDS (a 2048-site trimer ring: the kernel's own layout alone
+    exceeds the 32 KiB limit) reads them from global memory; bit-exact with
+    the oracle after every sweep, as the staged path is in the other tests."""
+    flat = trimer_ring(sse_pkg, 2048)[0]
+    walkers = [sse_pkg.walker_desc(T=1.0, seed=80 + k, thermalization=3) for k in range(2)]
+    _say("multisite wave, lookups from global memory: 2048-site trimer ring")
+    batch = sse_pkg.Batch([flat], walkers, mode=0)
+    oracles = [oracle_mod.OracleWalker(sse_pkg.capi, flat, w) for w in walkers]
+    batch.init()
+    for o in oracles:
+        o.init()
+    _compare(batch, oracles, "init")
+    for s in range(8):
+        batch.sweep(1)
+        for o in oracles:
+            o.sweep(1)
+        _compare(batch, oracles, f"sweep {s}")
+    batch.close()
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_vertex_list_legs(sse_pkg, oracle_mod, mode):
     """sse_get_vertex_list returns 2 NSites legs per slot, equal to the
