@@ -379,7 +379,8 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
  * SSE_OPT_TUNE (default 1): autotuner.  The first measuring sse_run calls of
  *                a wave-mode batch run their sweeps in turn (segments of four
  *                sweeps, twice round-robin) with the candidate launch forms: stream
- *                groups at 2, 4 and hardware-queue-count groups; the
+ *                groups at 2, 4, 8 (with more hardware queues than 8) and
+ *                hardware-queue-count groups; the
  *                persistent work-queue launch with the worm-kernel LDS sized
  *                from the largest walker or capped at 32 KiB; then the
  *                winner with the record-lookahead chase.  Each is scored by

@@ -1621,6 +1621,8 @@ static void tune_init(sse_batch *b, sse_batch::Tuner &t, bool budgets)
     const int n = b->n_walkers;
     if (!budgets) {
         std::vector<int> gs = {2, 4, hw_queues()};
+        if (hw_queues() > 8)
+            gs.push_back(8);
         std::sort(gs.begin(), gs.end());
         gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
         for (int g : gs)
