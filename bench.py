@@ -512,11 +512,35 @@ def single_walker(sse, thermalization=10000, sweeps=5000):
     return out
 
 
+def dispatch_concurrency(spans):
+    """Overlap of kernel dispatches given as (start, end) timestamps in ns: the
+    window from the first start to the last end, the time at least one runs
+    (busy), the time-weighted mean number running over the window and over the
+    busy time, and the largest number running at once."""
+    events = sorted([(t0, 1) for t0, _ in spans] + [(t1, -1) for _, t1 in spans],
+                    key=lambda e: (e[0], e[1]))
+    running = peak = 0
+    area = busy = 0
+    prev = events[0][0]
+    for t, d in events:
+        if running:
+            area += running * (t - prev)
+            busy += t - prev
+        prev = t
+        running += d
+        peak = max(peak, running)
+    window = events[-1][0] - events[0][0]
+    return {"dispatches": len(spans), "window_ms": window * 1e-6, "busy_ms": busy * 1e-6,
+            "mean_running": area / window if window else 0.0,
+            "mean_running_busy": area / busy if busy else 0.0, "max_running": peak}
+
+
 def summarize_rocprof(directory, last=0, regex=None):
     """Markdown tables from rocprofv3 CSV output under `directory`: per kernel
-    and launch grid, the kernel-trace durations (calls, mean/min/max ms) and,
-    for counter collections, each counter's mean per dispatch over the last
-    `last` dispatches of that kernel and grid (all when 0)."""
+    and launch grid, the kernel-trace durations (calls, mean/min/max ms), the
+    overlap of the last `last` matched dispatches of each trace, and, for
+    counter collections, each counter's mean per dispatch over the last `last`
+    dispatches of that kernel and grid (all when 0)."""
     import csv
     import glob
     import re
@@ -531,13 +555,15 @@ def summarize_rocprof(directory, last=0, regex=None):
 
     for path in sorted(glob.glob(os.path.join(directory, "**", "*kernel_trace.csv"), recursive=True)):
         groups = {}
+        spans = []
         for r in csv.DictReader(open(path)):
             name = (col(r, "Kernel_Name") or "").split("(")[0]
             if pat and not pat.search(name):
                 continue
             grid = col(r, "Grid_Size_X", "Grid_Size") or "?"
-            t = (int(col(r, "End_Timestamp")) - int(col(r, "Start_Timestamp"))) * 1e-6
-            groups.setdefault((name, grid), []).append(t)
+            t0, t1 = int(col(r, "Start_Timestamp")), int(col(r, "End_Timestamp"))
+            groups.setdefault((name, grid), []).append((t1 - t0) * 1e-6)
+            spans.append((t0, t1))
         lines.append(f"kernel trace {os.path.relpath(path, directory)}:")
         lines.append("| kernel | grid | calls | avg ms | min ms | max ms | avg ms, last |")
         lines.append("|---|---|---|---|---|---|---|")
@@ -545,6 +571,12 @@ def summarize_rocprof(directory, last=0, regex=None):
             tail = ts[-last:] if last else ts
             lines.append(f"| {name} | {grid} | {len(ts)} | {np.mean(ts):.4f} | {min(ts):.4f} | "
                          f"{max(ts):.4f} | {np.mean(tail):.4f} ({len(tail)}) |")
+        if spans:
+            c = dispatch_concurrency(sorted(spans)[-last:] if last else spans)
+            lines.append(f"concurrency over the last {c['dispatches']} dispatches: window "
+                         f"{c['window_ms']:.3f} ms, busy {c['busy_ms']:.3f} ms, "
+                         f"mean dispatches running {c['mean_running']:.2f} "
+                         f"({c['mean_running_busy']:.2f} while any runs), max {c['max_running']}")
     for path in sorted(glob.glob(os.path.join(directory, "**", "*counter_collection.csv"),
                                  recursive=True)):
         groups = {}
