@@ -252,6 +252,62 @@ def parallelism(choice, walkers, generic=False):
             f"{chase}")
 
 
+def kernel_region(sse, batch, flats, wds, steps, budgets=None):
+    """The timed calls once more with HIP events around every kernel launch on
+    its stream (SSE_OPT_TIMING 2, direct launches): per kernel the launches,
+    mean launch time and algorithmic bytes per launch, and the HBM roofline of
+    the kernel with the most time.  Bytes per walker-sweep, with P = 2 x sites
+    per bond link words per operator and N sites: diagonal update + vertex list
+    12L + 4Pn + 2N, worm update 12W + N, measurement 4L; the persistent launch
+    runs all three."""
+    batch.reset_accumulators()
+    batch.set_option(sse.capi.SSE_OPT_TIMING, 2)
+    ms = np.zeros(4)
+    cnt = np.zeros(4, dtype=np.int64)
+    calls = 1 if budgets is None else steps
+    for _ in range(calls):
+        if budgets is None:
+            batch.run(steps, measure=True)
+        else:
+            batch.run_budgets(budgets, measure=True)
+        m, c = batch.kernel_times()
+        ms += m
+        cnt += c
+    batch.set_option(sse.capi.SSE_OPT_TIMING, 1)
+    L, n, W, wsw = batch.work_counters()
+    sites = np.array([flats[w.model].n_sites for w in wds], dtype=np.float64)
+    sweeps = np.full(len(wds), steps) if budgets is None else np.asarray(budgets) * steps
+    Nsw = float(np.sum(sites * sweeps))
+    P = 2 * max(f.bond_n_sites for f in flats)
+    generic = P != 4
+    names = ("diag_vlist", "worm", "measure", "persistent")
+    kernels = dict(zip(names, ("waven_diag_vlist_kernel", "waven_worm_kernel",
+                               "waven_measure_kernel") if generic else
+                       ("wave_diag_vlist_kernel", "wave_worm_kernel", "wave_measure_kernel")))
+    kernels["persistent"] = "wave_persist_kernel"
+    region = {"diag_vlist": 12 * L + 4 * P * n + 2 * Nsw, "worm": 12 * W + Nsw, "measure": 4 * L}
+    region["persistent"] = sum(region.values())
+    per_kernel = {}
+    for i, nm in enumerate(names):
+        if cnt[i]:
+            avg = ms[i] / cnt[i]
+            bpl = region[nm] / cnt[i]
+            per_kernel[nm] = {"kernel": kernels[nm], "launches": int(cnt[i]), "avg_launch_ms": avg,
+                              "algorithmic_bytes_per_launch": bpl,
+                              "achieved_gbs_per_launch": bpl / (avg * 1e-3) / 1e9}
+    if not per_kernel:
+        return {}, None
+    dom = max(per_kernel, key=lambda nm: ms[names.index(nm)])
+    d = per_kernel[dom]
+    roof = {"bound": "hbm", "kernel": d["kernel"], "achieved": d["achieved_gbs_per_launch"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["achieved_gbs_per_launch"] / HBM_PEAK_GBS,
+            "traffic": None, "walker_sweeps": int(wsw),
+            "avg_launch_ms": d["avg_launch_ms"], "launches": d["launches"]}
+    log("kernels: " + ", ".join(f"{v['kernel']} {v['launches']} x {v['avg_launch_ms']:.3f} ms"
+                                for v in per_kernel.values()))
+    return per_kernel, roof
+
+
 def run_batch(sse, flats, wds, thermalization, steps, what, budgets=None):
     """Thermalize, tune, then time `steps` sweeps (equal budgets) or one
     budgeted call per step (budgets: per-walker sweeps per call)."""
@@ -284,9 +340,11 @@ def run_batch(sse, flats, wds, thermalization, steps, what, budgets=None):
             batch.run_budgets(budgets, measure=True)
     dt = time.perf_counter() - t1
     Ls, ns, Ws, wsw = batch.work_counters()
+    per_kernel, roof = kernel_region(sse, batch, flats, wds, steps, budgets)
     batch.close()
     generic = any(f.bond_n_sites != 2 for f in flats)
     return {"vertex_updates_per_s": (Ls + Ws) / dt, "sweeps_per_s": wsw / dt,
+            "kernels_timed_configuration": per_kernel, "roofline": roof,
             "walkers": len(wds), "thermalization_sweeps": thermalization,
             "timed": (f"{steps} sweeps of every walker" if budgets is None else
                       f"{steps} budgeted calls, {int(np.sum(budgets))} walker-sweeps each"),
@@ -393,6 +451,10 @@ def extra_trimer(sse, walkers=4096, thermalization=1000, steps=20):
     r["workload"] = ("three-site bonds (SSEBond{3}): 256-site spin-1/2 trimer ring, J=-1, h=0.15, "
                      "4096 walkers, beta=8, generic wave kernels")
     return r
+
+
+EXTRA_WORKLOADS = {"heterogeneous": extra_heterogeneous, "ladder": extra_ladder,
+                   "trimer": extra_trimer}
 
 
 def serving_stream(sse, seconds=20.0, count_for=60.0, sweep_scale=1.0, slots=1024):
@@ -639,9 +701,13 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=20.0,
                     help="arrival window of the serving stream (0 skips it); results are "
                          "counted for three times as long")
-    ap.add_argument("--extra", action="store_true",
-                    help="also run the heterogeneous, ladder and three-site-bond workloads")
+    ap.add_argument("--extra", nargs="?", const=",".join(EXTRA_WORKLOADS), default="",
+                    help="also run the heterogeneous, ladder and three-site-bond (trimer) "
+                         "workloads, or the comma-separated ones named")
     args = ap.parse_args()
+    unknown = [x for x in args.extra.split(",") if x and x not in EXTRA_WORKLOADS]
+    if unknown:
+        ap.error(f"--extra: unknown workload(s) {unknown}; choose from {list(EXTRA_WORKLOADS)}")
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); walkers are independent "
               f"chains, so N GPUs run N independent batches.", file=sys.stderr)
@@ -817,10 +883,10 @@ def main():
             out["serving"] = {"error": f"{type(e).__name__}: {e}"}
     if args.extra:
         extras = {}
-        for name, fn in (("heterogeneous", extra_heterogeneous), ("ladder", extra_ladder),
-                         ("trimer", extra_trimer)):
-            log(f"extra workload: {name}")
-            extras[name] = fn(sse)
+        for name in args.extra.split(","):
+            if name:
+                log(f"extra workload: {name}")
+                extras[name] = EXTRA_WORKLOADS[name](sse)
         out["extra_workloads"] = extras
     if args.calibrate_copy_mb:
         copy_calibration(args.calibrate_copy_mb)

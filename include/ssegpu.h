@@ -239,8 +239,9 @@ int sse_get_vertex_list(sse_batch *b, int32_t walker, int32_t *links,
  * (diagonal+vertex list, worm, measure), measured with HIP events on the
  * batch stream.                                                            */
 int sse_phase_times(const sse_batch *b, double *ms3);
-/* With SSE_OPT_TIMING 2: per kernel of the last sse_run/sse_sweep call the
- * summed launch duration in ms and the launch count, [0] diagonal update +
+/* With SSE_OPT_TIMING 2: per kernel of the last sse_run/sse_sweep/
+ * sse_run_budgets call the summed launch duration in ms and the launch count,
+ * [0] diagonal update +
  * vertex list, [1] worm update, [2] measurement, [3] persistent launch.
  * No reference counterpart (measurement).                                  */
 int sse_kernel_times(const sse_batch *b, double *ms4, int64_t *launches4);
@@ -332,11 +333,12 @@ int sse_pt_get_ladders(sse_batch *b, int32_t *slot_walkers, int64_t *accepted,
  * SSE_OPT_TIMING (default 1): every sweep/run call is bracketed by HIP events
  *                (sse_call_time); single-stream calls of <= 256 sweeps also
  *                record events around each kernel (sse_phase_times) and
- *                launch eagerly.  2: in addition, sse_run/sse_sweep record
- *                events around every kernel launch on the stream it runs on
- *                (each stream group's diagonal/vertex-list, worm and
- *                measurement kernels, or the persistent launch), with direct
- *                launches instead of graph replays (sse_kernel_times).
+ *                launch eagerly.  2: in addition, sse_run/sse_sweep/
+ *                sse_run_budgets record events around every kernel launch on
+ *                the stream it runs on (each stream group's diagonal/vertex-
+ *                list, worm and measurement kernels, or the persistent
+ *                launch), with direct launches instead of graph replays
+ *                (sse_kernel_times).
  * SSE_OPT_WORM_RECORDS (default 0): record layout of the worm kernel.  0 picks
  *                the smallest that fits (8-byte LDS records for n <= 4096 and
  *                <= 256 chase codes, 10-byte LDS records for n <= 16384, chase

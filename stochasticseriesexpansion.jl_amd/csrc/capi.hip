@@ -2049,13 +2049,31 @@ int sse_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure)
     if (rc)
         return rc;
     const auto t0 = std::chrono::steady_clock::now();
+    b->kern_ev.clear();
     if (!order.empty()) {
         HIPCHECK(bmemcpy(b, b->d_budget, bud.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
         HIPCHECK(bmemcpy(b, b->d_task_order, order.data(), sizeof(int32_t) * order.size(),
                          hipMemcpyHostToDevice));
+        // SSE_OPT_TIMING 2: events around the persistent launch (a direct
+        // launch, as for sse_run), summed by sse_kernel_times
+        hipEvent_t *e = nullptr;
+        const bool graph_k = b->use_graph;
+        if (b->kernel_timing) {
+            const size_t ev0 = b->kern_ev.size() * 4;
+            rc = ensure_kernel_events(b, ev0 + 4);
+            if (rc)
+                return rc;
+            e = &b->kevpool[ev0];
+            b->kern_ev.push_back({3, ev0});
+            b->use_graph = false;
+            HIPCHECK(hipEventRecord(e[0], b->stream));
+        }
         rc = persist_issue(b, 0, measure ? 1 : 0, b->d_budget, (int)order.size(), b->d_task_order);
+        b->use_graph = graph_k;
         if (rc)
             return rc;
+        if (e)
+            HIPCHECK(hipEventRecord(e[1], b->stream));
     }
     if (cand >= 0) {
         HIPCHECK(hipStreamSynchronize(b->stream));
@@ -2077,6 +2095,8 @@ int sse_run_budgets(sse_batch *b, const int64_t *sweeps, int32_t measure)
     if (rc)
         return rc;
     rc = check_walker_errors(b);
+    if (b->kernel_timing)
+        collect_kernel_times(b);
     if (!rc)
         rc = call_timer(b, 2, 0);
     clear_phase_times(b);
