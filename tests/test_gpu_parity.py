@@ -565,6 +565,36 @@ def test_run_budgets_against_oracle(sse_pkg, oracle_mod, records, mode):
     batch.close()
 
 
+def test_run_budgets_kernel_timing(sse_pkg):
+    """SSE_OPT_TIMING 2 on sse_run_budgets: each call that launches reports
+    one persistent launch with a positive duration through sse_kernel_times
+    (none for a call whose budgets are all 0), and the chains and
+    accumulators equal those of the same calls without per-kernel events."""
+    flat = square_heisenberg(sse_pkg, 6).flat()
+    wd = [sse_pkg.walker_desc(T=0.5, seed=1300 + s, thermalization=4) for s in range(40)]
+    _say("run_budgets kernel timing: 40 walkers, 3 calls, timing 2 vs 1")
+    a = sse_pkg.Batch([flat], wd)
+    b = sse_pkg.Batch([flat], wd)
+    a.set_option(sse_pkg.capi.SSE_OPT_TIMING, 2)
+    a.init()
+    b.init()
+    rng = np.random.default_rng(5)
+    for budgets in (rng.integers(1, 9, size=40), np.zeros(40, dtype=np.int64),
+                    rng.integers(0, 9, size=40)):
+        a.run_budgets(budgets, measure=True)
+        b.run_budgets(budgets, measure=True)
+        ms, n = a.kernel_times()
+        if budgets.any():
+            assert list(n) == [0, 0, 0, 1] and ms[3] > 0.0, (ms, n)
+        else:
+            assert list(n) == [0, 0, 0, 0], (ms, n)
+    _assert_same_walkers(a, b, len(wd), "timed budgets")
+    for (s1, q1, c1), (s2, q2, c2) in zip(a.accumulators(), b.accumulators()):
+        assert c1 == c2 and np.array_equal(s1, s2) and np.array_equal(q1, q2)
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("graph", [1, 2])
 def test_walker_groups_equal_single_stream(sse_pkg, oracle_mod, graph):
     """Walker groups on separate streams give the single-stream result bit for
