@@ -31,6 +31,10 @@ Reported beside it, in the same run:
     process's CPU share, on thermalized states copied from the GPU, for a
     bounded time.
 
+With --extra [NAME,...] it also runs the heterogeneous XXZ batch (per-walker
+budgets), the dim-4 ladder and the three-site-bond trimer ring, each with its
+own per-kernel times and roofline from HIP events around every launch.
+
 Vertex updates per walker-sweep U = L + W (W = scatter calls); algorithmic
 HBM bytes per walker-sweep B = 16L + 16n + 12W + 3N.
 """
