@@ -312,9 +312,10 @@ def kernel_region(sse, batch, flats, wds, steps, budgets=None):
     return per_kernel, roof
 
 
-def run_batch(sse, flats, wds, thermalization, steps, what, budgets=None):
+def run_batch(sse, flats, wds, thermalization, steps, what, budgets=None, refine=None):
     """Thermalize, tune, then time `steps` sweeps (equal budgets) or one
-    budgeted call per step (budgets: per-walker sweeps per call)."""
+    budgeted call per step (budgets: per-walker sweeps per call; refine, when
+    given, replaces them after tuning: refine(batch, choice) -> budgets)."""
     batch = sse.Batch(flats, wds)
     t0 = time.perf_counter()
     batch.init()
@@ -335,6 +336,9 @@ def run_batch(sse, flats, wds, thermalization, steps, what, budgets=None):
             if choice is not None:
                 break
         log(f"{what}: tuned {choice}")
+        if refine is not None:
+            budgets = refine(batch, choice)
+            batch.run_budgets(budgets, measure=True)
     batch.reset_accumulators()
     t1 = time.perf_counter()
     if budgets is None:
@@ -354,8 +358,8 @@ def run_batch(sse, flats, wds, thermalization, steps, what, budgets=None):
                       f"{steps} budgeted calls, {int(np.sum(budgets))} walker-sweeps each"),
             "mean_L": Ls / wsw, "mean_n": ns / wsw, "mean_W": Ws / wsw,
             "parallelism": parallelism(choice, len(wds), generic) if budgets is None else
-            (f"persistent work-queue launch with per-walker budgets (largest walker "
-             f"{int(np.min(budgets))} sweep(s) per call, smallest {int(np.max(budgets))}), tuned "
+            (f"persistent work-queue launch with per-walker budgets (costliest walker "
+             f"{int(np.min(budgets))} sweep(s) per call, cheapest {int(np.max(budgets))}), tuned "
              f"{choice}")}
 
 
@@ -386,16 +390,50 @@ def heterogeneous_models(sse, walkers=2048, seed=3):
     return flats, betas
 
 
+def sweep_budgets(cost, factor=4, cap=400):
+    """Per-walker sweeps per call of about `factor` sweeps of the costliest
+    walker's work, so that every walker's budget takes about the same time."""
+    cost = np.asarray(cost, dtype=np.float64)
+    return np.minimum(cap, np.maximum(1, np.round(factor * cost.max() / cost))).astype(np.int64)
+
+
+def operator_count_costs(batch, n_walkers, k2_lds, slow=3.0):
+    """Work per sweep of each walker from its chain: the operator count n,
+    times `slow` when the worm update cannot hold the walker's records in
+    LDS (8-byte records need n <= 4096 and 8n <= the worm LDS per walker;
+    beyond, links are read from L2 at about three times the time per step)."""
+    ns = np.array([batch.get_walker(k, with_ops=False, with_spins=False)[0]["num_operators"]
+                   for k in range(n_walkers)], dtype=np.float64)
+    in_lds = ns <= min(4096, k2_lds // 8)
+    return np.maximum(ns, 1.0) * np.where(in_lds, 1.0, slow), ns, in_lds
+
+
 def extra_heterogeneous(sse, walkers=2048, thermalization=300, steps=6, seed=3):
     """The heterogeneous XXZ batch, each walker advancing by its own budget per
-    call: about 4 sweeps of the largest walker's expected work (bonds x beta)
-    per call, so that every walker's budget takes about the same time."""
+    call: about 4 sweeps of the costliest walker's work per call.  The
+    autotuner measures on budgets from the expected work (bonds x beta); the
+    timed calls use budgets from each walker's operator count after
+    thermalization, weighted by the worm update's record layout."""
     flats, betas = heterogeneous_models(sse, walkers, seed)
     wds = [sse.walker_desc(model=k, T=1.0 / float(betas[k]), seed=100000 + k,
                            thermalization=thermalization) for k in range(walkers)]
-    cost = np.array([flats[k].n_bonds * float(betas[k]) for k in range(walkers)])
-    budgets = np.minimum(400, np.maximum(1, np.round(4 * cost.max() / cost))).astype(np.int64)
-    r = run_batch(sse, flats, wds, thermalization, steps, "heterogeneous", budgets=budgets)
+    budgets = sweep_budgets([flats[k].n_bonds * float(betas[k]) for k in range(walkers)])
+    info = {}
+
+    def refine(batch, choice):
+        k2 = int(choice["k2_lds"]) if choice and choice["k2_lds"] else int(batch.lds_bytes()[1])
+        cost, ns, in_lds = operator_count_costs(batch, walkers, k2)
+        info.update({"worm_lds_bytes": k2, "walkers_records_in_lds": int(in_lds.sum()),
+                     "n_quantiles_50_90_99_100": np.percentile(ns, [50, 90, 99, 100]).tolist()})
+        log(f"heterogeneous: budgets from operator counts, {int(in_lds.sum())} of {walkers} "
+            f"walkers with records in LDS ({k2} B)")
+        return sweep_budgets(cost)
+
+    r = run_batch(sse, flats, wds, thermalization, steps, "heterogeneous", budgets=budgets,
+                  refine=refine)
+    r["budgets"] = dict(info, rule="about 4 sweeps of the costliest walker's work per call, "
+                                   "cost = operator count x (3 if the records exceed the worm "
+                                   "LDS), at most 400 sweeps")
     r["workload"] = ("heterogeneous XXZ batch: 2048 walkers, L in {8,12,16,24,32}, "
                      "beta log-uniform [0.5,16], Jz ~ U[0.5,1.5], h=0.1")
     return r

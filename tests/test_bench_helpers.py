@@ -59,3 +59,30 @@ def test_summarize_trace_and_counters(tmp_path):
     out_all = bench.summarize_rocprof(str(d), last=0, regex="wave_")
     assert "| wave_worm_kernel | 32768 | SQ_WAVES | 2 | 768 (2) |" in out_all
     assert "window 4.000 ms, busy 4.000 ms, mean dispatches running 1.25" in out_all
+
+
+def test_sweep_budgets_known_answer():
+    # costliest walker 4 sweeps; a walker 8x cheaper 32; the cap holds at 400
+    b = bench.sweep_budgets([100.0, 12.5, 0.01, 50.0])
+    assert b.tolist() == [4, 32, 400, 8]
+    assert b.dtype.name == "int64"
+
+
+class _FakeBatch:
+    def __init__(self, ns):
+        self.ns = ns
+
+    def get_walker(self, k, with_ops=True, with_spins=True):
+        assert not with_ops and not with_spins
+        return {"num_operators": self.ns[k]}, None, None
+
+
+def test_operator_count_costs_layout_weight():
+    # 32 KiB of worm LDS holds 8-byte records up to n = 4096; beyond, x3
+    ns = [0, 1000, 4096, 4097, 20000]
+    cost, n, in_lds = bench.operator_count_costs(_FakeBatch(ns), len(ns), 32768)
+    assert in_lds.tolist() == [True, True, True, False, False]
+    assert cost.tolist() == [1.0, 1000.0, 4096.0, 3 * 4097.0, 60000.0]
+    # a smaller worm LDS moves the limit to 8n <= bytes
+    _, _, in_lds = bench.operator_count_costs(_FakeBatch(ns), len(ns), 16384)
+    assert in_lds.tolist() == [True, True, False, False, False]
